@@ -1,0 +1,167 @@
+/*
+ * flock_hip.h: C ABI of libflock_hip.so, the MI355X (gfx950) flock step.
+ *
+ * The reference has no library interface for this path: its CUDA step is a
+ * program (cuda/flock.cu:374-523) driven through a CLI and .flk files, with
+ * the pass sequence of cuda/flock.cu:452-470 (memset, grid_histogram,
+ * grid_scan_blocks/totals/add, grid_scatter, decide, integrate) and the
+ * argument lists spelled out in oxide/src/main.rs:226-534. This header exposes
+ * that pass-level API as plain C (opaque handles, pointers, sizes), so a
+ * Rust/cgo/ctypes caller binds it without any C++ or HIP type.
+ *
+ * Conventions
+ *   - Every call returns fh_status (0 = FH_OK). On failure fh_last_error()
+ *     returns a thread-local message describing the last error of this thread.
+ *   - One handle per host thread; calls on one handle are not thread-safe.
+ *   - State crosses the boundary as .flk records: FH_WORDS 4-byte words per
+ *     bird (pos xyz, vel xyz, roll, spin, radius, steering xyz,
+ *     next_decision, neighbours as u32 bits, centrality), as in
+ *     crates/flock-ref/src/state.rs:1-7. The caller owns host memory; the
+ *     library copies it.
+ *   - Device buffers are allocated once per handle (fh_create /
+ *     fh_batch_create) and freed by the matching destroy call.
+ *   - The library never falls back to a CPU path: without a usable gfx950
+ *     device every device call fails with FH_ERR_NODEVICE.
+ */
+#ifndef FLOCK_HIP_H
+#define FLOCK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FH_WORDS 15
+
+typedef int32_t fh_status;
+enum {
+    FH_OK = 0,
+    FH_ERR_INVALID = 1,  /* bad argument (n = 0 where not allowed, bad size, NULL) */
+    FH_ERR_FLK = 2,      /* malformed .flk: bad magic, words != 15, truncated */
+    FH_ERR_NOMEM = 3,    /* device or host allocation failed */
+    FH_ERR_HIP = 4,      /* a HIP runtime call or a kernel reported an error */
+    FH_ERR_BUDGET = 5,   /* admission refused: over the HBM budget or slot cap */
+    FH_ERR_STATE = 6,    /* call out of order (e.g. scatter before scan) */
+    FH_ERR_IO = 7,       /* file cannot be opened or written */
+    FH_ERR_NODEVICE = 8  /* no usable GPU */
+};
+
+/* Tunables of one flock (or one batch). Zero fields take the defaults. */
+typedef struct fh_params {
+    float cell;                /* grid cell edge [m]; 0 -> 4 (params.rs:58 GRID_CELL) */
+    uint32_t buckets;          /* hash buckets; 0 -> buckets_for(n) (cuda/flock.cu:368-372); else a power of two */
+    uint32_t decide_block;     /* decide workgroup size, multiple of 64 in [64, 1024]; 0 -> 256 */
+    uint32_t birds_per_thread; /* decide scans block * this sorted slots per workgroup; 0 -> 8 */
+    uint32_t flags;            /* FH_FLAG_* */
+    uint32_t reserved[3];
+} fh_params;
+
+/* Within a bucket, order birds by ascending index (a stable counting sort):
+ * the step becomes bit-reproducible and independent of what else is batched. */
+#define FH_FLAG_CANONICAL 1u
+/* Run the histogram as its own pass instead of fusing it into the previous
+ * step's integrate (the reference's pass layout, for per-pass timing). */
+#define FH_FLAG_UNFUSED 2u
+/* Replay each step from a captured hipGraph instead of launching kernels. */
+#define FH_FLAG_GRAPH 4u
+/* Wave-aggregated histogram atomics: lanes of a wave that share a bucket
+ * issue one atomic (pays off when many birds share a cell). */
+#define FH_FLAG_WAVE_AGG 8u
+
+/* Per-pass GPU time of fh_step, mean ms per timed step (hipEvents on the
+ * handle's stream), in the rows of the reference report
+ * (cuda/flock.cu:486-491). With the fused layout the histogram runs inside
+ * the previous step's integrate and grid_histogram reads 0. */
+typedef struct fh_pass_ms {
+    double grid_histogram;
+    double grid_scan;
+    double grid_scatter;
+    double decide;
+    double integrate;
+    double step_total;   /* sum of the rows above */
+    double wall_ms;      /* host wall clock over the timed steps, synchronised */
+    uint32_t timed_steps;
+    uint32_t due_last;   /* birds that decided in the last step */
+} fh_pass_ms;
+
+typedef struct fh_flock fh_flock;
+
+/* ---- library ------------------------------------------------------------ */
+const char *fh_version(void);
+const char *fh_last_error(void);
+/* Writes the model constants in the order of crates/flock-ref/src/params.rs:6-58
+ * (DT .. GRID_CELL) plus RADIUS_LERP/3; returns how many exist. Host only. */
+int fh_constants(float *out, int cap);
+/* buckets_for, cuda/flock.cu:368-372. Host only. */
+uint32_t fh_buckets_for(uint32_t n);
+fh_status fh_params_default(fh_params *p);
+/* Device name, compute units and HBM bytes of device 0. */
+fh_status fh_device_info(char *name, int name_cap, int *compute_units, size_t *hbm_bytes);
+/* Resources of the kernel named `kernel` ("decide", "integrate", "scatter",
+ * "hash_count", "scan"): VGPRs per lane, static LDS bytes, scratch bytes. */
+fh_status fh_kernel_info(const char *kernel, int *vgprs, int *lds_bytes, int *scratch_bytes);
+
+/* ---- .flk files and generators (host only; state.rs:58-170) ------------- */
+/* Replaces Flock::read (state.rs:135-170): words == NULL reads only the header. */
+fh_status fh_read_flk(const char *path, uint32_t *n, float *time, float *words, uint32_t cap);
+/* Replaces Flock::write (state.rs:104-132). */
+fh_status fh_write_flk(const char *path, uint32_t n, float time, const float *words);
+/* Flock::scatter (state.rs:79-101): the reference benchmark start (density
+ * 0.5, +x heading, SplitMix64). */
+fh_status fh_gen_reference(uint32_t n, uint64_t seed, float *words);
+/* Uniform cube of edge `side` [m] centred on (0, 0, 100); directions uniform
+ * on the sphere, speed 10 + U(-1, 1) m/s; radius 1.5; next ~ U[0, DU). */
+fh_status fh_gen_cube(uint32_t n, float side, uint64_t seed, float *words);
+/* `clumps` isotropic Gaussian clumps of std `sigma` [m] whose centres are
+ * uniform in a cube of edge `cube` [m] about (0, 0, 100); each clump has a
+ * heading uniform on the sphere, members fly it rotated by up to 10 degrees
+ * (uniform in the cap), at 10 m/s. */
+fh_status fh_gen_clumps(uint32_t n, uint32_t clumps, float sigma, float cube, uint64_t seed,
+                        float *words);
+
+/* ---- one flock: the reference pass API ---------------------------------- */
+/* Allocates the device state and grid for n birds (cuda/flock.cu:403-412). */
+fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out);
+fh_status fh_destroy(fh_flock *h);
+fh_status fh_get_params(const fh_flock *h, fh_params *out, uint32_t *buckets);
+/* Host -> device state (cuda/flock.cu:414-437). n must equal the handle's n. */
+fh_status fh_upload_words(fh_flock *h, const float *words, uint32_t n, float time);
+/* Device -> host state (cuda/flock.cu:494-521); *time gets the flock clock. */
+fh_status fh_download_words(fh_flock *h, float *words, float *time);
+/* grid_histogram (cuda/flock.cu:118-124), including the counts memset (:457). */
+fh_status fh_hash_count(fh_flock *h);
+/* grid_scan_blocks/totals/add (cuda/flock.cu:144-172): offsets[B+1]. */
+fh_status fh_scan(fh_flock *h);
+/* grid_scatter (cuda/flock.cu:175-179) into the sorted position/velocity
+ * buffers that decide walks. */
+fh_status fh_scatter(fh_flock *h);
+/* decide (cuda/flock.cu:190-290) for birds with time >= next_decision. */
+fh_status fh_decide(fh_flock *h, float time);
+/* integrate (cuda/flock.cu:293-328). */
+fh_status fh_integrate(fh_flock *h);
+/* integrate fused with the next step's grid_histogram. */
+fh_status fh_integrate_hash(fh_flock *h);
+/* `steps` whole steps in the order of cuda/flock.cu:452-470 (time += DT in
+ * f32 first). The first `warmup` steps are not timed. out may be NULL. */
+fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out);
+/* Waits for the handle's stream and reports any kernel-side error. */
+fh_status fh_sync(fh_flock *h);
+fh_status fh_get_time(const fh_flock *h, float *time);
+
+/* ---- debug read-back for parity ----------------------------------------- */
+/* Grid of the last build (any pointer may be NULL): keys[n], counts[B],
+ * offsets[B+1], sorted[n] (bird indices in slot order). After fh_scan the
+ * counts are offsets[b+1] - offsets[b]. */
+fh_status fh_read_grid(fh_flock *h, uint32_t *keys, uint32_t *counts, uint32_t *offsets,
+                       uint32_t *sorted);
+/* Neighbours of `bird` found by the device grid walk: walk 1 (d^2 <= r^2) or
+ * walk 2 (d^2 <= 4 r^2), ascending bird index. *count may exceed cap. */
+fh_status fh_neighbours(fh_flock *h, uint32_t bird, int walk, uint32_t *ids, uint32_t cap,
+                        uint32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,573 @@
+// One flock behind the reference pass API: the handle owns one device arena,
+// one HIP stream (default priority) and the host copy of the flock clock.
+// Pass order and clock follow cuda/flock.cu:452-470.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "host_util.h"
+#include "layout.h"
+#include "model.h"
+
+namespace fh {
+hipError_t grid_kernel_attributes(const char *name, hipFuncAttributes *out);
+hipError_t decide_kernel_attributes(const char *name, hipFuncAttributes *out);
+hipError_t integrate_kernel_attributes(const char *name, hipFuncAttributes *out);
+hipError_t launch_fill(uint32_t *dst, uint32_t count, uint32_t value, hipStream_t s);
+}  // namespace fh
+
+using namespace fh;
+
+// Grid state of the current positions.
+enum class Stage { None, Hashed, Scanned, Scattered, Decided };
+
+struct fh_flock {
+    uint32_t n = 0, buckets = 0, padded_buckets = 0;
+    fh_params prm{};
+    float time = 0.0f;
+    hipStream_t stream = nullptr;
+    void *arena = nullptr;
+    Pools P{};
+    Seg seg{};
+    Seg *d_seg = nullptr;
+    Geometry geo{};
+    Stage stage = Stage::None;
+    bool counts_dirty = false;  // histogram filled and not yet consumed by a scan
+    bool absolute_due = true;   // snext holds next_decision (not next - time)
+    uint32_t *d_nbr = nullptr;  // neighbour read-back buffer
+    uint32_t nbr_cap = 0;
+    hipGraphExec_t graph = nullptr;
+};
+
+static fh_status check_handle(const fh_flock *h, const char *fn) {
+    if (!h) return set_error(FH_ERR_INVALID, "%s: NULL handle", fn);
+    return FH_OK;
+}
+
+static fh_status read_error_word(fh_flock *h) {
+    uint32_t err = 0;
+    FH_HIP(hipMemcpyAsync(&err, h->P.err, 4, hipMemcpyDeviceToHost, h->stream));
+    FH_HIP(hipStreamSynchronize(h->stream));
+    if (err & ERR_SLOT) return set_error(FH_ERR_HIP, "kernel reported a slot outside the sorted range (err 0x%x)", err);
+    if (err & ERR_NBCAP) return set_error(FH_ERR_HIP, "kernel reported error 0x%x", err);
+    return FH_OK;
+}
+
+// ---- passes on the handle's stream -----------------------------------------
+
+static fh_status do_hash(fh_flock *h) {
+    if (h->counts_dirty) FH_HIP(launch_fill(h->P.counts, h->padded_buckets, 0u, h->stream));
+    FH_HIP(launch_hash_count(h->geo, h->P, (h->prm.flags & FH_FLAG_WAVE_AGG) != 0, h->stream));
+    h->counts_dirty = true;
+    h->stage = Stage::Hashed;
+    return FH_OK;
+}
+
+static fh_status do_scan(fh_flock *h) {
+    FH_HIP(launch_scan(h->geo, h->P, h->stream));
+    h->counts_dirty = false;
+    h->stage = Stage::Scanned;
+    return FH_OK;
+}
+
+static fh_status do_scatter(fh_flock *h, bool relative) {
+    FH_HIP(launch_scatter(h->geo, h->P, (h->prm.flags & FH_FLAG_CANONICAL) != 0, relative, h->stream));
+    h->absolute_due = !relative;
+    h->stage = Stage::Scattered;
+    return FH_OK;
+}
+
+static fh_status do_decide(fh_flock *h, float time_arg) {
+    FH_HIP(launch_decide(h->geo, h->P, time_arg, h->prm.decide_block, h->prm.birds_per_thread, h->stream));
+    h->stage = Stage::Decided;
+    return FH_OK;
+}
+
+static fh_status do_integrate(fh_flock *h, bool hash) {
+    if (hash && h->counts_dirty) FH_HIP(launch_fill(h->P.counts, h->padded_buckets, 0u, h->stream));
+    FH_HIP(launch_integrate(h->geo, h->P, hash, (h->prm.flags & FH_FLAG_WAVE_AGG) != 0, h->stream));
+    if (hash) {
+        h->counts_dirty = true;
+        h->stage = Stage::Hashed;
+    } else {
+        h->stage = Stage::None;
+    }
+    return FH_OK;
+}
+
+// One step after time += DT, in the relative-due form (scatter stores
+// next - time, decide tests <= 0) so the same launches replay in a graph.
+static fh_status step_launches(fh_flock *h, bool fused, hipEvent_t *ev) {
+    FH_HIP(launch_set_time(h->d_seg, 1, 0.0f, DT, h->stream));
+    if (ev) FH_HIP(hipEventRecord(ev[0], h->stream));
+    if (!fused || h->stage != Stage::Hashed) {
+        fh_status st = do_hash(h);
+        if (st) return st;
+    }
+    if (ev) FH_HIP(hipEventRecord(ev[1], h->stream));
+    fh_status st = do_scan(h);
+    if (st) return st;
+    if (ev) FH_HIP(hipEventRecord(ev[2], h->stream));
+    if ((st = do_scatter(h, true))) return st;
+    if (ev) FH_HIP(hipEventRecord(ev[3], h->stream));
+    if ((st = do_decide(h, 0.0f))) return st;
+    if (ev) FH_HIP(hipEventRecord(ev[4], h->stream));
+    if ((st = do_integrate(h, fused))) return st;
+    if (ev) FH_HIP(hipEventRecord(ev[5], h->stream));
+    return FH_OK;
+}
+
+static fh_status sync_clock(fh_flock *h) {
+    // The device clock must equal the host clock before a relative-due step.
+    FH_HIP(launch_set_time(h->d_seg, 1, h->time, 0.0f, h->stream));
+    return FH_OK;
+}
+
+static fh_status capture_graph(fh_flock *h) {
+    if (h->graph) return FH_OK;
+    // The graph body is the fused step from the Hashed stage.
+    hipGraph_t g = nullptr;
+    FH_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const Stage stage0 = h->stage;
+    const bool dirty0 = h->counts_dirty;
+    fh_status st = step_launches(h, true, nullptr);
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    h->stage = stage0;
+    h->counts_dirty = dirty0;
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        h->graph = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    return FH_OK;
+}
+
+extern "C" {
+
+fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
+    clear_error();
+    if (!out) return set_error(FH_ERR_INVALID, "fh_create: NULL out");
+    *out = nullptr;
+    fh_params prm;
+    fh_params_default(&prm);
+    if (params) {
+        if (params->cell != 0.0f) prm.cell = params->cell;
+        prm.buckets = params->buckets;
+        if (params->decide_block) prm.decide_block = params->decide_block;
+        if (params->birds_per_thread) prm.birds_per_thread = params->birds_per_thread;
+        prm.flags = params->flags;
+    }
+    if (!(prm.cell >= 0.5f) || !(prm.cell <= 1000.0f))
+        return set_error(FH_ERR_INVALID, "fh_create: cell %g outside [0.5, 1000] m", prm.cell);
+    if (prm.buckets && (prm.buckets & (prm.buckets - 1)))
+        return set_error(FH_ERR_INVALID, "fh_create: buckets %u is not a power of two", prm.buckets);
+    if (prm.buckets > (1u << 28)) return set_error(FH_ERR_INVALID, "fh_create: buckets %u > 2^28", prm.buckets);
+    const uint32_t blk = prm.decide_block;
+    if (blk != 64 && blk != 128 && blk != 256 && blk != 512 && blk != 1024)
+        return set_error(FH_ERR_INVALID, "fh_create: decide_block %u not in {64,128,256,512,1024}", blk);
+    if (prm.birds_per_thread > 64 || blk * prm.birds_per_thread > 16384)
+        return set_error(FH_ERR_INVALID, "fh_create: decide chunk %u x %u too large", blk, prm.birds_per_thread);
+    if (n > (1u << 30)) return set_error(FH_ERR_INVALID, "fh_create: n = %u too large", n);
+    fh_status st = require_device();
+    if (st) return st;
+
+    fh_flock *h = new fh_flock();
+    h->n = n;
+    h->prm = prm;
+    h->buckets = prm.buckets ? prm.buckets : buckets_for(n);
+    const uint32_t ntiles = (h->buckets + SCAN_TILE - 1) / SCAN_TILE;
+    h->padded_buckets = ntiles * SCAN_TILE;
+
+    Arena a;
+    const size_t nn = n ? n : 1;
+    size_t o_pos = a.take<float4>(nn), o_vel = a.take<float4>(nn), o_steer = a.take<float4>(nn);
+    size_t o_spos = a.take<float4>(nn), o_svel = a.take<float4>(nn);
+    size_t o_spin = a.take<float>(nn), o_cen = a.take<float>(nn), o_nb = a.take<uint32_t>(nn);
+    size_t o_key = a.take<uint32_t>(nn), o_rank = a.take<uint32_t>(nn), o_snext = a.take<float>(nn);
+    size_t o_tmp = a.take<uint32_t>(nn);
+    size_t o_counts = a.take<uint32_t>(h->padded_buckets);
+    size_t o_offsets = a.take<uint32_t>((size_t)h->padded_buckets + 1);
+    size_t o_tsum = a.take<uint32_t>(ntiles), o_tpre = a.take<uint32_t>(ntiles);
+    size_t o_err = a.take<uint32_t>(4), o_due = a.take<uint32_t>(4), o_seg = a.take<Seg>(1);
+    hipError_t e = hipMalloc(&h->arena, a.used);
+    if (e != hipSuccess) {
+        delete h;
+        return hip_fail(e, "hipMalloc(flock arena)");
+    }
+    char *base = static_cast<char *>(h->arena);
+    h->P.pos4 = reinterpret_cast<float4 *>(base + o_pos);
+    h->P.vel4 = reinterpret_cast<float4 *>(base + o_vel);
+    h->P.steer4 = reinterpret_cast<float4 *>(base + o_steer);
+    h->P.spos = reinterpret_cast<float4 *>(base + o_spos);
+    h->P.svel = reinterpret_cast<float4 *>(base + o_svel);
+    h->P.spin = reinterpret_cast<float *>(base + o_spin);
+    h->P.cen = reinterpret_cast<float *>(base + o_cen);
+    h->P.nb = reinterpret_cast<uint32_t *>(base + o_nb);
+    h->P.key = reinterpret_cast<uint32_t *>(base + o_key);
+    h->P.rank = reinterpret_cast<uint32_t *>(base + o_rank);
+    h->P.snext = reinterpret_cast<float *>(base + o_snext);
+    h->P.sorted_tmp = reinterpret_cast<uint32_t *>(base + o_tmp);
+    h->P.counts = reinterpret_cast<uint32_t *>(base + o_counts);
+    h->P.offsets = reinterpret_cast<uint32_t *>(base + o_offsets);
+    h->P.tile_sums = reinterpret_cast<uint32_t *>(base + o_tsum);
+    h->P.tile_prefix = reinterpret_cast<uint32_t *>(base + o_tpre);
+    h->P.err = reinterpret_cast<uint32_t *>(base + o_err);
+    h->P.due_count = reinterpret_cast<uint32_t *>(base + o_due);
+    h->d_seg = reinterpret_cast<Seg *>(base + o_seg);
+
+    h->seg.bird_base = 0;
+    h->seg.n = n;
+    h->seg.bucket_base = 0;
+    h->seg.mask = h->buckets - 1;
+    h->seg.cell = prm.cell;
+    h->seg.inv_cell = 1.0f / prm.cell;
+    h->seg.time = 0.0f;
+    h->seg.live = 1;
+    h->geo.segs = h->d_seg;
+    h->geo.tile_seg = nullptr;
+    h->geo.n_birds = n;
+    h->geo.n_slots = n;
+    h->geo.n_buckets = h->buckets;
+
+    auto fail = [&](hipError_t err, const char *what) {
+        fh_status s2 = hip_fail(err, what);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+        (void)hipFree(h->arena);
+        delete h;
+        return s2;
+    };
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail(e, "hipStreamCreateWithFlags");
+    // Zero everything once so every index array starts in range.
+    if ((e = hipMemsetAsync(h->arena, 0, a.used, h->stream)) != hipSuccess) return fail(e, "hipMemsetAsync");
+    if ((e = hipMemcpyAsync(h->d_seg, &h->seg, sizeof(Seg), hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+        return fail(e, "hipMemcpyAsync(seg)");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return fail(e, "hipStreamSynchronize");
+    *out = h;
+    return FH_OK;
+}
+
+fh_status fh_destroy(fh_flock *h) {
+    clear_error();
+    if (!h) return FH_OK;
+    (void)hipStreamSynchronize(h->stream);
+    if (h->graph) (void)hipGraphExecDestroy(h->graph);
+    if (h->d_nbr) (void)hipFree(h->d_nbr);
+    (void)hipFree(h->arena);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FH_OK;
+}
+
+fh_status fh_get_params(const fh_flock *h, fh_params *out, uint32_t *buckets) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_get_params");
+    if (st) return st;
+    if (out) *out = h->prm;
+    if (buckets) *buckets = h->buckets;
+    return FH_OK;
+}
+
+fh_status fh_upload_words(fh_flock *h, const float *words, uint32_t n, float time) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_upload_words");
+    if (st) return st;
+    if (n != h->n) return set_error(FH_ERR_INVALID, "fh_upload_words: n = %u but the flock holds %u", n, h->n);
+    if (n && !words) return set_error(FH_ERR_INVALID, "fh_upload_words: NULL words");
+    std::vector<float4> pos(n), vel(n), steer(n);
+    std::vector<float> spin(n), cen(n);
+    std::vector<uint32_t> nb(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float *w = words + (size_t)i * FH_WORDS;
+        pos[i] = make_float4(w[PX], w[PY], w[PZ], w[NEXT]);
+        vel[i] = make_float4(w[VX], w[VY], w[VZ], w[ROLL]);
+        steer[i] = make_float4(w[SX], w[SY], w[SZ], w[RADIUS]);
+        spin[i] = w[SPIN];
+        std::memcpy(&nb[i], &w[NB], 4);
+        cen[i] = w[CEN];
+    }
+    if (n) {
+        FH_HIP(hipMemcpyAsync(h->P.pos4, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+        FH_HIP(hipMemcpyAsync(h->P.vel4, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+        FH_HIP(hipMemcpyAsync(h->P.steer4, steer.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+        FH_HIP(hipMemcpyAsync(h->P.spin, spin.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        FH_HIP(hipMemcpyAsync(h->P.cen, cen.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        FH_HIP(hipMemcpyAsync(h->P.nb, nb.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+    }
+    h->time = time;
+    st = sync_clock(h);
+    if (st) return st;
+    FH_HIP(hipStreamSynchronize(h->stream));
+    h->stage = Stage::None;
+    return FH_OK;
+}
+
+fh_status fh_download_words(fh_flock *h, float *words, float *time) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_download_words");
+    if (st) return st;
+    if (h->n && !words) return set_error(FH_ERR_INVALID, "fh_download_words: NULL words");
+    const uint32_t n = h->n;
+    std::vector<float4> pos(n), vel(n), steer(n);
+    std::vector<float> spin(n), cen(n);
+    std::vector<uint32_t> nb(n);
+    if (n) {
+        FH_HIP(hipMemcpyAsync(pos.data(), h->P.pos4, n * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+        FH_HIP(hipMemcpyAsync(vel.data(), h->P.vel4, n * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+        FH_HIP(hipMemcpyAsync(steer.data(), h->P.steer4, n * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+        FH_HIP(hipMemcpyAsync(spin.data(), h->P.spin, n * 4, hipMemcpyDeviceToHost, h->stream));
+        FH_HIP(hipMemcpyAsync(cen.data(), h->P.cen, n * 4, hipMemcpyDeviceToHost, h->stream));
+        FH_HIP(hipMemcpyAsync(nb.data(), h->P.nb, n * 4, hipMemcpyDeviceToHost, h->stream));
+    }
+    FH_HIP(hipStreamSynchronize(h->stream));
+    for (uint32_t i = 0; i < n; ++i) {
+        float *w = words + (size_t)i * FH_WORDS;
+        w[PX] = pos[i].x; w[PY] = pos[i].y; w[PZ] = pos[i].z;
+        w[VX] = vel[i].x; w[VY] = vel[i].y; w[VZ] = vel[i].z;
+        w[ROLL] = vel[i].w;
+        w[SPIN] = spin[i];
+        w[RADIUS] = steer[i].w;
+        w[SX] = steer[i].x; w[SY] = steer[i].y; w[SZ] = steer[i].z;
+        w[NEXT] = pos[i].w;
+        std::memcpy(&w[NB], &nb[i], 4);
+        w[CEN] = cen[i];
+    }
+    if (time) *time = h->time;
+    return read_error_word(h);
+}
+
+fh_status fh_hash_count(fh_flock *h) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_hash_count");
+    return st ? st : do_hash(h);
+}
+
+fh_status fh_scan(fh_flock *h) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_scan");
+    if (st) return st;
+    if (h->stage != Stage::Hashed) return set_error(FH_ERR_STATE, "fh_scan: no histogram of the current positions");
+    return do_scan(h);
+}
+
+fh_status fh_scatter(fh_flock *h) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_scatter");
+    if (st) return st;
+    if (h->stage != Stage::Scanned) return set_error(FH_ERR_STATE, "fh_scatter: scan the histogram first");
+    return do_scatter(h, false);
+}
+
+fh_status fh_decide(fh_flock *h, float time) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_decide");
+    if (st) return st;
+    if (h->stage != Stage::Scattered || !h->absolute_due)
+        return set_error(FH_ERR_STATE, "fh_decide: build the grid (hash, scan, scatter) first");
+    h->time = time;
+    if ((st = sync_clock(h))) return st;
+    return do_decide(h, time);
+}
+
+fh_status fh_integrate(fh_flock *h) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_integrate");
+    return st ? st : do_integrate(h, false);
+}
+
+fh_status fh_integrate_hash(fh_flock *h) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_integrate_hash");
+    return st ? st : do_integrate(h, true);
+}
+
+fh_status fh_sync(fh_flock *h) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_sync");
+    if (st) return st;
+    FH_HIP(hipStreamSynchronize(h->stream));
+    FH_HIP(hipGetLastError());
+    return read_error_word(h);
+}
+
+fh_status fh_get_time(const fh_flock *h, float *time) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_get_time");
+    if (st) return st;
+    if (!time) return set_error(FH_ERR_INVALID, "fh_get_time: NULL");
+    *time = h->time;
+    return FH_OK;
+}
+
+fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_step");
+    if (st) return st;
+    if (warmup > steps) warmup = steps;  // cuda/flock.cu:389
+    const bool fused = !(h->prm.flags & FH_FLAG_UNFUSED);
+    const bool graph = fused && (h->prm.flags & FH_FLAG_GRAPH);
+    if ((st = sync_clock(h))) return st;
+    const uint32_t timed = steps - warmup;
+    std::vector<hipEvent_t> ev;
+    const bool per_pass = out && !graph;
+    if (out) {
+        ev.resize(per_pass ? (size_t)timed * 6 : 2);
+        for (auto &x : ev) FH_HIP(hipEventCreate(&x));
+    }
+    auto destroy_events = [&] {
+        for (auto &x : ev) (void)hipEventDestroy(x);
+    };
+    std::chrono::steady_clock::time_point t0{}, t1{};
+    for (uint32_t s = 0; s < steps; ++s) {
+        if (s == warmup) {
+            FH_HIP(hipStreamSynchronize(h->stream));
+            t0 = std::chrono::steady_clock::now();
+            if (out && graph) FH_HIP(hipEventRecord(ev[0], h->stream));
+        }
+        h->time += DT;  // host f32 clock (cuda/flock.cu:455); the device adds DT the same way
+        if (graph) {
+            if (h->stage != Stage::Hashed) {
+                if ((st = do_hash(h))) break;
+            }
+            if ((st = capture_graph(h))) break;
+            hipError_t e = hipGraphLaunch(h->graph, h->stream);
+            if (e != hipSuccess) {
+                st = hip_fail(e, "hipGraphLaunch");
+                break;
+            }
+            h->stage = Stage::Hashed;
+            h->counts_dirty = true;
+            h->absolute_due = false;
+        } else {
+            hipEvent_t *e6 = (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 6] : nullptr;
+            if ((st = step_launches(h, fused, e6))) break;
+        }
+    }
+    if (st) {
+        (void)hipStreamSynchronize(h->stream);
+        destroy_events();
+        return st;
+    }
+    if (out && graph && timed) FH_HIP(hipEventRecord(ev[1], h->stream));
+    FH_HIP(hipStreamSynchronize(h->stream));
+    t1 = std::chrono::steady_clock::now();
+    if (out) {
+        std::memset(out, 0, sizeof *out);
+        out->timed_steps = timed;
+        if (timed) {
+            out->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            if (per_pass) {
+                double acc[5] = {0, 0, 0, 0, 0};
+                for (uint32_t s = 0; s < timed; ++s)
+                    for (int k = 0; k < 5; ++k) {
+                        float ms = 0.0f;
+                        FH_HIP(hipEventElapsedTime(&ms, ev[(size_t)s * 6 + k], ev[(size_t)s * 6 + k + 1]));
+                        acc[k] += ms;
+                    }
+                out->grid_histogram = acc[0] / timed;
+                out->grid_scan = acc[1] / timed;
+                out->grid_scatter = acc[2] / timed;
+                out->decide = acc[3] / timed;
+                out->integrate = acc[4] / timed;
+                out->step_total = (acc[0] + acc[1] + acc[2] + acc[3] + acc[4]) / timed;
+            } else {
+                float ms = 0.0f;
+                FH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+                out->step_total = ms / timed;
+            }
+        }
+    }
+    destroy_events();
+    if (out) {
+        uint32_t due = 0;
+        FH_HIP(hipMemcpy(&due, h->P.due_count, 4, hipMemcpyDeviceToHost));
+        out->due_last = due;
+    }
+    return read_error_word(h);
+}
+
+fh_status fh_read_grid(fh_flock *h, uint32_t *keys, uint32_t *counts, uint32_t *offsets, uint32_t *sorted) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_read_grid");
+    if (st) return st;
+    if (h->stage == Stage::None) return set_error(FH_ERR_STATE, "fh_read_grid: no grid built");
+    const uint32_t n = h->n, B = h->buckets;
+    FH_HIP(hipStreamSynchronize(h->stream));
+    if (keys && n) FH_HIP(hipMemcpy(keys, h->P.key, (size_t)n * 4, hipMemcpyDeviceToHost));
+    const bool have_offsets = h->stage != Stage::Hashed;
+    std::vector<uint32_t> off;
+    if (offsets || (counts && have_offsets)) {
+        if (!have_offsets) return set_error(FH_ERR_STATE, "fh_read_grid: offsets need fh_scan first");
+        off.resize((size_t)B + 1);
+        FH_HIP(hipMemcpy(off.data(), h->P.offsets, ((size_t)B + 1) * 4, hipMemcpyDeviceToHost));
+        if (offsets) std::memcpy(offsets, off.data(), ((size_t)B + 1) * 4);
+    }
+    if (counts) {
+        if (have_offsets) {
+            for (uint32_t b = 0; b < B; ++b) counts[b] = off[b + 1] - off[b];
+        } else {
+            FH_HIP(hipMemcpy(counts, h->P.counts, (size_t)B * 4, hipMemcpyDeviceToHost));
+        }
+    }
+    if (sorted) {
+        if (h->stage != Stage::Scattered && h->stage != Stage::Decided)
+            return set_error(FH_ERR_STATE, "fh_read_grid: sorted needs fh_scatter first");
+        std::vector<float4> sp(n);
+        if (n) FH_HIP(hipMemcpy(sp.data(), h->P.spos, (size_t)n * sizeof(float4), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) std::memcpy(&sorted[i], &sp[i].w, 4);
+    }
+    return read_error_word(h);
+}
+
+fh_status fh_neighbours(fh_flock *h, uint32_t bird, int walk, uint32_t *ids, uint32_t cap, uint32_t *count) {
+    clear_error();
+    fh_status st = check_handle(h, "fh_neighbours");
+    if (st) return st;
+    if (bird >= h->n || (walk != 1 && walk != 2) || !count || (cap && !ids))
+        return set_error(FH_ERR_INVALID, "fh_neighbours: bad bird/walk/buffer");
+    if (h->stage != Stage::Scattered) return set_error(FH_ERR_STATE, "fh_neighbours: build the grid first");
+    if (cap + 1 > h->nbr_cap) {
+        if (h->d_nbr) FH_HIP(hipFree(h->d_nbr));
+        h->d_nbr = nullptr;
+        h->nbr_cap = 0;
+        FH_HIP(hipMalloc(&h->d_nbr, ((size_t)cap + 1) * 4));
+        h->nbr_cap = cap + 1;
+    }
+    FH_HIP(launch_neighbours(h->geo, h->P, bird, walk, h->d_nbr + 1, cap, h->d_nbr, h->stream));
+    FH_HIP(hipStreamSynchronize(h->stream));
+    uint32_t c = 0;
+    FH_HIP(hipMemcpy(&c, h->d_nbr, 4, hipMemcpyDeviceToHost));
+    *count = c;
+    const uint32_t k = c < cap ? c : cap;
+    if (k) FH_HIP(hipMemcpy(ids, h->d_nbr + 1, (size_t)k * 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> v(ids, ids + k);
+    std::sort(v.begin(), v.end());
+    std::memcpy(ids, v.data(), (size_t)k * 4);
+    return FH_OK;
+}
+
+fh_status fh_kernel_info(const char *kernel, int *vgprs, int *lds_bytes, int *scratch_bytes) {
+    clear_error();
+    if (!kernel) return set_error(FH_ERR_INVALID, "fh_kernel_info: NULL name");
+    fh_status st = require_device();
+    if (st) return st;
+    hipFuncAttributes at;
+    hipError_t e = grid_kernel_attributes(kernel, &at);
+    if (e == hipErrorInvalidValue) e = decide_kernel_attributes(kernel, &at);
+    if (e == hipErrorInvalidValue) e = integrate_kernel_attributes(kernel, &at);
+    if (e == hipErrorInvalidValue) return set_error(FH_ERR_INVALID, "fh_kernel_info: unknown kernel %s", kernel);
+    if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes");
+    if (vgprs) *vgprs = at.numRegs;
+    if (lds_bytes) *lds_bytes = (int)at.sharedSizeBytes;
+    if (scratch_bytes) *scratch_bytes = (int)at.localSizeBytes;
+    return FH_OK;
+}
+
+}  // extern "C"

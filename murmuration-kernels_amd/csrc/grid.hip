@@ -1,0 +1,302 @@
+// Spatial-hash counting sort for gfx950: cell keys + histogram, a three-kernel
+// reduce/scan/apply exclusive scan built on a wave64 DPP scan, and the scatter
+// into the sorted position/velocity buffers that decide walks.
+//
+// Reference passes: grid_histogram (cuda/flock.cu:118-124), grid_scan_blocks /
+// totals / add (:144-172), grid_scatter (:175-179). Keys, counts and offsets
+// are the reference's exactly; the scatter keeps the reference's arrival order
+// within a bucket unless the canonical pass reorders each bucket by bird index.
+#include "layout.h"
+#include "model.h"
+
+namespace fh {
+
+// ---- wave64 / workgroup scans --------------------------------------------
+
+// Inclusive prefix sum across the 64 lanes of a wave with DPP row shifts and
+// row broadcasts (rows of 16 lanes). All 64 lanes must be active.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return v;
+}
+
+// Exclusive prefix of v over a workgroup of NWAVES waves; *total = sum.
+template <int NWAVES>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
+    __shared__ uint32_t wave_tot[NWAVES];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t incl = wave_incl_scan(v);
+    if (lane == 63u) wave_tot[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVES; ++w) {
+        uint32_t t = wave_tot[w];
+        before += (uint32_t)w < wave ? t : 0u;
+        all += t;
+    }
+    __syncthreads();  // wave_tot may be reused by the caller's next scan
+    *total = all;
+    return before + incl - v;
+}
+
+__device__ __forceinline__ const Seg &tile_segment(const Geometry &g, uint32_t tile, bool *skip) {
+    uint32_t si = g.tile_seg ? g.tile_seg[tile] : 0u;
+    *skip = si == NO_SEG;
+    return g.segs[*skip ? 0u : si];
+}
+
+__device__ __forceinline__ uint32_t key_of(float x, float y, float z, const Seg &sg) {
+    return sg.bucket_base +
+           (bucket_hash(cell_coord(x, sg.inv_cell), cell_coord(y, sg.inv_cell), cell_coord(z, sg.inv_cell)) &
+            sg.mask);
+}
+
+// counts[key]++ returning the bird's arrival rank in its bucket. With
+// WAVE_AGG the lanes that share a key add once per wave (leader lane) and
+// take consecutive ranks; otherwise every lane issues its own atomic.
+template <bool WAVE_AGG>
+__device__ __forceinline__ uint32_t count_key(uint32_t *counts, uint32_t k, bool active) {
+    if (!WAVE_AGG) return active ? atomicAdd(&counts[k], 1u) : 0u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+    uint64_t left = __ballot(active);
+    uint32_t rank = 0;
+    while (left) {
+        const int leader = __ffsll((long long)left) - 1;
+        const uint32_t lk = __shfl(k, leader);
+        const uint64_t grp = __ballot(active && k == lk) & left;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&counts[lk], (uint32_t)__popcll(grp));
+        base = __shfl(base, leader);
+        if ((grp >> lane) & 1ull) rank = base + (uint32_t)__popcll(grp & below);
+        left &= ~grp;
+    }
+    return rank;
+}
+
+// grid_histogram: key = bucket of the bird's cell; rank = arrival order.
+template <bool WAVE_AGG>
+__global__ __launch_bounds__(TILE) void k_hash_count(Geometry g, Pools p) {
+    bool skip;
+    const Seg &sg = tile_segment(g, blockIdx.x, &skip);
+    if (skip) return;
+    const uint32_t bird = blockIdx.x * TILE + threadIdx.x;
+    const bool active = bird < g.n_birds && bird - sg.bird_base < sg.n;
+    uint32_t k = 0;
+    if (active) {
+        const float4 q = p.pos4[bird];
+        k = key_of(q.x, q.y, q.z, sg);
+        p.key[bird] = k;
+    }
+    const uint32_t r = count_key<WAVE_AGG>(p.counts, k, active);
+    if (active) p.rank[bird] = r;
+}
+
+// Scan, level 1: the sum of each SCAN_TILE-bucket tile.
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t *__restrict__ counts,
+                                                     uint32_t *__restrict__ tile_sums) {
+    const uint4 *c4 = reinterpret_cast<const uint4 *>(counts + (size_t)blockIdx.x * SCAN_TILE);
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint4 v = c4[k * 256 + threadIdx.x];
+        sum += v.x + v.y + v.z + v.w;
+    }
+    uint32_t total;
+    block_excl_scan<4>(sum, &total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// Scan, level 2: one workgroup scans the tile sums; offsets[end] = grand total.
+__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *__restrict__ tile_sums,
+                                                     uint32_t *__restrict__ tile_prefix, uint32_t ntiles,
+                                                     uint32_t *__restrict__ offsets_end,
+                                                     uint32_t *__restrict__ due_count) {
+    if (threadIdx.x == 0) *due_count = 0;  // a new grid starts a new step
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += 4096u) {
+        uint32_t v[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t t = base + threadIdx.x * 4u + (uint32_t)k;
+            v[k] = t < ntiles ? tile_sums[t] : 0u;
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t run = carry + block_excl_scan<16>(sum, &total);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t t = base + threadIdx.x * 4u + (uint32_t)k;
+            if (t < ntiles) tile_prefix[t] = run;
+            run += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) *offsets_end = carry;
+}
+
+// Scan, level 3: offsets of one tile = its prefix + local exclusive scan.
+// The counts are zeroed for the next histogram as they are consumed.
+__global__ __launch_bounds__(256) void k_scan_apply(uint32_t *__restrict__ counts,
+                                                    const uint32_t *__restrict__ tile_prefix,
+                                                    uint32_t *__restrict__ offsets) {
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * 16u;
+    uint4 *c4 = reinterpret_cast<uint4 *>(counts + base);
+    uint32_t c[16];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint4 v = c4[k];
+        c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w;
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) sum += c[k];
+    uint32_t total;
+    uint32_t run = tile_prefix[blockIdx.x] + block_excl_scan<4>(sum, &total);
+    uint32_t o[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        o[k] = run;
+        run += c[k];
+    }
+    uint4 *o4 = reinterpret_cast<uint4 *>(offsets + base);  // base is a multiple of 16
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o4[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c4[k] = z;
+}
+
+// grid_scatter: slot = offsets[key] + rank. Writes the sorted copies directly,
+// or (canonical mode) only the slot -> bird map for the reordering pass.
+__global__ __launch_bounds__(TILE) void k_scatter(Geometry g, Pools p, int canonical, int relative) {
+    bool skip;
+    const Seg &sg = tile_segment(g, blockIdx.x, &skip);
+    if (skip) return;
+    const uint32_t bird = blockIdx.x * TILE + threadIdx.x;
+    if (bird >= g.n_birds || bird - sg.bird_base >= sg.n) return;
+    const uint32_t s = p.offsets[p.key[bird]] + p.rank[bird];
+    if (s >= g.n_slots) {
+        atomicOr(p.err, ERR_SLOT);
+        return;
+    }
+    if (canonical) {
+        p.sorted_tmp[s] = bird;
+        return;
+    }
+    const float4 q = p.pos4[bird];
+    const float4 v = p.vel4[bird];
+    p.spos[s] = make_float4(q.x, q.y, q.z, __uint_as_float(bird));
+    p.svel[s] = make_float4(v.x, v.y, v.z, 0.0f);
+    p.snext[s] = relative ? q.w - sg.time : q.w;
+}
+
+// Canonical order: each bird moves to bucket start + number of birds of its
+// bucket with a smaller index, so a bucket lists its birds in ascending order
+// (the order of a stable counting sort over bird index). Buckets larger than
+// CANON_CAP keep the arrival order and raise ERR_CANON.
+__global__ __launch_bounds__(256) void k_canonical(Geometry g, Pools p, int relative) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= g.n_slots) return;
+    const uint32_t bird = p.sorted_tmp[s];
+    if (bird >= g.n_birds) {
+        atomicOr(p.err, ERR_SLOT);
+        return;
+    }
+    const uint32_t k = p.key[bird];
+    const uint32_t o0 = p.offsets[k], o1 = min(p.offsets[k + 1], g.n_slots);
+    uint32_t c = s;
+    if (o1 - o0 <= CANON_CAP && o0 <= s && s < o1) {
+        c = o0;
+        for (uint32_t t = o0; t < o1; ++t) c += p.sorted_tmp[t] < bird;
+    } else if (s == o0) {
+        atomicOr(p.err, ERR_CANON);
+    }
+    const float4 q = p.pos4[bird];
+    const float4 v = p.vel4[bird];
+    float t = 0.0f;
+    if (relative) t = g.segs[g.tile_seg ? g.tile_seg[bird / TILE] : 0u].time;
+    p.spos[c] = make_float4(q.x, q.y, q.z, __uint_as_float(bird));
+    p.svel[c] = make_float4(v.x, v.y, v.z, 0.0f);
+    p.snext[c] = relative ? q.w - t : q.w;
+}
+
+// Fill kernel (used instead of hipMemsetAsync so a captured graph holds only
+// kernel nodes).
+__global__ __launch_bounds__(256) void k_fill(uint32_t *dst, uint32_t count, uint32_t value) {
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) dst[i] = value;
+}
+
+hipError_t launch_fill(uint32_t *dst, uint32_t count, uint32_t value, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    const uint32_t blocks = min((count + 255u) / 256u, 8192u);
+    hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, s, dst, count, value);
+    return hipGetLastError();
+}
+
+__global__ void k_set_time(Seg *segs, uint32_t nseg, float time, float dt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg) return;
+    if (dt != 0.0f) {
+        if (segs[i].live) segs[i].time = segs[i].time + dt;
+    } else {
+        segs[i].time = time;
+    }
+}
+
+static inline uint32_t tiles_of(uint32_t n) { return (n + TILE - 1) / TILE; }
+
+hipError_t launch_hash_count(const Geometry &g, const Pools &p, bool wave_agg, hipStream_t s) {
+    if (g.n_birds == 0) return hipSuccess;
+    if (wave_agg)
+        hipLaunchKernelGGL(k_hash_count<true>, dim3(tiles_of(g.n_birds)), dim3(TILE), 0, s, g, p);
+    else
+        hipLaunchKernelGGL(k_hash_count<false>, dim3(tiles_of(g.n_birds)), dim3(TILE), 0, s, g, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const Geometry &g, const Pools &p, hipStream_t s) {
+    const uint32_t ntiles = (g.n_buckets + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_sums);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, p.tile_sums, p.tile_prefix, ntiles,
+                       p.offsets + (size_t)ntiles * SCAN_TILE, p.due_count);
+    hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_prefix, p.offsets);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, bool relative_time,
+                          hipStream_t s) {
+    if (g.n_birds == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter, dim3(tiles_of(g.n_birds)), dim3(TILE), 0, s, g, p, canonical ? 1 : 0,
+                       relative_time ? 1 : 0);
+    if (canonical && g.n_slots)
+        hipLaunchKernelGGL(k_canonical, dim3((g.n_slots + 255u) / 256u), dim3(256), 0, s, g, p,
+                           relative_time ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_time(Seg *segs, uint32_t nseg, float time, float dt, hipStream_t s) {
+    if (nseg == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_set_time, dim3((nseg + 255u) / 256u), dim3(256), 0, s, segs, nseg, time, dt);
+    return hipGetLastError();
+}
+
+hipError_t grid_kernel_attributes(const char *name, hipFuncAttributes *out) {
+    if (!__builtin_strcmp(name, "hash_count"))
+        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_hash_count<false>));
+    if (!__builtin_strcmp(name, "scan"))
+        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_scan_apply));
+    if (!__builtin_strcmp(name, "scatter"))
+        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_scatter));
+    if (!__builtin_strcmp(name, "canonical"))
+        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_canonical));
+    return hipErrorInvalidValue;
+}
+
+}  // namespace fh

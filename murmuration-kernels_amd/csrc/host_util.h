@@ -1,0 +1,45 @@
+// Host helpers shared by the library's C ABI translation units: the
+// thread-local error message, HIP error checks and the .flk record layout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/flock_hip.h"
+
+namespace fh {
+
+fh_status set_error(fh_status st, const char *fmt, ...);
+void clear_error();
+
+// Records a HIP failure (with the call site) and returns FH_ERR_HIP.
+fh_status hip_fail(hipError_t e, const char *what);
+
+#define FH_HIP(call)                                                  \
+    do {                                                              \
+        hipError_t e_ = (call);                                       \
+        if (e_ != hipSuccess) return ::fh::hip_fail(e_, #call);       \
+    } while (0)
+
+// Checks that device 0 is a usable gfx950 GPU (once per process).
+fh_status require_device();
+
+// .flk record words (crates/flock-ref/src/state.rs:4-7).
+enum Word { PX = 0, PY, PZ, VX, VY, VZ, ROLL, SPIN, RADIUS, SX, SY, SZ, NEXT, NB, CEN };
+
+// Carves one device allocation into 256-byte aligned sub-buffers.
+struct Arena {
+    size_t used = 0;
+    template <class T>
+    size_t take(size_t count) {
+        size_t at = used;
+        used += (count * sizeof(T) + 255) & ~size_t(255);
+        return at;
+    }
+};
+
+}  // namespace fh

@@ -1,0 +1,108 @@
+// integrate for gfx950: banked flight for every bird (crates/flock-ref/src/
+// model.rs:149-176, cuda/flock.cu:293-328), optionally fused with the next
+// step's grid_histogram so the new position is hashed while it is still in
+// registers instead of being read back by a separate pass.
+#include "layout.h"
+#include "model.h"
+
+namespace fh {
+
+template <bool WAVE_AGG>
+__device__ __forceinline__ uint32_t count_key_i(uint32_t *counts, uint32_t k, bool active) {
+    if (!WAVE_AGG) return active ? atomicAdd(&counts[k], 1u) : 0u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+    uint64_t left = __ballot(active);
+    uint32_t rank = 0;
+    while (left) {
+        const int leader = __ffsll((long long)left) - 1;
+        const uint32_t lk = __shfl(k, leader);
+        const uint64_t grp = __ballot(active && k == lk) & left;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&counts[lk], (uint32_t)__popcll(grp));
+        base = __shfl(base, leader);
+        if ((grp >> lane) & 1ull) rank = base + (uint32_t)__popcll(grp & below);
+        left &= ~grp;
+    }
+    return rank;
+}
+
+template <bool HASH, bool WAVE_AGG>
+__global__ __launch_bounds__(TILE) void k_integrate(Geometry g, Pools p) {
+    const uint32_t si = g.tile_seg ? g.tile_seg[blockIdx.x] : 0u;
+    if (si == NO_SEG) return;
+    const Seg &sg = g.segs[si];
+    const uint32_t bird = blockIdx.x * TILE + threadIdx.x;
+    const bool active = bird < g.n_birds && bird - sg.bird_base < sg.n;
+    uint32_t k = 0;
+    if (active) {
+        const float4 p4 = p.pos4[bird];
+        const float4 v4 = p.vel4[bird];
+        const float4 s4 = p.steer4[bird];
+        f3 vel = mk3(v4.x, v4.y, v4.z);
+        const float roll = v4.w;
+        const f3 steering = mk3(s4.x, s4.y, s4.z);
+
+        const f3 fwd = forward_of(vel);
+        const f3 side = side_of(fwd);
+        const float speed = fmaxf(sqrtf(dot3(vel, vel)), 1e-6f);
+
+        const float a_lat = dot3(steering, side) / MASS;
+        const float target = clampf_(atanf(a_lat / G), -MAX_BANK, MAX_BANK);
+        const float rate = clampf_((target - roll) / TAU_ROLL, -MAX_ROLL_RATE, MAX_ROLL_RATE);
+        const float beta = clampf_(roll + rate * DT, -MAX_BANK, MAX_BANK);
+
+        // body_up (model.rs:34-38): (side x f) cos b + side sin b
+        const f3 up0 = cross3(side, fwd);
+        const float cb = cosf(beta);
+        const f3 up = up0 * cb + side * sinf(beta);
+        const float cos_beta = fmaxf(cb, 0.2f);
+        const float sr = speed / V0;
+        const float lift = sr * sr * L0 / cos_beta;
+        const float drag = sr * sr * D0;
+        const f3 flight = ((up * lift - fwd * drag) + fwd * D0) - (mk3(0.f, 0.f, 1.f) * MASS) * G;
+
+        const f3 control = steering - side * dot3(steering, side);
+        vel = vel + ((control + flight) / MASS) * DT;
+        const float sp = sqrtf(dot3(vel, vel));
+        if (sp > V_MAX)
+            vel = vel * (V_MAX / sp);
+        else if (sp < V_MIN && sp > 1e-6f)
+            vel = vel * (V_MIN / sp);
+        const f3 pos = mk3(p4.x, p4.y, p4.z) + vel * DT;
+
+        p.pos4[bird] = make_float4(pos.x, pos.y, pos.z, p4.w);
+        p.vel4[bird] = make_float4(vel.x, vel.y, vel.z, beta);
+        p.spin[bird] = rate;
+        if (HASH) {
+            k = sg.bucket_base + (bucket_hash(cell_coord(pos.x, sg.inv_cell), cell_coord(pos.y, sg.inv_cell),
+                                              cell_coord(pos.z, sg.inv_cell)) &
+                                  sg.mask);
+            p.key[bird] = k;
+        }
+    }
+    if (HASH) {
+        const uint32_t r = count_key_i<WAVE_AGG>(p.counts, k, active);
+        if (active) p.rank[bird] = r;
+    }
+}
+
+hipError_t launch_integrate(const Geometry &g, const Pools &p, bool hash, bool wave_agg, hipStream_t s) {
+    if (g.n_birds == 0) return hipSuccess;
+    const dim3 grid((g.n_birds + TILE - 1) / TILE), block(TILE);
+    if (!hash)
+        hipLaunchKernelGGL((k_integrate<false, false>), grid, block, 0, s, g, p);
+    else if (wave_agg)
+        hipLaunchKernelGGL((k_integrate<true, true>), grid, block, 0, s, g, p);
+    else
+        hipLaunchKernelGGL((k_integrate<true, false>), grid, block, 0, s, g, p);
+    return hipGetLastError();
+}
+
+hipError_t integrate_kernel_attributes(const char *name, hipFuncAttributes *out) {
+    if (!__builtin_strcmp(name, "integrate"))
+        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_integrate<true, false>));
+    return hipErrorInvalidValue;
+}
+
+}  // namespace fh

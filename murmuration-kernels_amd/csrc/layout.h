@@ -1,0 +1,92 @@
+// Device data layout of the flock step and the launchers of its kernels.
+//
+// Bird state lives in bird-index order as three float4 vector fields plus
+// three scalar fields (FP32 structure of arrays; the w lanes carry the scalar
+// each pass reads next to that vector):
+//   pos4   = (px, py, pz, next_decision)   integrate: read+write; decide: w += DU
+//   vel4   = (vx, vy, vz, roll)            integrate: read+write
+//   steer4 = (sx, sy, sz, radius)          decide: write (due birds); integrate: read
+//   spin, neighbours, centrality           integrate / decide write only
+// The grid is the reference's counting sort (cuda/flock.cu:105-111):
+//   key[bird]        bucket of each bird (grid_histogram)
+//   rank[bird]       arrival order of the bird inside its bucket
+//   counts[bucket]   histogram, zeroed again by the scan
+//   offsets[bucket]  exclusive scan, buckets + 1 entries
+// and the scatter writes the sorted copies decide walks, in slot order:
+//   spos[slot] = (px, py, pz, bird index bits), svel[slot] = (vx, vy, vz, 0),
+//   snext[slot] = next_decision (or next_decision - time, see Seg::time).
+//
+// Several flocks can share one set of pools ("segments"): flock f owns birds
+// [bird_base, bird_base + n) and buckets [bucket_base, bucket_base + mask + 1).
+// Bird ranges start on a multiple of TILE birds so each 256-lane workgroup of
+// a bird-order pass belongs to exactly one flock (tile_seg[] maps tile ->
+// flock). A single flock is segment 0 with tile_seg == nullptr.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fh {
+
+constexpr uint32_t TILE = 256;        // birds per workgroup of the bird-order passes
+constexpr uint32_t SCAN_TILE = 4096;  // buckets per workgroup of the scan (256 lanes x 16)
+constexpr uint32_t CANON_CAP = 1024;  // largest bucket the canonical pass reorders
+constexpr uint32_t NO_SEG = 0xffffffffu;
+
+// Error bits a kernel may raise in the device error word.
+constexpr uint32_t ERR_SLOT = 1u;     // a scatter slot fell outside [0, n_slots)
+constexpr uint32_t ERR_CANON = 2u;    // a bucket exceeded CANON_CAP: left in arrival order
+constexpr uint32_t ERR_NBCAP = 4u;    // neighbour list longer than the caller's buffer
+
+struct Seg {
+    uint32_t bird_base;    // first bird of the flock in the pools
+    uint32_t n;            // birds
+    uint32_t bucket_base;  // first bucket of the flock
+    uint32_t mask;         // buckets - 1 (a power of two minus one)
+    float cell;            // grid cell edge [m]
+    float inv_cell;        // 1 / cell
+    float time;            // clock of the step in flight
+    uint32_t live;         // 0 for a freed segment
+};
+
+struct Pools {
+    float4 *pos4, *vel4, *steer4;
+    float *spin, *cen;
+    uint32_t *nb;
+    uint32_t *key, *rank;
+    uint32_t *counts, *offsets, *tile_sums, *tile_prefix;
+    float4 *spos, *svel;
+    float *snext;
+    uint32_t *sorted_tmp;  // canonical mode: slot -> bird before reordering
+    uint32_t *err;         // device error word (ERR_* bits)
+    uint32_t *due_count;   // birds that decided in the current step (zeroed by the scan)
+};
+
+// Launch geometry of one build/step over the pools.
+struct Geometry {
+    const Seg *segs;           // device segment table
+    const uint32_t *tile_seg;  // device tile -> segment map, nullptr for one flock
+    uint32_t n_birds;          // bird-index range covered (sum of padded ranges)
+    uint32_t n_slots;          // birds actually sorted (sum of n)
+    uint32_t n_buckets;        // total buckets over all segments
+};
+
+// Launchers (stream-ordered, no host synchronisation). Each returns the
+// hipError_t of its launch.
+hipError_t launch_hash_count(const Geometry &g, const Pools &p, bool wave_agg, hipStream_t s);
+hipError_t launch_scan(const Geometry &g, const Pools &p, hipStream_t s);
+hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, bool relative_time,
+                          hipStream_t s);
+hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
+                         uint32_t birds_per_thread, hipStream_t s);
+hipError_t launch_integrate(const Geometry &g, const Pools &p, bool hash, bool wave_agg,
+                            hipStream_t s);
+hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, int walk,
+                             uint32_t *ids, uint32_t cap, uint32_t *count, hipStream_t s);
+hipError_t launch_set_time(Seg *segs, uint32_t nseg, float time, float dt, hipStream_t s);
+
+// Resource usage of the kernels, for reports (name: "decide", "integrate",
+// "scatter", "hash_count", "scan").
+hipError_t kernel_attributes(const char *name, hipFuncAttributes *out);
+
+}  // namespace fh

@@ -1,0 +1,273 @@
+"""GPU parity: the gfx950 step against the oracle, through the C ABI.
+
+Bars (stated per test):
+  - keys, counts, offsets: bit-exact on identical positions;
+  - sorted order: per-bucket set equality (arrival order) or bit-exact
+    (canonical order = the oracle's stable counting sort);
+  - neighbour sets: equal to brute force except ties within 1e-5 of the radius;
+  - one step: positions/velocities within 1e-4 relative, neighbour counts equal;
+  - the reference ladder (scripts/parity.sh): 1/10/30 steps pointwise 1e-3,
+    100 steps with at most 2 outliers below 1e-2, 1000 steps statdiff.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+DT = np.float32(0.005)
+
+
+def cube(fh, n, seed=1):
+    return fh.gen_cube(n, fh.cube_side_for(n), seed)
+
+
+def build_grid(f):
+    f.hash_count()
+    f.scan()
+    f.scatter()
+    return f.read_grid()
+
+
+def rel_err(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)))) if a.size else 0.0
+
+
+def per_bucket_sets_equal(g_sorted, offsets, o_sorted):
+    for b in np.nonzero(np.diff(offsets.astype(np.int64)) > 1)[0]:
+        s, e = offsets[b], offsets[b + 1]
+        if set(g_sorted[s:e].tolist()) != set(o_sorted[s:e].tolist()):
+            return False
+    single = np.diff(offsets.astype(np.int64)) == 1
+    return np.array_equal(g_sorted[offsets[:-1][single]], o_sorted[offsets[:-1][single]])
+
+
+@pytest.mark.parametrize("n,kind", [(2000, "ref"), (10_000, "cube"), (100_000, "cube"),
+                                    (1_000_000, "cube"), (1, "cube"), (5000, "ref")])
+def test_grid_bit_exact(fh, fr, n, kind):
+    w = fh.gen_reference(n, 1) if kind == "ref" else cube(fh, n)
+    print(f"grid n={n} kind={kind}", flush=True)
+    with fh.Flock.from_words(w) as f:
+        f.hash_count()
+        raw = f.read_grid(offsets=False, sorted=False)
+        keys, counts, offsets, srt = fr.grid(w, f.buckets)
+        assert np.array_equal(raw.keys, keys)
+        assert np.array_equal(raw.counts, counts)
+        f.scan()
+        f.scatter()
+        g = f.read_grid()
+        assert np.array_equal(g.offsets, offsets)
+        assert np.array_equal(g.counts, counts)
+        assert fr.check_grid(g.keys, g.offsets, g.sorted, f.buckets) == (0, "")
+        assert per_bucket_sets_equal(g.sorted, offsets, srt)
+
+
+@pytest.mark.parametrize("n", [3000, 200_000])
+def test_canonical_order_is_the_stable_sort(fh, fr, n):
+    w = fh.gen_reference(n, 2)
+    with fh.Flock.from_words(w, flags=fh.FLAG_CANONICAL) as f:
+        g = build_grid(f)
+        _, _, offsets, srt = fr.grid(w, f.buckets)
+        assert np.array_equal(g.offsets, offsets)
+        assert np.array_equal(g.sorted, srt)
+
+
+def test_dense_cell_and_hash_collisions(fh, fr):
+    # 3000 birds inside one 4 m cell plus a spread, hashed into only 64 buckets.
+    rng = np.random.default_rng(1)
+    w = fh.gen_reference(6000, 3)
+    w[:3000, 0:3] = (rng.random((3000, 3)) * 3.9 + np.array([0.05, 0.05, 100.05])).astype(np.float32)
+    w[:, 8] = np.float32(1.0)
+    with fh.Flock.from_words(w, buckets=64) as f:
+        g = build_grid(f)
+        keys, counts, offsets, srt = fr.grid(w, 64)
+        assert np.array_equal(g.keys, keys) and np.array_equal(g.offsets, offsets)
+        assert per_bucket_sets_equal(g.sorted, offsets, srt)
+        for i in [0, 1, 2999, 3000, 4500, 5999]:
+            for walk in (1, 2):
+                got = f.neighbours(i, walk, cap=8192)
+                want = fr.neighbours_brute(w, i, walk)
+                assert np.array_equal(got, want), (i, walk)
+
+
+def ties_ok(w, i, walk, got, want):
+    """Set difference allowed only for birds within 1e-5 of the radius."""
+    r = float(w[i, 8]) * walk
+    for j in set(got.tolist()) ^ set(want.tolist()):
+        d = float(np.linalg.norm(w[j, :3].astype(np.float64) - w[i, :3].astype(np.float64)))
+        if abs(d - r) >= 1e-5:
+            return False
+    return True
+
+
+@pytest.mark.parametrize("n,radius", [(2000, None), (20_000, 2.5), (1_000_000, None)])
+def test_neighbour_sets_match_brute_force(fh, fr, n, radius):
+    w = fh.gen_reference(n, 1) if n == 2000 else cube(fh, n, 4)
+    if radius:
+        w[:, 8] = np.float32(radius)
+    with fh.Flock.from_words(w) as f:
+        build_grid(f)
+        for i in np.linspace(0, n - 1, 40).astype(int):
+            for walk in (1, 2):
+                got = f.neighbours(int(i), walk)
+                want = fr.neighbours_brute(w, int(i), walk)
+                assert ties_ok(w, int(i), walk, got, want), (i, walk)
+
+
+@pytest.mark.parametrize("canonical", [False, True])
+def test_decide_matches_brute_force(fh, fr, canonical):
+    w = fh.gen_reference(4000, 6)
+    t = np.float32(0.0) + DT
+    with fh.Flock.from_words(w, flags=fh.FLAG_CANONICAL if canonical else 0) as f:
+        build_grid(f)
+        f.decide(t)
+        got, _ = f.download()
+    due = np.nonzero(t >= w[:, 12])[0]
+    assert len(due) > 300
+    for i in due:
+        d = fr.decide(w, int(i))
+        assert got[i, 13].view(np.uint32) == d["neighbours"], i
+        assert rel_err(got[i, 9:12], d["force"]) <= 1e-4, i
+        assert abs(got[i, 8] - d["radius"]) <= 1e-6 * d["radius"]
+        assert abs(got[i, 14] - d["centrality"]) <= 1e-5
+        assert got[i, 12] == w[i, 12] + np.float32(0.05)
+    other = np.setdiff1d(np.arange(4000), due)
+    assert np.array_equal(got[other].view(np.uint32), w[other].view(np.uint32))
+
+
+def test_integrate_matches_step_bird(fh, fr):
+    w = cube(fh, 5000, 8)
+    w[:, 9:12] = (np.random.default_rng(2).standard_normal((5000, 3)) * 0.2).astype(np.float32)
+    w[:, 6] = np.float32(0.3)
+    with fh.Flock.from_words(w) as f:
+        f.integrate()
+        got, _ = f.download()
+    want = np.stack([fr.step_bird(w[i]) for i in range(5000)])
+    assert rel_err(got[:, 0:6], want[:, 0:6]) <= 1e-5
+    assert np.max(np.abs(got[:, 6] - want[:, 6])) <= 1e-5
+    assert np.max(np.abs(got[:, 7] - want[:, 7])) <= 1e-3
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 5])
+def test_one_step_parity_reference_start(fh, fr, flags):
+    w = fh.gen_reference(2000, 1)
+    with fh.Flock.from_words(w, flags=flags) as f:
+        f.step(1)
+        gpu, t = f.download()
+    cpu, tc = fr.step(w, 0.0, 1)
+    d = fr.diff(cpu, gpu)
+    print("1-step", flags, d)
+    assert t == tc and d["ok"] and d["nb_mismatch"] == 0
+    assert rel_err(gpu[:, 0:6], cpu[:, 0:6]) <= 1e-4
+
+
+def test_parity_ladder_2000(fh, fr):
+    # scripts/parity.sh:23-34 and docs/SPEC.md:66-72
+    w0 = fh.gen_reference(2000, 1)
+    for k in (1, 10, 30, 100, 1000):
+        with fh.Flock.from_words(w0, flags=fh.FLAG_CANONICAL) as f:
+            f.step(k)
+            gpu, t = f.download()
+        cpu, tc = fr.step(w0, 0.0, k, threads=THREADS)
+        assert t == tc
+        if k <= 30:
+            d = fr.diff(cpu, gpu)
+        elif k <= 100:
+            d = fr.diff(cpu, gpu, allow_outliers=2, outlier_max=1e-2)
+        else:
+            d = fr.statdiff(cpu, gpu)
+        print(f"ladder {k}: {d}", flush=True)
+        assert d["ok"], (k, d)
+
+
+def test_c1_cube_10k_20_steps_parity_every_step(fh, fr):
+    # 10,000 birds, 100 m cube, seed 1: oracle parity after every step.
+    w0 = cube(fh, 10_000)
+    cpu, tc = w0, 0.0
+    drift = []
+    with fh.Flock.from_words(w0, flags=fh.FLAG_CANONICAL) as f:
+        for s in range(20):
+            f.step(1)
+            gpu, t = f.download()
+            cpu, tc = fr.step(cpu, tc, 1, threads=THREADS)
+            assert t == tc
+            d = fr.diff(cpu, gpu)
+            drift.append(rel_err(gpu[:, 0:6], cpu[:, 0:6]))
+            assert d["nb_mismatch"] == 0, (s, d)
+            if s == 0:
+                assert drift[0] <= 1e-4
+    print("C1 drift per step (max rel pos/vel):", ["%.2e" % x for x in drift])
+    assert max(drift) <= 1e-3
+
+
+def test_c2_cube_1m_one_step_parity(fh, fr):
+    # 1,000,000 birds, 464.16 m cube: one step against the oracle's grid walk
+    # (same neighbour semantics as brute force, pinned in test_oracle.py), and
+    # brute-force neighbour sets for a sample of due birds.
+    w0 = cube(fh, 1_000_000)
+    with fh.Flock.from_words(w0, flags=fh.FLAG_CANONICAL) as f:
+        f.step(1)
+        gpu, t = f.download()
+    cpu, tc = fr.step_grid(w0, 0.0, 1, threads=THREADS)
+    assert t == tc
+    d = fr.diff(cpu, gpu)
+    print("C2 1-step:", d, "rel", rel_err(gpu[:, 0:6], cpu[:, 0:6]))
+    assert d["nb_mismatch"] == 0
+    assert rel_err(gpu[:, 0:6], cpu[:, 0:6]) <= 1e-4
+    due = np.nonzero(DT >= w0[:, 12])[0][:200]
+    for i in due:
+        want = fr.neighbours_brute(w0, int(i), 1)
+        assert gpu[i, 13].view(np.uint32) == len(want)
+
+
+def test_step_is_deterministic_in_canonical_mode(fh):
+    w0 = cube(fh, 50_000, 3)
+    outs = []
+    for _ in range(2):
+        with fh.Flock.from_words(w0, flags=fh.FLAG_CANONICAL) as f:
+            f.step(25)
+            outs.append(f.download()[0])
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+def test_graph_and_unfused_replays_match_eager(fh):
+    w0 = cube(fh, 30_000, 5)
+    res = {}
+    for name, flags in [("eager", 1), ("graph", 1 | 4), ("unfused", 1 | 2)]:
+        with fh.Flock.from_words(w0, flags=flags) as f:
+            snaps = []
+            for _ in range(3):  # several replays with host work in between
+                f.step(7)
+                snaps.append(f.download()[0])
+            res[name] = snaps
+    for name in ("graph", "unfused"):
+        for a, b in zip(res["eager"], res[name]):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
+
+
+def test_pass_order_is_enforced(fh):
+    with fh.Flock.from_words(cube(fh, 100)) as f:
+        with pytest.raises(fh.FlockHipError) as e:
+            f.scatter()
+        assert e.value.status == fh.ERR_STATE
+        with pytest.raises(fh.FlockHipError) as e:
+            f.upload(cube(fh, 99))
+        assert e.value.status == fh.ERR_INVALID
+
+
+def test_empty_flock_steps(fh):
+    with fh.Flock(0) as f:
+        f.step(3)
+        w, t = f.download()
+        assert w.shape == (0, 15) and abs(t - 0.015) < 1e-6
+
+
+def test_timing_report(fh):
+    with fh.Flock.from_words(cube(fh, 100_000)) as f:
+        ms = f.step(6, 2, timing=True)
+    assert ms["timed_steps"] == 4 and ms["decide"] > 0 and ms["step_total"] > 0
+    print("timing", ms)
