@@ -120,6 +120,9 @@ fh_status fh_gen_cube(uint32_t n, float side, uint64_t seed, float *words);
  * (uniform in the cap), at 10 m/s. */
 fh_status fh_gen_clumps(uint32_t n, uint32_t clumps, float sigma, float cube, uint64_t seed,
                         float *words);
+/* The start state of a served request: fh_gen_cube with the cube edge
+ * cell * cbrt(n / birds_per_cell), i.e. `birds_per_cell` birds per cell. */
+fh_status fh_gen_request(uint32_t n, uint64_t seed, float birds_per_cell, float cell, float *words);
 
 /* ---- one flock: the reference pass API ---------------------------------- */
 /* Allocates the device state and grid for n birds (cuda/flock.cu:403-412). */
@@ -160,6 +163,90 @@ fh_status fh_read_grid(fh_flock *h, uint32_t *keys, uint32_t *counts, uint32_t *
  * walk 2 (d^2 <= 4 r^2), ascending bird index. *count may exceed cap. */
 fh_status fh_neighbours(fh_flock *h, uint32_t bird, int walk, uint32_t *ids, uint32_t cap,
                         uint32_t *count);
+
+/* ---- many flocks, one launch per stage ----------------------------------- */
+/* No counterpart in the reference, which steps one flock per process
+ * (cuda/flock.cu:374-523): independent flocks share one set of device pools
+ * and every pass of a step runs once for all of them. Each flock keeps its own
+ * clock, buckets (its own buckets_for(n)) and results; with
+ * FH_FLAG_CANONICAL a flock's result is bit-identical to stepping it alone. */
+typedef struct fh_batch fh_batch;
+/* Capacities: total birds, total buckets, and flock slots. */
+fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint32_t flock_capacity,
+                          const fh_params *params, fh_batch **out);
+fh_status fh_batch_destroy(fh_batch *b);
+/* Adds a flock (n > 0 .flk records at clock `time`); FH_ERR_BUDGET when the
+ * pools or slots are full. */
+fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, uint32_t *flock_id);
+fh_status fh_batch_remove(fh_batch *b, uint32_t flock_id);
+fh_status fh_batch_download(fh_batch *b, uint32_t flock_id, float *words, float *time);
+/* `steps` steps of every live flock (time += DT each). */
+fh_status fh_batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *out);
+fh_status fh_batch_info(const fh_batch *b, uint32_t *live_flocks, uint32_t *live_birds, uint32_t *bucket_range,
+                        size_t *device_bytes);
+/* Device bytes one flock of n birds occupies in a batch (state, sorted copies,
+ * grid), used for admission against an HBM budget. Host only. */
+size_t fh_flock_bytes(uint32_t n, const fh_params *params);
+
+/* ---- the served loop: continuous batching of flock requests --------------- */
+/* No counterpart in the reference. Requests (birds, steps, seed) arrive at any
+ * time; a worker thread keeps admitted flocks resident in one fh_batch, admits
+ * waiting requests in arrival order whenever a slot and HBM bytes (by
+ * fh_flock_bytes) are free, steps every active flock together and retires
+ * each one when its step count is reached. A request's start state is
+ * fh_gen_cube(birds, cell * (birds / birds_per_cell)^(1/3), seed). */
+typedef struct fh_server_config {
+    double hbm_budget_bytes;   /* admission budget [bytes]; default 200e9 */
+    uint32_t max_active;       /* flocks resident at once (slot cap); default 64 */
+    uint32_t bird_capacity;    /* device pool size in birds; default 40 Mi */
+    uint32_t bucket_capacity;  /* device pool size in buckets; default 64 Mi */
+    uint32_t steps_per_iter;   /* batched steps between admissions; default 8 */
+    uint32_t prep_threads;     /* host threads generating start states; default 4 */
+    uint32_t keep_states;      /* keep each final state for fh_server_take_state */
+    float birds_per_cell;      /* request density; default 0.64 */
+    float cell_for_density;    /* cell edge the density refers to [m]; default 4 */
+    fh_params params;          /* step tunables; default canonical order */
+} fh_server_config;
+
+typedef struct fh_result {
+    uint64_t id, seed;
+    uint32_t birds, steps;
+    int32_t status;            /* FH_OK or the error that ended the request */
+    float final_time;          /* flock clock at completion */
+    double arrival_s, admit_s, done_s;  /* seconds since the server started */
+    uint64_t checksum;         /* FNV-1a 64 of the final .flk records */
+} fh_result;
+
+typedef struct fh_server_stats {
+    uint64_t submitted, iterations, steps;
+    double bird_steps;         /* sum over iterations of active birds x steps */
+    double busy_s;             /* time inside batched steps */
+    double wall_s;             /* time since the server started */
+    double peak_bytes;         /* largest admitted HBM bytes */
+    uint32_t peak_active, queued, active;
+} fh_server_stats;
+
+typedef struct fh_server fh_server;
+
+fh_status fh_server_config_default(fh_server_config *c);
+fh_status fh_server_create(const fh_server_config *cfg, fh_server **out);
+/* Stops the worker (active and queued requests are dropped) and frees all. */
+fh_status fh_server_destroy(fh_server *s);
+/* FH_ERR_INVALID for birds == 0 or steps == 0; FH_ERR_BUDGET when one request
+ * alone exceeds the budget or the pool. */
+fh_status fh_server_submit(fh_server *s, uint32_t birds, uint32_t steps, uint64_t seed, uint64_t *id);
+/* Moves up to cap completed results to out. */
+fh_status fh_server_poll(fh_server *s, fh_result *out, uint32_t cap, uint32_t *count);
+/* Waits until nothing is queued or active. */
+fh_status fh_server_drain(fh_server *s, double timeout_s);
+fh_status fh_server_stats_get(fh_server *s, fh_server_stats *out);
+/* Final .flk records of a completed request (keep_states), once. */
+fh_status fh_server_take_state(fh_server *s, uint64_t id, float *words, uint32_t cap, float *time);
+/* Open-loop replay: request i is submitted arrival_s[i] seconds after start;
+ * returns when all are done. results[i] belongs to request i. */
+fh_status fh_server_replay(const fh_server_config *cfg, const uint32_t *birds, const uint32_t *steps,
+                           const uint64_t *seeds, const double *arrival_s, uint32_t count,
+                           fh_result *results, fh_server_stats *stats);
 
 #ifdef __cplusplus
 }

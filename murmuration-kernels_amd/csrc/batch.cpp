@@ -1,0 +1,511 @@
+// Many independent flocks stepped together: one launch per stage for all of
+// them. Flocks share one set of device pools; flock f owns a run of 256-bird
+// tiles and a run of buckets (its own buckets_for(n_f)), so the counting sort
+// is segmented by (flock, bucket) simply by giving each flock its own bucket
+// range in one global bucket space. Each flock keeps its own clock.
+//
+// The reference steps one flock per process (cuda/flock.cu:374-523); this is
+// the batched growth on top of the same passes.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "batch.h"
+#include "host_util.h"
+#include "model.h"
+
+namespace fh {
+hipError_t launch_fill(uint32_t *dst, uint32_t count, uint32_t value, hipStream_t s);
+
+// First-fit allocator over [0, capacity) in whole units.
+uint32_t RangeAlloc::alloc(uint32_t len) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second >= len) {
+            const uint32_t at = it->first, left = it->second - len;
+            free_.erase(it);
+            if (left) free_[at + len] = left;
+            used_[at] = len;
+            return at;
+        }
+    }
+    return NONE;
+}
+
+void RangeAlloc::release(uint32_t at) {
+    auto u = used_.find(at);
+    if (u == used_.end()) return;
+    uint32_t len = u->second;
+    used_.erase(u);
+    auto nx = free_.lower_bound(at);
+    if (nx != free_.end() && at + len == nx->first) {
+        len += nx->second;
+        nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == at) {
+            pv->second += len;
+            return;
+        }
+    }
+    free_[at] = len;
+}
+
+uint32_t RangeAlloc::high_water() const { return used_.empty() ? 0u : used_.rbegin()->first + used_.rbegin()->second; }
+
+}  // namespace fh
+
+using namespace fh;
+
+static fh_status bcheck(const fh_batch *b, const char *fn) {
+    return b ? FH_OK : set_error(FH_ERR_INVALID, "%s: NULL batch", fn);
+}
+
+// Geometry of the live flocks: bird range up to the highest live tile, bucket
+// range up to the highest live bucket.
+static void refresh_geometry(fh_batch *b) {
+    b->geo.segs = b->d_segs;
+    b->geo.tile_seg = b->d_tile_seg;
+    b->geo.n_birds = b->tiles.high_water() * TILE;
+    b->geo.n_buckets = b->bucket_units.high_water() * SCAN_TILE;
+    uint32_t slots = 0;
+    for (const Seg &s : b->segs)
+        if (s.live) slots += s.n;
+    b->geo.n_slots = slots;
+    b->graph_valid = false;
+}
+
+static fh_status push_tables(fh_batch *b) {
+    if (!b->segs.empty())
+        FH_HIP(hipMemcpyAsync(b->d_segs, b->segs.data(), b->segs.size() * sizeof(Seg), hipMemcpyHostToDevice,
+                              b->stream));
+    if (!b->tile_seg.empty())
+        FH_HIP(hipMemcpyAsync(b->d_tile_seg, b->tile_seg.data(), b->tile_seg.size() * 4, hipMemcpyHostToDevice,
+                              b->stream));
+    FH_HIP(hipStreamSynchronize(b->stream));
+    return FH_OK;
+}
+
+static fh_status batch_hash(fh_batch *b) {
+    if (b->counts_dirty) FH_HIP(launch_fill(b->P.counts, b->bucket_cap_units * SCAN_TILE, 0u, b->stream));
+    FH_HIP(launch_hash_count(b->geo, b->P, (b->prm.flags & FH_FLAG_WAVE_AGG) != 0, b->stream));
+    b->counts_dirty = true;
+    b->hashed = true;
+    return FH_OK;
+}
+
+// One batched step: every live flock's clock += DT on the device, then the
+// passes over all flocks (relative due test, fused histogram).
+static fh_status batch_step_launches(fh_batch *b, hipEvent_t *ev) {
+    FH_HIP(launch_set_time(b->d_segs, (uint32_t)b->segs.size(), 0.0f, DT, b->stream));
+    if (ev) FH_HIP(hipEventRecord(ev[0], b->stream));
+    if (!b->hashed) {
+        fh_status st = batch_hash(b);
+        if (st) return st;
+    }
+    if (ev) FH_HIP(hipEventRecord(ev[1], b->stream));
+    FH_HIP(launch_scan(b->geo, b->P, b->stream));
+    b->counts_dirty = false;
+    if (ev) FH_HIP(hipEventRecord(ev[2], b->stream));
+    FH_HIP(launch_scatter(b->geo, b->P, (b->prm.flags & FH_FLAG_CANONICAL) != 0, true, b->stream));
+    if (ev) FH_HIP(hipEventRecord(ev[3], b->stream));
+    FH_HIP(launch_decide(b->geo, b->P, 0.0f, b->prm.decide_block, b->prm.birds_per_thread, b->stream));
+    if (ev) FH_HIP(hipEventRecord(ev[4], b->stream));
+    FH_HIP(launch_integrate(b->geo, b->P, true, (b->prm.flags & FH_FLAG_WAVE_AGG) != 0, b->stream));
+    b->counts_dirty = true;
+    b->hashed = true;
+    if (ev) FH_HIP(hipEventRecord(ev[5], b->stream));
+    return FH_OK;
+}
+
+static fh_status batch_capture(fh_batch *b) {
+    if (b->graph_valid) return FH_OK;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    FH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const bool h0 = b->hashed, d0 = b->counts_dirty;
+    fh_status st = batch_step_launches(b, nullptr);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    b->hashed = h0;
+    b->counts_dirty = d0;
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        b->graph = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    b->graph_valid = true;
+    return FH_OK;
+}
+
+namespace fh {
+
+fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *out) {
+    if (warmup > steps) warmup = steps;
+    const bool graph = (b->prm.flags & FH_FLAG_GRAPH) != 0;
+    const uint32_t timed = steps - warmup;
+    const bool per_pass = out && !graph;
+    std::vector<hipEvent_t> ev;
+    if (out) {
+        ev.resize(per_pass ? (size_t)timed * 6 : 2);
+        for (auto &x : ev) FH_HIP(hipEventCreate(&x));
+    }
+    auto destroy = [&] {
+        for (auto &x : ev) (void)hipEventDestroy(x);
+    };
+    fh_status st = FH_OK;
+    std::chrono::steady_clock::time_point t0{}, t1{};
+    for (uint32_t s = 0; s < steps && st == FH_OK; ++s) {
+        if (s == warmup) {
+            (void)hipStreamSynchronize(b->stream);
+            t0 = std::chrono::steady_clock::now();
+            if (out && graph) (void)hipEventRecord(ev[0], b->stream);
+        }
+        for (size_t f = 0; f < b->segs.size(); ++f)
+            if (b->segs[f].live) b->times[f] += DT;  // mirrors the device clock add
+        if (graph) {
+            if (!b->hashed && (st = batch_hash(b))) break;
+            if ((st = batch_capture(b))) break;
+            hipError_t e = hipGraphLaunch(b->graph, b->stream);
+            if (e != hipSuccess) st = hip_fail(e, "hipGraphLaunch");
+            b->hashed = true;
+            b->counts_dirty = true;
+        } else {
+            st = batch_step_launches(b, (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 6] : nullptr);
+        }
+    }
+    if (st) {
+        (void)hipStreamSynchronize(b->stream);
+        destroy();
+        return st;
+    }
+    if (out && graph && timed) FH_HIP(hipEventRecord(ev[1], b->stream));
+    FH_HIP(hipStreamSynchronize(b->stream));
+    t1 = std::chrono::steady_clock::now();
+    if (out) {
+        std::memset(out, 0, sizeof *out);
+        out->timed_steps = timed;
+        if (timed) {
+            out->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            if (per_pass) {
+                double acc[5] = {0, 0, 0, 0, 0};
+                for (uint32_t s = 0; s < timed; ++s)
+                    for (int k = 0; k < 5; ++k) {
+                        float ms = 0.0f;
+                        FH_HIP(hipEventElapsedTime(&ms, ev[(size_t)s * 6 + k], ev[(size_t)s * 6 + k + 1]));
+                        acc[k] += ms;
+                    }
+                out->grid_histogram = acc[0] / timed;
+                out->grid_scan = acc[1] / timed;
+                out->grid_scatter = acc[2] / timed;
+                out->decide = acc[3] / timed;
+                out->integrate = acc[4] / timed;
+                out->step_total = (acc[0] + acc[1] + acc[2] + acc[3] + acc[4]) / timed;
+            } else {
+                float ms = 0.0f;
+                FH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+                out->step_total = ms / timed;
+            }
+        }
+        uint32_t due = 0;
+        FH_HIP(hipMemcpy(&due, b->P.due_count, 4, hipMemcpyDeviceToHost));
+        out->due_last = due;
+    }
+    destroy();
+    uint32_t err = 0;
+    FH_HIP(hipMemcpy(&err, b->P.err, 4, hipMemcpyDeviceToHost));
+    if (err & ERR_SLOT) return set_error(FH_ERR_HIP, "kernel reported a slot outside the sorted range (0x%x)", err);
+    return FH_OK;
+}
+
+size_t batch_bytes_per_bird() {
+    // pos4, vel4, steer4, spos, svel (16 B each) + spin, cen, nb, key, rank,
+    // snext, sorted_tmp (4 B each)
+    return 5 * 16 + 7 * 4;
+}
+
+size_t batch_bytes_per_bucket() { return 4 /*counts*/ + 4 /*offsets*/; }
+
+}  // namespace fh
+
+extern "C" {
+
+fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint32_t flock_capacity,
+                          const fh_params *params, fh_batch **out) {
+    clear_error();
+    if (!out) return set_error(FH_ERR_INVALID, "fh_batch_create: NULL out");
+    *out = nullptr;
+    if (flock_capacity == 0 || flock_capacity > (1u << 20) || bird_capacity > (1u << 30) ||
+        bucket_capacity > (1u << 30))
+        return set_error(FH_ERR_INVALID, "fh_batch_create: bad capacity");
+    fh_params prm;
+    fh_params_default(&prm);
+    if (params) {
+        if (params->cell != 0.0f) prm.cell = params->cell;
+        prm.buckets = params->buckets;
+        if (params->decide_block) prm.decide_block = params->decide_block;
+        if (params->birds_per_thread) prm.birds_per_thread = params->birds_per_thread;
+        prm.flags = params->flags;
+    }
+    const uint32_t blk = prm.decide_block;
+    if (blk != 64 && blk != 128 && blk != 256 && blk != 512 && blk != 1024)
+        return set_error(FH_ERR_INVALID, "fh_batch_create: decide_block %u", blk);
+    if (prm.birds_per_thread > 64 || blk * prm.birds_per_thread > 16384)
+        return set_error(FH_ERR_INVALID, "fh_batch_create: decide chunk too large");
+    if (!(prm.cell >= 0.5f) || !(prm.cell <= 1000.0f))
+        return set_error(FH_ERR_INVALID, "fh_batch_create: cell %g outside [0.5, 1000] m", prm.cell);
+    fh_status st = require_device();
+    if (st) return st;
+
+    fh_batch *b = new fh_batch();
+    b->prm = prm;
+    const uint32_t tile_cap = (bird_capacity + TILE - 1) / TILE;
+    b->bird_cap = tile_cap * TILE;
+    b->bucket_cap_units = (bucket_capacity + SCAN_TILE - 1) / SCAN_TILE;
+    b->flock_cap = flock_capacity;
+    b->tiles.reset(tile_cap);
+    b->bucket_units.reset(b->bucket_cap_units);
+    const size_t nb = (size_t)b->bucket_cap_units * SCAN_TILE;
+    const size_t nn = b->bird_cap ? b->bird_cap : 1;
+
+    Arena a;
+    size_t o_pos = a.take<float4>(nn), o_vel = a.take<float4>(nn), o_steer = a.take<float4>(nn);
+    size_t o_spos = a.take<float4>(nn), o_svel = a.take<float4>(nn);
+    size_t o_spin = a.take<float>(nn), o_cen = a.take<float>(nn), o_nb = a.take<uint32_t>(nn);
+    size_t o_key = a.take<uint32_t>(nn), o_rank = a.take<uint32_t>(nn), o_snext = a.take<float>(nn);
+    size_t o_tmp = a.take<uint32_t>(nn);
+    size_t o_counts = a.take<uint32_t>(nb), o_offsets = a.take<uint32_t>(nb + 1);
+    size_t o_tsum = a.take<uint32_t>(b->bucket_cap_units + 1), o_tpre = a.take<uint32_t>(b->bucket_cap_units + 1);
+    size_t o_err = a.take<uint32_t>(4), o_due = a.take<uint32_t>(4);
+    size_t o_segs = a.take<Seg>(flock_capacity), o_tseg = a.take<uint32_t>(tile_cap ? tile_cap : 1);
+    hipError_t e = hipMalloc(&b->arena, a.used);
+    if (e != hipSuccess) {
+        delete b;
+        return hip_fail(e, "hipMalloc(batch arena)");
+    }
+    b->arena_bytes = a.used;
+    char *base = static_cast<char *>(b->arena);
+    b->P.pos4 = reinterpret_cast<float4 *>(base + o_pos);
+    b->P.vel4 = reinterpret_cast<float4 *>(base + o_vel);
+    b->P.steer4 = reinterpret_cast<float4 *>(base + o_steer);
+    b->P.spos = reinterpret_cast<float4 *>(base + o_spos);
+    b->P.svel = reinterpret_cast<float4 *>(base + o_svel);
+    b->P.spin = reinterpret_cast<float *>(base + o_spin);
+    b->P.cen = reinterpret_cast<float *>(base + o_cen);
+    b->P.nb = reinterpret_cast<uint32_t *>(base + o_nb);
+    b->P.key = reinterpret_cast<uint32_t *>(base + o_key);
+    b->P.rank = reinterpret_cast<uint32_t *>(base + o_rank);
+    b->P.snext = reinterpret_cast<float *>(base + o_snext);
+    b->P.sorted_tmp = reinterpret_cast<uint32_t *>(base + o_tmp);
+    b->P.counts = reinterpret_cast<uint32_t *>(base + o_counts);
+    b->P.offsets = reinterpret_cast<uint32_t *>(base + o_offsets);
+    b->P.tile_sums = reinterpret_cast<uint32_t *>(base + o_tsum);
+    b->P.tile_prefix = reinterpret_cast<uint32_t *>(base + o_tpre);
+    b->P.err = reinterpret_cast<uint32_t *>(base + o_err);
+    b->P.due_count = reinterpret_cast<uint32_t *>(base + o_due);
+    b->d_segs = reinterpret_cast<Seg *>(base + o_segs);
+    b->d_tile_seg = reinterpret_cast<uint32_t *>(base + o_tseg);
+    b->tile_seg.assign(tile_cap, NO_SEG);
+
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) {
+        (void)hipFree(b->arena);
+        delete b;
+        return hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    if ((e = hipMemsetAsync(b->arena, 0, a.used, b->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(b->stream)) != hipSuccess) {
+        (void)hipStreamDestroy(b->stream);
+        (void)hipFree(b->arena);
+        delete b;
+        return hip_fail(e, "hipMemsetAsync(batch arena)");
+    }
+    if ((st = push_tables(b))) {
+        (void)hipStreamDestroy(b->stream);
+        (void)hipFree(b->arena);
+        delete b;
+        return st;
+    }
+    refresh_geometry(b);
+    *out = b;
+    return FH_OK;
+}
+
+fh_status fh_batch_destroy(fh_batch *b) {
+    clear_error();
+    if (!b) return FH_OK;
+    (void)hipStreamSynchronize(b->stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    (void)hipFree(b->arena);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return FH_OK;
+}
+
+fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, uint32_t *flock_id) {
+    clear_error();
+    fh_status st = bcheck(b, "fh_batch_add");
+    if (st) return st;
+    if (!flock_id || (n && !words) || n == 0)
+        return set_error(FH_ERR_INVALID, "fh_batch_add: need n > 0, words and flock_id");
+    const uint32_t buckets = b->prm.buckets ? b->prm.buckets : buckets_for(n);
+    const uint32_t ntile = (n + TILE - 1) / TILE;
+    const uint32_t nunit = (buckets + SCAN_TILE - 1) / SCAN_TILE;
+    // find a free segment slot
+    uint32_t f = NO_SEG;
+    for (uint32_t i = 0; i < b->segs.size(); ++i)
+        if (!b->segs[i].live) {
+            f = i;
+            break;
+        }
+    if (f == NO_SEG) {
+        if (b->segs.size() >= b->flock_cap) return set_error(FH_ERR_BUDGET, "fh_batch_add: flock slots full");
+        f = (uint32_t)b->segs.size();
+        b->segs.push_back(Seg{});
+        b->times.push_back(0.0f);
+    }
+    const uint32_t t0 = b->tiles.alloc(ntile);
+    if (t0 == RangeAlloc::NONE) return set_error(FH_ERR_BUDGET, "fh_batch_add: no room for %u birds", n);
+    const uint32_t u0 = b->bucket_units.alloc(nunit);
+    if (u0 == RangeAlloc::NONE) {
+        b->tiles.release(t0);
+        return set_error(FH_ERR_BUDGET, "fh_batch_add: no room for %u buckets", buckets);
+    }
+    Seg s{};
+    s.bird_base = t0 * TILE;
+    s.n = n;
+    s.bucket_base = u0 * SCAN_TILE;
+    s.mask = buckets - 1;
+    s.cell = b->prm.cell;
+    s.inv_cell = 1.0f / b->prm.cell;
+    s.time = time;
+    s.live = 1;
+    b->segs[f] = s;
+    b->times[f] = time;
+    for (uint32_t t = 0; t < ntile; ++t) b->tile_seg[t0 + t] = f;
+
+    std::vector<float4> pos(n), vel(n), steer(n);
+    std::vector<float> spin(n), cen(n);
+    std::vector<uint32_t> nbv(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float *w = words + (size_t)i * FH_WORDS;
+        pos[i] = make_float4(w[PX], w[PY], w[PZ], w[NEXT]);
+        vel[i] = make_float4(w[VX], w[VY], w[VZ], w[ROLL]);
+        steer[i] = make_float4(w[SX], w[SY], w[SZ], w[RADIUS]);
+        spin[i] = w[SPIN];
+        std::memcpy(&nbv[i], &w[NB], 4);
+        cen[i] = w[CEN];
+    }
+    const size_t o = s.bird_base;
+    FH_HIP(hipMemcpyAsync(b->P.pos4 + o, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
+    FH_HIP(hipMemcpyAsync(b->P.vel4 + o, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
+    FH_HIP(hipMemcpyAsync(b->P.steer4 + o, steer.data(), n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
+    FH_HIP(hipMemcpyAsync(b->P.spin + o, spin.data(), n * 4, hipMemcpyHostToDevice, b->stream));
+    FH_HIP(hipMemcpyAsync(b->P.cen + o, cen.data(), n * 4, hipMemcpyHostToDevice, b->stream));
+    FH_HIP(hipMemcpyAsync(b->P.nb + o, nbv.data(), n * 4, hipMemcpyHostToDevice, b->stream));
+    if ((st = push_tables(b))) return st;
+    // Count the new flock's birds into the histogram if the others are already
+    // counted; otherwise the next step hashes everything.
+    refresh_geometry(b);
+    if (b->hashed) {
+        Geometry g = b->geo;
+        g.tile_begin = t0;
+        g.tile_count = ntile;
+        FH_HIP(launch_hash_count(g, b->P, (b->prm.flags & FH_FLAG_WAVE_AGG) != 0, b->stream));
+    }
+    *flock_id = f;
+    return FH_OK;
+}
+
+fh_status fh_batch_remove(fh_batch *b, uint32_t flock_id) {
+    clear_error();
+    fh_status st = bcheck(b, "fh_batch_remove");
+    if (st) return st;
+    if (flock_id >= b->segs.size() || !b->segs[flock_id].live)
+        return set_error(FH_ERR_INVALID, "fh_batch_remove: no live flock %u", flock_id);
+    Seg &s = b->segs[flock_id];
+    if (b->counts_dirty)  // drop this flock's birds from the pending histogram
+        FH_HIP(launch_fill(b->P.counts + s.bucket_base, s.mask + 1, 0u, b->stream));
+    const uint32_t t0 = s.bird_base / TILE, ntile = (s.n + TILE - 1) / TILE;
+    for (uint32_t t = 0; t < ntile; ++t) b->tile_seg[t0 + t] = NO_SEG;
+    b->tiles.release(t0);
+    b->bucket_units.release(s.bucket_base / SCAN_TILE);
+    s.live = 0;
+    if ((st = push_tables(b))) return st;
+    refresh_geometry(b);
+    return FH_OK;
+}
+
+fh_status fh_batch_download(fh_batch *b, uint32_t flock_id, float *words, float *time) {
+    clear_error();
+    fh_status st = bcheck(b, "fh_batch_download");
+    if (st) return st;
+    if (flock_id >= b->segs.size() || !b->segs[flock_id].live)
+        return set_error(FH_ERR_INVALID, "fh_batch_download: no live flock %u", flock_id);
+    const Seg &s = b->segs[flock_id];
+    const uint32_t n = s.n;
+    if (!words) return set_error(FH_ERR_INVALID, "fh_batch_download: NULL words");
+    std::vector<float4> pos(n), vel(n), steer(n);
+    std::vector<float> spin(n), cen(n);
+    std::vector<uint32_t> nbv(n);
+    const size_t o = s.bird_base;
+    FH_HIP(hipMemcpyAsync(pos.data(), b->P.pos4 + o, n * sizeof(float4), hipMemcpyDeviceToHost, b->stream));
+    FH_HIP(hipMemcpyAsync(vel.data(), b->P.vel4 + o, n * sizeof(float4), hipMemcpyDeviceToHost, b->stream));
+    FH_HIP(hipMemcpyAsync(steer.data(), b->P.steer4 + o, n * sizeof(float4), hipMemcpyDeviceToHost, b->stream));
+    FH_HIP(hipMemcpyAsync(spin.data(), b->P.spin + o, n * 4, hipMemcpyDeviceToHost, b->stream));
+    FH_HIP(hipMemcpyAsync(cen.data(), b->P.cen + o, n * 4, hipMemcpyDeviceToHost, b->stream));
+    FH_HIP(hipMemcpyAsync(nbv.data(), b->P.nb + o, n * 4, hipMemcpyDeviceToHost, b->stream));
+    FH_HIP(hipStreamSynchronize(b->stream));
+    for (uint32_t i = 0; i < n; ++i) {
+        float *w = words + (size_t)i * FH_WORDS;
+        w[PX] = pos[i].x; w[PY] = pos[i].y; w[PZ] = pos[i].z;
+        w[VX] = vel[i].x; w[VY] = vel[i].y; w[VZ] = vel[i].z;
+        w[ROLL] = vel[i].w;
+        w[SPIN] = spin[i];
+        w[RADIUS] = steer[i].w;
+        w[SX] = steer[i].x; w[SY] = steer[i].y; w[SZ] = steer[i].z;
+        w[NEXT] = pos[i].w;
+        std::memcpy(&w[NB], &nbv[i], 4);
+        w[CEN] = cen[i];
+    }
+    if (time) *time = b->times[flock_id];
+    return FH_OK;
+}
+
+fh_status fh_batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *out) {
+    clear_error();
+    fh_status st = bcheck(b, "fh_batch_step");
+    return st ? st : batch_step(b, steps, warmup, out);
+}
+
+fh_status fh_batch_info(const fh_batch *b, uint32_t *live_flocks, uint32_t *live_birds, uint32_t *bucket_range,
+                        size_t *device_bytes) {
+    clear_error();
+    fh_status st = bcheck(b, "fh_batch_info");
+    if (st) return st;
+    uint32_t lf = 0;
+    for (const Seg &s : b->segs) lf += s.live ? 1u : 0u;
+    if (live_flocks) *live_flocks = lf;
+    if (live_birds) *live_birds = b->geo.n_slots;
+    if (bucket_range) *bucket_range = b->geo.n_buckets;
+    if (device_bytes) *device_bytes = b->arena_bytes;
+    return FH_OK;
+}
+
+size_t fh_flock_bytes(uint32_t n, const fh_params *params) {
+    const uint32_t buckets = (params && params->buckets) ? params->buckets : buckets_for(n);
+    const size_t birds = (size_t)((n + TILE - 1) / TILE) * TILE;
+    const size_t bk = (size_t)((buckets + SCAN_TILE - 1) / SCAN_TILE) * SCAN_TILE;
+    return birds * batch_bytes_per_bird() + bk * batch_bytes_per_bucket();
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// The batched multi-flock engine shared by fh_batch_* and the served loop.
+#pragma once
+
+#include <map>
+#include <vector>
+
+#include "../../include/flock_hip.h"
+#include "layout.h"
+
+namespace fh {
+
+// First-fit allocator over [0, capacity) in whole units (tiles of birds or
+// tiles of buckets), with coalescing of neighbouring free runs.
+class RangeAlloc {
+   public:
+    static constexpr uint32_t NONE = 0xffffffffu;
+    void reset(uint32_t capacity) {
+        free_.clear();
+        used_.clear();
+        if (capacity) free_[0] = capacity;
+    }
+    uint32_t alloc(uint32_t len);
+    void release(uint32_t at);
+    uint32_t high_water() const;
+
+   private:
+    std::map<uint32_t, uint32_t> free_, used_;  // start -> length
+};
+
+size_t batch_bytes_per_bird();
+size_t batch_bytes_per_bucket();
+
+}  // namespace fh
+
+struct fh_batch {
+    fh_params prm{};
+    uint32_t bird_cap = 0, bucket_cap_units = 0, flock_cap = 0;
+    hipStream_t stream = nullptr;
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    fh::Pools P{};
+    fh::Seg *d_segs = nullptr;
+    uint32_t *d_tile_seg = nullptr;
+    std::vector<fh::Seg> segs;        // host mirror of the segment table
+    std::vector<float> times;         // host mirror of each flock clock
+    std::vector<uint32_t> tile_seg;   // host mirror of tile -> segment
+    fh::RangeAlloc tiles, bucket_units;
+    fh::Geometry geo{};
+    bool hashed = false;        // counts hold the histogram of every live bird
+    bool counts_dirty = false;  // histogram filled and not yet consumed by a scan
+    hipGraphExec_t graph = nullptr;
+    bool graph_valid = false;
+};
+
+namespace fh {
+fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *out);
+}

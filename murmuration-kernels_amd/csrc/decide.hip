@@ -57,29 +57,71 @@ __device__ __forceinline__ Walk make_walk(f3 p, float r, const Seg &sg) {
     return w;
 }
 
+// Cells of the box are visited in z-y-x order (x fastest), as the nested
+// loops of cuda/flock.cu:209-211. They are taken PREFETCH at a time: the
+// bucket ranges of all PREFETCH cells are loaded together, then the first
+// candidate of each non-empty bucket, so one lane has up to 2 * PREFETCH
+// independent loads in flight instead of a chain of dependent ones. The visit
+// order of candidates is unchanged.
+constexpr int PREFETCH = 8;
+
 template <class Visit>
 __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const Pools &p, uint32_t n_slots,
                                            uint32_t me, Visit &&visit) {
     if (!w.ok) return;
-    for (int32_t cz = w.lo[2]; cz <= w.hi[2]; ++cz)
-        for (int32_t cy = w.lo[1]; cy <= w.hi[1]; ++cy)
-            for (int32_t cx = w.lo[0]; cx <= w.hi[0]; ++cx) {
-                if (!cell_meets_sphere(cx, cy, cz, sg.cell, w.p, w.c2)) continue;
-                const uint32_t b = sg.bucket_base + (bucket_hash(cx, cy, cz) & sg.mask);
-                const uint32_t s0 = p.offsets[b];
-                const uint32_t s1 = min(p.offsets[b + 1], n_slots);
-                for (uint32_t t = s0; t < s1; ++t) {
-                    const float4 c = p.spos[t];
-                    if (__float_as_uint(c.w) == me) continue;
-                    if (cell_coord(c.x, sg.inv_cell) != cx || cell_coord(c.y, sg.inv_cell) != cy ||
-                        cell_coord(c.z, sg.inv_cell) != cz)
-                        continue;
-                    const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
-                    const float d2 = dot3(d, d);
-                    if (d2 > w.c2) continue;
-                    visit(t, d, d2);
+    const int32_t nx = w.hi[0] - w.lo[0] + 1, ny = w.hi[1] - w.lo[1] + 1, nz = w.hi[2] - w.lo[2] + 1;
+    const int32_t ncell = nx * ny * nz;
+    int32_t cx0 = w.lo[0], cy0 = w.lo[1], cz0 = w.lo[2];  // cell of index `base`
+    for (int32_t base = 0; base < ncell; base += PREFETCH) {
+        int32_t cx[PREFETCH], cy[PREFETCH], cz[PREFETCH];
+        uint32_t s0[PREFETCH], s1[PREFETCH];
+        {
+            int32_t x = cx0, y = cy0, z = cz0;
+#pragma unroll
+            for (int k = 0; k < PREFETCH; ++k) {
+                cx[k] = x;
+                cy[k] = y;
+                cz[k] = z;
+                s0[k] = 0u;
+                s1[k] = 0u;
+                if (base + k < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2)) {
+                    const uint32_t b = sg.bucket_base + (bucket_hash(x, y, z) & sg.mask);
+                    s0[k] = p.offsets[b];
+                    s1[k] = p.offsets[b + 1];
+                }
+                if (++x > w.hi[0]) {
+                    x = w.lo[0];
+                    if (++y > w.hi[1]) {
+                        y = w.lo[1];
+                        ++z;
+                    }
                 }
             }
+            cx0 = x;
+            cy0 = y;
+            cz0 = z;
+        }
+        float4 first[PREFETCH];
+#pragma unroll
+        for (int k = 0; k < PREFETCH; ++k) {
+            s1[k] = min(s1[k], n_slots);
+            first[k] = s0[k] < s1[k] ? p.spos[s0[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int k = 0; k < PREFETCH; ++k) {
+            for (uint32_t t = s0[k]; t < s1[k]; ++t) {
+                const float4 c = t == s0[k] ? first[k] : p.spos[t];
+                if (__float_as_uint(c.w) == me) continue;
+                if (cell_coord(c.x, sg.inv_cell) != cx[k] || cell_coord(c.y, sg.inv_cell) != cy[k] ||
+                    cell_coord(c.z, sg.inv_cell) != cz[k])
+                    continue;
+                const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
+                const float d2 = dot3(d, d);
+                if (d2 > w.c2) continue;
+                visit(t, d, d2);
+            }
+        }
+    }
 }
 
 __device__ __forceinline__ Decision decide_bird(const Pools &p, const Seg &sg, uint32_t n_slots, uint32_t me,

@@ -259,6 +259,14 @@ fh_status fh_gen_clumps(uint32_t n, uint32_t clumps, float sigma, float cube, ui
     return FH_OK;
 }
 
+fh_status fh_gen_request(uint32_t n, uint64_t seed, float birds_per_cell, float cell, float *words) {
+    clear_error();
+    if (!(birds_per_cell > 0.0f) || !(cell > 0.0f))
+        return set_error(FH_ERR_INVALID, "fh_gen_request: birds_per_cell and cell must be > 0");
+    const float side = cell * cbrtf((float)(n ? n : 1) / birds_per_cell);
+    return fh_gen_cube(n, side, seed, words);
+}
+
 fh_status fh_params_default(fh_params *p) {
     clear_error();
     if (!p) return set_error(FH_ERR_INVALID, "fh_params_default: NULL");

@@ -84,9 +84,10 @@ __device__ __forceinline__ uint32_t count_key(uint32_t *counts, uint32_t k, bool
 template <bool WAVE_AGG>
 __global__ __launch_bounds__(TILE) void k_hash_count(Geometry g, Pools p) {
     bool skip;
-    const Seg &sg = tile_segment(g, blockIdx.x, &skip);
+    const uint32_t tile = g.tile_begin + blockIdx.x;
+    const Seg &sg = tile_segment(g, tile, &skip);
     if (skip) return;
-    const uint32_t bird = blockIdx.x * TILE + threadIdx.x;
+    const uint32_t bird = tile * TILE + threadIdx.x;
     const bool active = bird < g.n_birds && bird - sg.bird_base < sg.n;
     uint32_t k = 0;
     if (active) {
@@ -177,9 +178,10 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *__restrict__ count
 // or (canonical mode) only the slot -> bird map for the reordering pass.
 __global__ __launch_bounds__(TILE) void k_scatter(Geometry g, Pools p, int canonical, int relative) {
     bool skip;
-    const Seg &sg = tile_segment(g, blockIdx.x, &skip);
+    const uint32_t tile = g.tile_begin + blockIdx.x;
+    const Seg &sg = tile_segment(g, tile, &skip);
     if (skip) return;
-    const uint32_t bird = blockIdx.x * TILE + threadIdx.x;
+    const uint32_t bird = tile * TILE + threadIdx.x;
     if (bird >= g.n_birds || bird - sg.bird_base >= sg.n) return;
     const uint32_t s = p.offsets[p.key[bird]] + p.rank[bird];
     if (s >= g.n_slots) {
@@ -250,14 +252,16 @@ __global__ void k_set_time(Seg *segs, uint32_t nseg, float time, float dt) {
     }
 }
 
-static inline uint32_t tiles_of(uint32_t n) { return (n + TILE - 1) / TILE; }
+static inline uint32_t launch_tiles(const Geometry &g) {
+    return g.tile_count ? g.tile_count : (g.n_birds + TILE - 1) / TILE;
+}
 
 hipError_t launch_hash_count(const Geometry &g, const Pools &p, bool wave_agg, hipStream_t s) {
     if (g.n_birds == 0) return hipSuccess;
     if (wave_agg)
-        hipLaunchKernelGGL(k_hash_count<true>, dim3(tiles_of(g.n_birds)), dim3(TILE), 0, s, g, p);
+        hipLaunchKernelGGL(k_hash_count<true>, dim3(launch_tiles(g)), dim3(TILE), 0, s, g, p);
     else
-        hipLaunchKernelGGL(k_hash_count<false>, dim3(tiles_of(g.n_birds)), dim3(TILE), 0, s, g, p);
+        hipLaunchKernelGGL(k_hash_count<false>, dim3(launch_tiles(g)), dim3(TILE), 0, s, g, p);
     return hipGetLastError();
 }
 
@@ -273,7 +277,7 @@ hipError_t launch_scan(const Geometry &g, const Pools &p, hipStream_t s) {
 hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, bool relative_time,
                           hipStream_t s) {
     if (g.n_birds == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter, dim3(tiles_of(g.n_birds)), dim3(TILE), 0, s, g, p, canonical ? 1 : 0,
+    hipLaunchKernelGGL(k_scatter, dim3(launch_tiles(g)), dim3(TILE), 0, s, g, p, canonical ? 1 : 0,
                        relative_time ? 1 : 0);
     if (canonical && g.n_slots)
         hipLaunchKernelGGL(k_canonical, dim3((g.n_slots + 255u) / 256u), dim3(256), 0, s, g, p,

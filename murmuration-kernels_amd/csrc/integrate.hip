@@ -29,10 +29,11 @@ __device__ __forceinline__ uint32_t count_key_i(uint32_t *counts, uint32_t k, bo
 
 template <bool HASH, bool WAVE_AGG>
 __global__ __launch_bounds__(TILE) void k_integrate(Geometry g, Pools p) {
-    const uint32_t si = g.tile_seg ? g.tile_seg[blockIdx.x] : 0u;
+    const uint32_t tile = g.tile_begin + blockIdx.x;
+    const uint32_t si = g.tile_seg ? g.tile_seg[tile] : 0u;
     if (si == NO_SEG) return;
     const Seg &sg = g.segs[si];
-    const uint32_t bird = blockIdx.x * TILE + threadIdx.x;
+    const uint32_t bird = tile * TILE + threadIdx.x;
     const bool active = bird < g.n_birds && bird - sg.bird_base < sg.n;
     uint32_t k = 0;
     if (active) {
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(TILE) void k_integrate(Geometry g, Pools p) {
 
 hipError_t launch_integrate(const Geometry &g, const Pools &p, bool hash, bool wave_agg, hipStream_t s) {
     if (g.n_birds == 0) return hipSuccess;
-    const dim3 grid((g.n_birds + TILE - 1) / TILE), block(TILE);
+    const dim3 grid(g.tile_count ? g.tile_count : (g.n_birds + TILE - 1) / TILE), block(TILE);
     if (!hash)
         hipLaunchKernelGGL((k_integrate<false, false>), grid, block, 0, s, g, p);
     else if (wave_agg)

@@ -69,6 +69,8 @@ struct Geometry {
     uint32_t n_birds;          // bird-index range covered (sum of padded ranges)
     uint32_t n_slots;          // birds actually sorted (sum of n)
     uint32_t n_buckets;        // total buckets over all segments
+    uint32_t tile_begin;       // first tile of a bird-order launch (0 = from the start)
+    uint32_t tile_count;       // tiles of a bird-order launch (0 = up to n_birds)
 };
 
 // Launchers (stream-ordered, no host synchronisation). Each returns the

@@ -85,6 +85,7 @@ _SIGS = {
     "fh_gen_reference": ([_u32, C.c_uint64, _P], C.c_int32),
     "fh_gen_cube": ([_u32, C.c_float, C.c_uint64, _P], C.c_int32),
     "fh_gen_clumps": ([_u32, _u32, C.c_float, C.c_float, C.c_uint64, _P], C.c_int32),
+    "fh_gen_request": ([_u32, C.c_uint64, C.c_float, C.c_float, _P], C.c_int32),
     "fh_create": ([_u32, _P, _P], C.c_int32),
     "fh_destroy": ([_P], C.c_int32),
     "fh_get_params": ([_P, _P, _P], C.c_int32),
@@ -101,6 +102,14 @@ _SIGS = {
     "fh_get_time": ([_P, _P], C.c_int32),
     "fh_read_grid": ([_P, _P, _P, _P, _P], C.c_int32),
     "fh_neighbours": ([_P, _u32, C.c_int, _P, _u32, _P], C.c_int32),
+    "fh_batch_create": ([_u32, _u32, _u32, _P, _P], C.c_int32),
+    "fh_batch_destroy": ([_P], C.c_int32),
+    "fh_batch_add": ([_P, _P, _u32, C.c_float, _P], C.c_int32),
+    "fh_batch_remove": ([_P, _u32], C.c_int32),
+    "fh_batch_download": ([_P, _u32, _P, _P], C.c_int32),
+    "fh_batch_step": ([_P, _u32, _u32, _P], C.c_int32),
+    "fh_batch_info": ([_P, _P, _P, _P, _P], C.c_int32),
+    "fh_flock_bytes": ([_u32, _P], C.c_size_t),
 }
 
 
@@ -198,6 +207,13 @@ def gen_clumps(n: int, clumps: int = 8, sigma: float = 5.0, cube: float = 1000.0
     """Gaussian clumps with aligned headings (10 degree cap) at 10 m/s."""
     w = np.zeros((n, WORDS), np.float32)
     _check(lib().fh_gen_clumps(n, clumps, np.float32(sigma), np.float32(cube), seed, _ptr(w)))
+    return w
+
+
+def gen_request(n: int, seed: int, birds_per_cell: float = 0.64, cell: float = 4.0) -> np.ndarray:
+    """Start state of a served request (a cube at `birds_per_cell` birds per cell)."""
+    w = np.zeros((n, WORDS), np.float32)
+    _check(lib().fh_gen_request(n, seed, np.float32(birds_per_cell), np.float32(cell), _ptr(w)))
     return w
 
 
@@ -314,3 +330,223 @@ class Flock:
         cnt = C.c_uint32(0)
         _check(lib().fh_neighbours(self._h, bird, walk, _ptr(ids), cap, C.addressof(cnt)))
         return ids[: min(cnt.value, cap)].copy() if cnt.value <= cap else ids.copy()
+
+
+def flock_bytes(n: int, buckets: int = 0) -> int:
+    """Device bytes one flock of n birds occupies in a batch (admission unit)."""
+    p = Params()
+    p.buckets = buckets
+    return int(lib().fh_flock_bytes(n, C.addressof(p)))
+
+
+class Batch:
+    """Many independent flocks stepped together, one launch per stage."""
+
+    def __init__(self, bird_capacity: int, bucket_capacity: int, flock_capacity: int, *, cell: float = 0.0,
+                 buckets: int = 0, decide_block: int = 0, birds_per_thread: int = 0, flags: int = 0):
+        p = Params()
+        p.cell, p.buckets, p.decide_block = cell, buckets, decide_block
+        p.birds_per_thread, p.flags = birds_per_thread, flags
+        h = C.c_void_p()
+        _check(lib().fh_batch_create(bird_capacity, bucket_capacity, flock_capacity, C.addressof(p),
+                                     C.addressof(h)))
+        self._h = h
+        self.sizes: dict[int, int] = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().fh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add(self, words: np.ndarray, time: float = 0.0) -> int:
+        w = np.ascontiguousarray(words, np.float32).reshape(-1, WORDS)
+        fid = C.c_uint32(0)
+        _check(lib().fh_batch_add(self._h, _ptr(w), w.shape[0], np.float32(time), C.addressof(fid)))
+        self.sizes[fid.value] = w.shape[0]
+        return fid.value
+
+    def remove(self, flock_id: int) -> None:
+        _check(lib().fh_batch_remove(self._h, flock_id))
+        self.sizes.pop(flock_id, None)
+
+    def download(self, flock_id: int) -> tuple[np.ndarray, float]:
+        w = np.zeros((self.sizes[flock_id], WORDS), np.float32)
+        t = C.c_float(0)
+        _check(lib().fh_batch_download(self._h, flock_id, _ptr(w), C.addressof(t)))
+        return w, float(t.value)
+
+    def step(self, steps: int = 1, warmup: int = 0, timing: bool = False) -> dict | None:
+        out = PassMs() if timing else None
+        _check(lib().fh_batch_step(self._h, steps, warmup, C.addressof(out) if timing else None))
+        return out.as_dict() if timing else None
+
+    def info(self) -> dict:
+        lf, lb, br, db = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0), C.c_size_t(0)
+        _check(lib().fh_batch_info(self._h, C.addressof(lf), C.addressof(lb), C.addressof(br), C.addressof(db)))
+        return {"live_flocks": lf.value, "live_birds": lb.value, "bucket_range": br.value,
+                "device_bytes": db.value}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_double),
+        ("max_active", C.c_uint32),
+        ("bird_capacity", C.c_uint32),
+        ("bucket_capacity", C.c_uint32),
+        ("steps_per_iter", C.c_uint32),
+        ("prep_threads", C.c_uint32),
+        ("keep_states", C.c_uint32),
+        ("birds_per_cell", C.c_float),
+        ("cell_for_density", C.c_float),
+        ("params", Params),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint64),
+        ("seed", C.c_uint64),
+        ("birds", C.c_uint32),
+        ("steps", C.c_uint32),
+        ("status", C.c_int32),
+        ("final_time", C.c_float),
+        ("arrival_s", C.c_double),
+        ("admit_s", C.c_double),
+        ("done_s", C.c_double),
+        ("checksum", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("iterations", C.c_uint64),
+        ("steps", C.c_uint64),
+        ("bird_steps", C.c_double),
+        ("busy_s", C.c_double),
+        ("wall_s", C.c_double),
+        ("peak_bytes", C.c_double),
+        ("peak_active", C.c_uint32),
+        ("queued", C.c_uint32),
+        ("active", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_SIGS.update({
+    "fh_server_config_default": ([_P], C.c_int32),
+    "fh_server_create": ([_P, _P], C.c_int32),
+    "fh_server_destroy": ([_P], C.c_int32),
+    "fh_server_submit": ([_P, _u32, _u32, C.c_uint64, _P], C.c_int32),
+    "fh_server_poll": ([_P, _P, _u32, _P], C.c_int32),
+    "fh_server_drain": ([_P, C.c_double], C.c_int32),
+    "fh_server_stats_get": ([_P, _P], C.c_int32),
+    "fh_server_take_state": ([_P, C.c_uint64, _P, _u32, _P], C.c_int32),
+    "fh_server_replay": ([_P, _P, _P, _P, _P, _u32, _P, _P], C.c_int32),
+})
+
+
+def server_config(**kw) -> ServerConfig:
+    c = ServerConfig()
+    _check(lib().fh_server_config_default(C.addressof(c)))
+    for k, v in kw.items():
+        if k in ("cell", "buckets", "decide_block", "birds_per_thread", "flags"):
+            setattr(c.params, k, v)
+        else:
+            setattr(c, k, v)
+    return c
+
+
+class Server:
+    """Continuous-batching request loop (a worker thread inside the library)."""
+
+    def __init__(self, **kw):
+        self.cfg = server_config(**kw)
+        h = C.c_void_p()
+        _check(lib().fh_server_create(C.addressof(self.cfg), C.addressof(h)))
+        self._h = h
+        self.sizes: dict[int, int] = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().fh_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, birds: int, steps: int, seed: int) -> int:
+        rid = C.c_uint64(0)
+        _check(lib().fh_server_submit(self._h, birds, steps, seed, C.addressof(rid)))
+        self.sizes[rid.value] = birds
+        return rid.value
+
+    def poll(self, cap: int = 1024) -> list[dict]:
+        buf = (Result * cap)()
+        n = C.c_uint32(0)
+        _check(lib().fh_server_poll(self._h, C.addressof(buf), cap, C.addressof(n)))
+        return [buf[i].as_dict() for i in range(n.value)]
+
+    def drain(self, timeout_s: float = 600.0) -> None:
+        _check(lib().fh_server_drain(self._h, timeout_s))
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _check(lib().fh_server_stats_get(self._h, C.addressof(s)))
+        return s.as_dict()
+
+    def take_state(self, rid: int) -> tuple[np.ndarray, float]:
+        w = np.zeros((self.sizes[rid], WORDS), np.float32)
+        t = C.c_float(0)
+        _check(lib().fh_server_take_state(self._h, rid, _ptr(w), w.shape[0], C.addressof(t)))
+        return w, float(t.value)
+
+
+def server_replay(birds, steps, seeds, arrival_s, **kw) -> tuple[list[dict], dict]:
+    """Open-loop replay of a request stream; returns (results, stats)."""
+    cfg = server_config(**kw)
+    b = np.ascontiguousarray(birds, np.uint32)
+    s = np.ascontiguousarray(steps, np.uint32)
+    sd = np.ascontiguousarray(seeds, np.uint64)
+    at = np.ascontiguousarray(arrival_s, np.float64)
+    n = len(b)
+    res = (Result * max(1, n))()
+    st = ServerStats()
+    _check(lib().fh_server_replay(C.addressof(cfg), _ptr(b), _ptr(s), _ptr(sd), _ptr(at), n, C.addressof(res),
+                                  C.addressof(st)))
+    return [res[i].as_dict() for i in range(n)], st.as_dict()
+
+
+def state_checksum(words: np.ndarray) -> int:
+    """FNV-1a 64 of .flk records, as fh_result.checksum."""
+    h = 1469598103934665603
+    for byte in np.ascontiguousarray(words, np.float32).tobytes():
+        h ^= byte
+        h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h

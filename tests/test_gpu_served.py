@@ -1,0 +1,98 @@
+"""The served loop: requests arriving at any time are batched together, and each
+result is the same as stepping that flock alone (bit-identical in canonical
+order); admission honours the slot cap and the HBM budget."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DT = np.float32(0.005)
+
+
+def start_state(fh, birds, seed):
+    return fh.gen_request(birds, seed)
+
+
+def clock_after(steps):
+    t = np.float32(0.0)
+    for _ in range(steps):
+        t = np.float32(t + DT)
+    return t
+
+
+def test_each_result_equals_the_flock_alone(fh):
+    rng = np.random.default_rng(5)
+    reqs = [(int(b), int(s), 100 + i) for i, (b, s) in
+            enumerate(zip(rng.integers(2000, 30_000, 12), rng.integers(10, 60, 12)))]
+    with fh.Server(max_active=5, steps_per_iter=4, keep_states=1, bird_capacity=1 << 20,
+                   bucket_capacity=1 << 23) as srv:
+        ids = [srv.submit(b, s, seed) for b, s, seed in reqs]
+        srv.drain(300)
+        res = {r["id"]: r for r in srv.poll()}
+        stats = srv.stats()
+        assert stats["peak_active"] <= 5 and stats["peak_active"] >= 2
+        for rid, (b, s, seed) in zip(ids, reqs):
+            r = res[rid]
+            assert r["status"] == 0 and r["final_time"] == clock_after(s)
+            assert r["arrival_s"] <= r["admit_s"] <= r["done_s"]
+            got, t = srv.take_state(rid)
+            with fh.Flock.from_words(start_state(fh, b, seed), flags=fh.FLAG_CANONICAL) as f:
+                f.step(s)
+                want, tw = f.download()
+            assert t == tw
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), rid
+
+
+def test_served_request_matches_the_oracle(fh, fr):
+    with fh.Server(max_active=4, keep_states=1, bird_capacity=1 << 20, bucket_capacity=1 << 22) as srv:
+        other = srv.submit(20_000, 30, 7)
+        rid = srv.submit(3000, 10, 9)
+        srv.drain(120)
+        got, t = srv.take_state(rid)
+    cpu, tc = fr.step(start_state(fh, 3000, 9), 0.0, 10)
+    d = fr.diff(cpu, got)
+    assert t == tc and d["ok"], d
+
+
+def test_requests_are_validated(fh):
+    with fh.Server(max_active=2, bird_capacity=100_000, bucket_capacity=1 << 20,
+                   hbm_budget_bytes=float(fh.flock_bytes(20_000))) as srv:
+        with pytest.raises(fh.FlockHipError) as e:
+            srv.submit(0, 10, 1)
+        assert e.value.status == fh.ERR_INVALID
+        with pytest.raises(fh.FlockHipError) as e:
+            srv.submit(100, 0, 1)
+        assert e.value.status == fh.ERR_INVALID
+        with pytest.raises(fh.FlockHipError) as e:
+            srv.submit(90_000, 10, 1)  # alone over the HBM budget
+        assert e.value.status == fh.ERR_BUDGET
+
+
+def test_budget_serialises_admission(fh):
+    budget = float(fh.flock_bytes(20_000)) * 1.5  # room for one 20k flock at a time
+    with fh.Server(max_active=8, hbm_budget_bytes=budget, bird_capacity=1 << 20,
+                   bucket_capacity=1 << 22) as srv:
+        for i in range(4):
+            srv.submit(20_000, 12, i)
+        srv.drain(120)
+        res = srv.poll()
+        st = srv.stats()
+    assert len(res) == 4 and all(r["status"] == 0 for r in res)
+    assert st["peak_active"] == 1 and st["peak_bytes"] <= budget
+
+
+def test_open_loop_replay(fh):
+    rng = np.random.default_rng(11)
+    n = 40
+    birds = np.exp(rng.uniform(np.log(2000), np.log(20_000), n)).astype(np.uint32)
+    steps = rng.integers(20, 80, n).astype(np.uint32)
+    seeds = np.arange(n, dtype=np.uint64) + 1000
+    arrivals = np.cumsum(rng.exponential(1 / 200.0, n))
+    res, st = fh.server_replay(birds, steps, seeds, arrivals, max_active=16, bird_capacity=1 << 21,
+                               bucket_capacity=1 << 24)
+    assert all(r["status"] == 0 for r in res)
+    lat = np.array([r["done_s"] - r["arrival_s"] for r in res])
+    assert np.all(lat > 0)
+    assert abs(st["bird_steps"] - float(np.sum(birds.astype(np.float64) * steps))) < 1.0
+    print("replay p50 %.1f ms p99 %.1f ms, %.3g bird-steps/s busy" %
+          (1e3 * np.percentile(lat, 50), 1e3 * np.percentile(lat, 99), st["bird_steps"] / st["busy_s"]))
