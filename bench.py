@@ -75,6 +75,14 @@ def main() -> int:
     ap.add_argument("--flags", type=int, default=4, help="fh flags for the timed run (default: graph)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2500)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-reference", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--flocks", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--slots", type=int, default=64)
+    ap.add_argument("--load", type=float, default=0.7, help="offered load / batched throughput")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -147,8 +155,122 @@ def main() -> int:
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(words, n, args.cpu_sample)
+    del words
+    if not args.no_reference:
+        out["reference_protocol"] = reference_leg(args)
+    if not args.no_batched:
+        out["batched"] = batched_leg(args)
+        if not args.no_served:
+            out["served"] = served_leg(args, out["batched"])
     print(json.dumps(out), flush=True)
     return 0
+
+
+def reference_leg(args) -> dict:
+    """The reference's own protocol (docs/SPEC.md:48-58): Flock::scatter at
+    density 0.5 birds/m^3 (32 birds per 4 m cell), +x heading, seed 1, 20
+    warm-up + 20 timed steps, mean GPU ms per pass from events; the rows of
+    docs/results/rtx4000ada-2026-10-21/bench-table-4000ada.txt."""
+    import flock_hip as fh
+
+    rows = {}
+    for n in (100_000, 1_000_000):
+        w = fh.gen_reference(n, 1)
+        with fh.Flock.from_words(w, flags=fh.FLAG_UNFUSED) as f:
+            f.step(20)
+            ms = f.step(20, 0, timing=True)
+        rows[str(n)] = {k: ms[k] for k in ("grid_histogram", "grid_scan", "grid_scatter", "decide", "integrate",
+                                           "step_total")}
+        rows[str(n)]["bird_steps_per_s"] = n / (ms["step_total"] * 1e-3)
+    rows["published_rtx4000ada_decide_ms"] = {"100000": 1.24, "1000000": 9.46}
+    rows["published_rtx4000ada_step_total_ms"] = {"100000": 1.276, "1000000": 9.979}
+    return rows
+
+
+def batched_leg(args) -> dict:
+    """256 flocks of log-uniform sizes 1k-200k at 0.64 birds per cell: all of
+    them in one batch (one launch per stage, hipGraph, autotuned) against the
+    same flocks stepped one flock per launch sequence (each its own graph)."""
+    import flock_hip as fh
+    from flock_hip import autotune, workloads
+
+    t0 = time.perf_counter()
+    starts = workloads.mix(args.flocks, seed=args.seed)
+    sizes = [int(w.shape[0]) for w in starts]
+    total = sum(sizes)
+    tuned = autotune.tune(starts, "batch", flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, warmup=2, steps=5)
+    prm = tuned["params"]
+    buckets = sum(autotune._buckets(s) for s in sizes)
+    with fh.Batch(total + 256 * len(starts), buckets + 4096 * len(starts), len(starts),
+                  flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, **prm) as b:
+        for w in starts:
+            b.add(w)
+        b.step(3)
+        tb = time.perf_counter()
+        b.step(args.batch_steps)
+        batched_s = time.perf_counter() - tb
+    flocks = [fh.Flock.from_words(w, flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL) for w in starts]
+    for f in flocks:
+        f.step(3)
+    tp = time.perf_counter()
+    for f in flocks:
+        f.step(args.batch_steps)
+    plain_s = time.perf_counter() - tp
+    for f in flocks:
+        f.close()
+    bs = total * args.batch_steps
+    return {
+        "workload": f"{len(starts)} flocks, sizes log-uniform 1e3..2e5 (seed {args.seed}), {total} birds, "
+                    f"0.64 birds per 4 m cell, {args.batch_steps} steps, canonical order",
+        "birds": total,
+        "steps": args.batch_steps,
+        "batched_bird_steps_per_s": bs / batched_s,
+        "plain_bird_steps_per_s": bs / plain_s,
+        "speedup": plain_s / batched_s,
+        "batched_ms_per_step": batched_s * 1e3 / args.batch_steps,
+        "plain_ms_per_step": plain_s * 1e3 / args.batch_steps,
+        "autotuned": prm,
+        "autotune_trials": len(tuned["trials"]),
+        "leg_seconds": time.perf_counter() - t0,
+    }
+
+
+def served_leg(args, batched: dict) -> dict:
+    """A Poisson stream of requests (birds log-uniform 2e3..5e5, steps uniform
+    50..500, own seeds) through the continuous-batching server: admission by
+    HBM bytes against a 200 GB budget and a slot cap, state resident on the
+    device. The rate is set to `load` x the batched throughput."""
+    import flock_hip as fh
+    from flock_hip import workloads
+
+    mean_bird_steps = 90_200 * 275.0  # E[birds] x E[steps] of the request mix
+    rate = args.load * batched["batched_bird_steps_per_s"] / mean_bird_steps
+    req = workloads.stream(args.requests, rate, seed=args.seed)
+    t0 = time.perf_counter()
+    prm = batched["autotuned"]
+    res, st = fh.server_replay(req["birds"], req["steps"], req["seeds"], req["arrival_s"],
+                               max_active=args.slots, hbm_budget_bytes=200e9, steps_per_iter=8,
+                               flags=fh.FLAG_CANONICAL, **prm)
+    wall = time.perf_counter() - t0
+    ok = [r for r in res if r["status"] == 0]
+    lat = np.array([r["done_s"] - r["arrival_s"] for r in ok])
+    span = max(r["done_s"] for r in ok) - min(r["arrival_s"] for r in ok)
+    total_bs = float(np.sum(req["birds"].astype(np.float64) * req["steps"]))
+    return {
+        "workload": f"{args.requests} requests, Poisson {rate:.1f}/s, birds log-uniform 2e3..5e5, "
+                    f"steps uniform 50..500, slot cap {args.slots}, 200 GB HBM budget "
+                    f"({fh.flock_bytes(1) // 256} B per bird + 8 B per bucket)",
+        "rate_per_s": rate,
+        "completed": len(ok),
+        "bird_steps_per_s": total_bs / span,
+        "bird_steps_per_busy_s": st["bird_steps"] / st["busy_s"],
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "peak_active": st["peak_active"],
+        "peak_hbm_bytes": st["peak_bytes"],
+        "iterations": st["iterations"],
+        "leg_seconds": wall,
+    }
 
 
 if __name__ == "__main__":

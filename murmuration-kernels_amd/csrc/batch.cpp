@@ -78,6 +78,9 @@ static void refresh_geometry(fh_batch *b) {
 }
 
 static fh_status push_tables(fh_batch *b) {
+    // The device advances each clock by DT per step; the host mirror in
+    // times[] does the same f32 additions, so it is the current value.
+    for (size_t f = 0; f < b->segs.size(); ++f) b->segs[f].time = b->times[f];
     if (!b->segs.empty())
         FH_HIP(hipMemcpyAsync(b->d_segs, b->segs.data(), b->segs.size() * sizeof(Seg), hipMemcpyHostToDevice,
                               b->stream));
