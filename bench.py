@@ -83,7 +83,19 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--load", type=float, default=0.7, help="offered load / batched throughput")
+    ap.add_argument("--copy-cal", action="store_true",
+                    help="only run a 256 MiB device-to-device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_cal:
+        import torch
+
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": x.numel() * 4, "launches": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(flocks would shard by HBM bytes across GPUs, see the design notes)", file=sys.stderr)
