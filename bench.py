@@ -77,6 +77,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=2500)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-reference", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="plain leg at the default tunables")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--flocks", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=100)
@@ -108,20 +109,30 @@ def main() -> int:
     words = fh.gen_cube(n, side, args.seed)
     dev = fh.device_info()
 
-    # Timed run: fused step replayed from a hipGraph; wall clock over K steps,
-    # synchronised on both sides, inputs resident in HBM.
-    with fh.Flock.from_words(words, flags=args.flags) as f:
-        buckets = f.buckets
-        f.step(args.warmup)
-        f.sync()
-        t0 = time.perf_counter()
-        f.step(args.steps)
-        f.sync()
-        wall = time.perf_counter() - t0
-    ms_per_step = wall * 1000.0 / args.steps
+    # Tunables of the plain step, picked by the autotuner on this workload.
+    prm, tuned = {}, None
+    if not args.no_tune:
+        from flock_hip import autotune
+
+        tuned = autotune.tune([words], "plain", flags=args.flags, warmup=3, steps=8)
+        prm = tuned["params"]
+
+    def timed_run(params: dict) -> tuple[float, int]:
+        """Fused step replayed from a hipGraph; wall clock over K steps,
+        synchronised on both sides, inputs resident in HBM."""
+        with fh.Flock.from_words(words, flags=args.flags, **params) as f:
+            f.step(args.warmup)
+            f.sync()
+            t0 = time.perf_counter()
+            f.step(args.steps)
+            f.sync()
+            return (time.perf_counter() - t0) * 1000.0 / args.steps, f.buckets
+
+    ms_per_step, buckets = timed_run(prm)
+    default_ms = timed_run({})[0] if prm else ms_per_step
 
     # Per-pass breakdown with hipEvents on the library's stream (same start).
-    with fh.Flock.from_words(words, flags=args.flags & ~fh.FLAG_GRAPH) as f:
+    with fh.Flock.from_words(words, flags=args.flags & ~fh.FLAG_GRAPH, **prm) as f:
         f.step(args.warmup)
         passes = f.step(args.steps, 0, timing=True)
     due_last = int(passes["due_last"])  # birds that decided in the last timed step
@@ -150,10 +161,12 @@ def main() -> int:
                         f"isotropic headings 10+-1 m/s, seed {args.seed}",
             "birds": n,
             "buckets": buckets,
-            "cell_m": 4.0,
             "flags": args.flags,
+            "params": {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, **prm},
+            "autotuned": tuned is not None,
             "device": dev["name"],
         },
+        "ms_per_step_default_params": default_ms,
         "decide_ms": dec_ms,
         "passes_ms": {k: passes[k] for k in ("grid_histogram", "grid_scan", "grid_scatter", "decide",
                                             "integrate", "step_total")},

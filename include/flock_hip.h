@@ -52,10 +52,13 @@ enum {
 typedef struct fh_params {
     float cell;                /* grid cell edge [m]; 0 -> 4 (params.rs:58 GRID_CELL) */
     uint32_t buckets;          /* hash buckets; 0 -> buckets_for(n) (cuda/flock.cu:368-372); else a power of two */
-    uint32_t decide_block;     /* decide workgroup size, multiple of 64 in [64, 1024]; 0 -> 256 */
+    uint32_t decide_block;     /* decide workgroup size: 64, 128, 256 or 512; 0 -> 256 */
     uint32_t birds_per_thread; /* decide scans block * this sorted slots per workgroup; 0 -> 8 */
     uint32_t flags;            /* FH_FLAG_* */
-    uint32_t reserved[3];
+    uint32_t decide_group;     /* lanes sharing one bird's walk: 1, 2, 4 or 8; 0 -> 1. With 1 the
+                                  sums follow the walk order exactly; with more, each lane sums
+                                  every G-th cell and the partial sums are added */
+    uint32_t reserved[2];
 } fh_params;
 
 /* Within a bucket, order birds by ascending index (a stable counting sort):

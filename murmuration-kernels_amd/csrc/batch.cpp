@@ -114,7 +114,8 @@ static fh_status batch_step_launches(fh_batch *b, hipEvent_t *ev) {
     if (ev) FH_HIP(hipEventRecord(ev[2], b->stream));
     FH_HIP(launch_scatter(b->geo, b->P, (b->prm.flags & FH_FLAG_CANONICAL) != 0, true, b->stream));
     if (ev) FH_HIP(hipEventRecord(ev[3], b->stream));
-    FH_HIP(launch_decide(b->geo, b->P, 0.0f, b->prm.decide_block, b->prm.birds_per_thread, b->stream));
+    FH_HIP(launch_decide(b->geo, b->P, 0.0f, b->prm.decide_block, b->prm.birds_per_thread, b->prm.decide_group,
+                         b->stream));
     if (ev) FH_HIP(hipEventRecord(ev[4], b->stream));
     FH_HIP(launch_integrate(b->geo, b->P, true, (b->prm.flags & FH_FLAG_WAVE_AGG) != 0, b->stream));
     b->counts_dirty = true;
@@ -252,22 +253,9 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
         bucket_capacity > (1u << 30))
         return set_error(FH_ERR_INVALID, "fh_batch_create: bad capacity");
     fh_params prm;
-    fh_params_default(&prm);
-    if (params) {
-        if (params->cell != 0.0f) prm.cell = params->cell;
-        prm.buckets = params->buckets;
-        if (params->decide_block) prm.decide_block = params->decide_block;
-        if (params->birds_per_thread) prm.birds_per_thread = params->birds_per_thread;
-        prm.flags = params->flags;
-    }
-    const uint32_t blk = prm.decide_block;
-    if (blk != 64 && blk != 128 && blk != 256 && blk != 512 && blk != 1024)
-        return set_error(FH_ERR_INVALID, "fh_batch_create: decide_block %u", blk);
-    if (prm.birds_per_thread > 64 || blk * prm.birds_per_thread > 16384)
-        return set_error(FH_ERR_INVALID, "fh_batch_create: decide chunk too large");
-    if (!(prm.cell >= 0.5f) || !(prm.cell <= 1000.0f))
-        return set_error(FH_ERR_INVALID, "fh_batch_create: cell %g outside [0.5, 1000] m", prm.cell);
-    fh_status st = require_device();
+    fh_status st = resolve_params(params, &prm, "fh_batch_create");
+    if (st) return st;
+    st = require_device();
     if (st) return st;
 
     fh_batch *b = new fh_batch();

@@ -58,37 +58,47 @@ __device__ __forceinline__ Walk make_walk(f3 p, float r, const Seg &sg) {
 }
 
 // Cells of the box are visited in z-y-x order (x fastest), as the nested
-// loops of cuda/flock.cu:209-211. They are taken PREFETCH at a time: the
-// bucket ranges of all PREFETCH cells are loaded together, then the first
-// candidate of each non-empty bucket, so one lane has up to 2 * PREFETCH
-// independent loads in flight instead of a chain of dependent ones. The visit
-// order of candidates is unchanged.
-constexpr int PREFETCH = 8;
-
-template <class Visit>
+// loops of cuda/flock.cu:209-211. A lane visits every G-th cell starting at
+// `first` (G = 1: every cell). Its cells are taken PF at a time: the bucket
+// ranges of all PF cells are loaded together, then the first candidate of
+// each non-empty bucket, so a lane has up to 2 * PF independent loads in
+// flight instead of a chain of dependent ones. Within a lane the visit order
+// is the walk order.
+template <int G, int PF, class Visit>
 __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const Pools &p, uint32_t n_slots,
-                                           uint32_t me, Visit &&visit) {
+                                           uint32_t me, int32_t first, Visit &&visit) {
     if (!w.ok) return;
     const int32_t nx = w.hi[0] - w.lo[0] + 1, ny = w.hi[1] - w.lo[1] + 1, nz = w.hi[2] - w.lo[2] + 1;
     const int32_t ncell = nx * ny * nz;
-    int32_t cx0 = w.lo[0], cy0 = w.lo[1], cz0 = w.lo[2];  // cell of index `base`
-    for (int32_t base = 0; base < ncell; base += PREFETCH) {
-        int32_t cx[PREFETCH], cy[PREFETCH], cz[PREFETCH];
-        uint32_t s0[PREFETCH], s1[PREFETCH];
-        {
-            int32_t x = cx0, y = cy0, z = cz0;
-#pragma unroll
-            for (int k = 0; k < PREFETCH; ++k) {
-                cx[k] = x;
-                cy[k] = y;
-                cz[k] = z;
-                s0[k] = 0u;
-                s1[k] = 0u;
-                if (base + k < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2)) {
-                    const uint32_t b = sg.bucket_base + (bucket_hash(x, y, z) & sg.mask);
-                    s0[k] = p.offsets[b];
-                    s1[k] = p.offsets[b + 1];
+    int32_t x = w.lo[0], y = w.lo[1], z = w.lo[2];
+    auto advance = [&](int32_t steps) {
+        for (int32_t s = 0; s < steps; ++s)
+            if (++x > w.hi[0]) {
+                x = w.lo[0];
+                if (++y > w.hi[1]) {
+                    y = w.lo[1];
+                    ++z;
                 }
+            }
+    };
+    advance(first);
+    for (int32_t base = first; base < ncell; base += G * PF) {
+        int32_t cx[PF], cy[PF], cz[PF];
+        uint32_t s0[PF], s1[PF];
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            cx[k] = x;
+            cy[k] = y;
+            cz[k] = z;
+            s0[k] = 0u;
+            s1[k] = 0u;
+            if (base + k * G < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2)) {
+                const uint32_t b = sg.bucket_base + (bucket_hash(x, y, z) & sg.mask);
+                s0[k] = p.offsets[b];
+                s1[k] = p.offsets[b + 1];
+            }
+#pragma unroll
+            for (int s = 0; s < G; ++s)
                 if (++x > w.hi[0]) {
                     x = w.lo[0];
                     if (++y > w.hi[1]) {
@@ -96,21 +106,17 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
                         ++z;
                     }
                 }
-            }
-            cx0 = x;
-            cy0 = y;
-            cz0 = z;
         }
-        float4 first[PREFETCH];
+        float4 head[PF];
 #pragma unroll
-        for (int k = 0; k < PREFETCH; ++k) {
+        for (int k = 0; k < PF; ++k) {
             s1[k] = min(s1[k], n_slots);
-            first[k] = s0[k] < s1[k] ? p.spos[s0[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            head[k] = s0[k] < s1[k] ? p.spos[s0[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
-        for (int k = 0; k < PREFETCH; ++k) {
+        for (int k = 0; k < PF; ++k) {
             for (uint32_t t = s0[k]; t < s1[k]; ++t) {
-                const float4 c = t == s0[k] ? first[k] : p.spos[t];
+                const float4 c = t == s0[k] ? head[k] : p.spos[t];
                 if (__float_as_uint(c.w) == me) continue;
                 if (cell_coord(c.x, sg.inv_cell) != cx[k] || cell_coord(c.y, sg.inv_cell) != cy[k] ||
                     cell_coord(c.z, sg.inv_cell) != cz[k])
@@ -124,53 +130,92 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
     }
 }
 
-__device__ __forceinline__ Decision decide_bird(const Pools &p, const Seg &sg, uint32_t n_slots, uint32_t me,
-                                                f3 pos, f3 vel, float r) {
-    const f3 f_i = forward_of(vel);
-    f3 sep = mk3(0.f, 0.f, 0.f), coh = sep, align = sep, csum = sep;
-    uint32_t n_all = 0, n_star = 0, n_align = 0, n_g = 0;
+// The running sums of one bird's walk (model.rs:62-115).
+struct Acc {
+    f3 sep, coh, align, csum;
+    uint32_t n_all, n_star, n_align, n_g;
+};
+
+template <int G, int PF>
+__device__ __forceinline__ Acc walk_sums(const Pools &p, const Seg &sg, uint32_t n_slots, uint32_t me, f3 pos,
+                                         f3 f_i, float r, int32_t first) {
+    Acc a;
+    a.sep = mk3(0.f, 0.f, 0.f);
+    a.coh = a.sep;
+    a.align = a.sep;
+    a.csum = a.sep;
+    a.n_all = a.n_star = a.n_align = a.n_g = 0;
     const Walk w = make_walk(pos, r, sg);
-    walk_cells(w, sg, p, n_slots, me, [&](uint32_t t, f3 d, float d2) {
+    walk_cells<G, PF>(w, sg, p, n_slots, me, first, [&](uint32_t t, f3 d, float d2) {
         const float dist = sqrtf(d2);
         const f3 dhat = d / fmaxf(dist, 1e-6f);
         // walk 2 (model.rs:104-115): csum += d / max(sqrt(d2), 1e-6)
-        csum = csum + dhat;
-        ++n_g;
+        a.csum = a.csum + dhat;
+        ++a.n_g;
         if (d2 > w.r2) return;
         // walk 1 (model.rs:67-93)
-        ++n_all;
+        ++a.n_all;
         const float g = dist <= R_H ? 1.0f : expf(-((dist - R_H) * (dist - R_H)) / SEP_DENOM);
-        sep = sep - dhat * g;
+        a.sep = a.sep - dhat * g;
         if (dist > R_H) {
-            coh = coh + dhat;
-            ++n_star;
+            a.coh = a.coh + dhat;
+            ++a.n_star;
         }
         if (dot3(f_i, dhat) >= BLIND_COS) {
             const float4 vj = p.svel[t];
-            align = align + forward_of(mk3(vj.x, vj.y, vj.z));
-            ++n_align;
+            a.align = a.align + forward_of(mk3(vj.x, vj.y, vj.z));
+            ++a.n_align;
         }
     });
-    // combine (model.rs:95-137, cuda/flock.cu:242-283)
+    return a;
+}
+
+__device__ __forceinline__ f3 shfl_xor3(f3 v, int m) {
+    return mk3(__shfl_xor(v.x, m), __shfl_xor(v.y, m), __shfl_xor(v.z, m));
+}
+
+// Sums the partial walks of the G lanes of a group (butterfly; every lane
+// ends with bit-identical totals because float addition is commutative).
+template <int G>
+__device__ __forceinline__ void reduce_group(Acc &a) {
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) {
+        a.sep = a.sep + shfl_xor3(a.sep, m);
+        a.coh = a.coh + shfl_xor3(a.coh, m);
+        a.align = a.align + shfl_xor3(a.align, m);
+        a.csum = a.csum + shfl_xor3(a.csum, m);
+        a.n_all += __shfl_xor(a.n_all, m);
+        a.n_star += __shfl_xor(a.n_star, m);
+        a.n_align += __shfl_xor(a.n_align, m);
+        a.n_g += __shfl_xor(a.n_g, m);
+    }
+}
+
+// combine (model.rs:95-137, cuda/flock.cu:242-283)
+__device__ __forceinline__ Decision finish(const Acc &a, f3 pos, f3 vel, f3 f_i, float r) {
     f3 force = mk3(0.f, 0.f, 0.f);
-    if (n_all > 0) force = force + sep * (W_SEP / (float)n_all);
-    const float centrality = n_g > 0 ? sqrtf(dot3(csum, csum)) / (float)n_g : 0.0f;
-    if (n_star > 0) force = force + normalize_or_zero(coh) * (W_COH * centrality);
-    const f3 a = n_align > 0 ? align / (float)n_align - f_i : mk3(0.f, 0.f, 0.f);
-    force = force + a * W_ALIGN;
+    if (a.n_all > 0) force = force + a.sep * (W_SEP / (float)a.n_all);
+    const float centrality = a.n_g > 0 ? sqrtf(dot3(a.csum, a.csum)) / (float)a.n_g : 0.0f;
+    if (a.n_star > 0) force = force + normalize_or_zero(a.coh) * (W_COH * centrality);
+    const f3 al = a.n_align > 0 ? a.align / (float)a.n_align - f_i : mk3(0.f, 0.f, 0.f);
+    force = force + al * W_ALIGN;
     force = force + mk3(0.f, 0.f, 1.f) * (W_ROOST_V * clampf_((Z0 - pos.z) / H_ROOST_V, -1.0f, 1.0f));
     const float speed = sqrtf(dot3(vel, vel));
     force = force + f_i * ((MASS / TAU) * (V0 - speed));
     Decision dcs;
     dcs.force = force;
-    dcs.radius = radius_step(r, n_all);
-    dcs.n_all = n_all;
+    dcs.radius = radius_step(r, a.n_all);
+    dcs.n_all = a.n_all;
     dcs.centrality = centrality;
     return dcs;
 }
 
-template <int BLOCK>
+// BLOCK lanes per workgroup, G lanes per due bird. With G = 1 each bird's sums
+// follow the walk order exactly (the order of cuda/flock.cu:209-238); with
+// G > 1 lane j sums cells j, j+G, ... and the G partial sums are added.
+template <int BLOCK, int G>
 __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float time, uint32_t chunk) {
+    constexpr int PF = G >= 4 ? 4 : 8;
     extern __shared__ uint32_t due_slots[];
     __shared__ uint32_t n_due;
     if (threadIdx.x == 0) n_due = 0;
@@ -182,22 +227,46 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
     __syncthreads();
     const uint32_t nd = n_due;
     if (threadIdx.x == 0 && nd) atomicAdd(p.due_count, nd);
-    for (uint32_t k = threadIdx.x; k < nd; k += BLOCK) {
-        const uint32_t s = due_slots[k];
-        const float4 q = p.spos[s];
-        const uint32_t me = __float_as_uint(q.w);
-        if (me >= g.n_birds) {
-            atomicOr(p.err, ERR_SLOT);
-            continue;
+    constexpr uint32_t groups = BLOCK / G;
+    const uint32_t lane_g = threadIdx.x % G;
+    for (uint32_t base = 0; base < nd; base += groups) {  // same trip count for every lane
+        const uint32_t k = base + threadIdx.x / G;
+        bool active = k < nd;
+        uint32_t me = 0;
+        f3 pos = mk3(0.f, 0.f, 0.f), vel = pos, f_i = pos;
+        float r = 0.0f;
+        const Seg *sg = &g.segs[0];
+        Acc a;
+        a.sep = mk3(0.f, 0.f, 0.f);
+        a.coh = a.sep;
+        a.align = a.sep;
+        a.csum = a.sep;
+        a.n_all = a.n_star = a.n_align = a.n_g = 0;
+        if (active) {
+            const uint32_t s = due_slots[k];
+            const float4 q = p.spos[s];
+            me = __float_as_uint(q.w);
+            if (me >= g.n_birds) {
+                if (lane_g == 0) atomicOr(p.err, ERR_SLOT);
+                active = false;
+            } else {
+                sg = &g.segs[g.tile_seg ? g.tile_seg[me / TILE] : 0u];
+                const float4 v = p.svel[s];
+                pos = mk3(q.x, q.y, q.z);
+                vel = mk3(v.x, v.y, v.z);
+                f_i = forward_of(vel);
+                r = p.steer4[me].w;
+                a = walk_sums<G, PF>(p, *sg, g.n_slots, me, pos, f_i, r, (int32_t)lane_g);
+            }
         }
-        const Seg &sg = g.segs[g.tile_seg ? g.tile_seg[me / TILE] : 0u];
-        const float4 v = p.svel[s];
-        const float r = p.steer4[me].w;
-        const Decision d = decide_bird(p, sg, g.n_slots, me, mk3(q.x, q.y, q.z), mk3(v.x, v.y, v.z), r);
-        p.steer4[me] = make_float4(d.force.x, d.force.y, d.force.z, d.radius);
-        p.pos4[me].w = p.pos4[me].w + DU;
-        p.nb[me] = d.n_all;
-        p.cen[me] = d.centrality;
+        if (G > 1) reduce_group<G>(a);
+        if (active && lane_g == 0) {
+            const Decision d = finish(a, pos, vel, f_i, r);
+            p.steer4[me] = make_float4(d.force.x, d.force.y, d.force.z, d.radius);
+            p.pos4[me].w = p.pos4[me].w + DU;
+            p.nb[me] = d.n_all;
+            p.cen[me] = d.centrality;
+        }
     }
 }
 
@@ -211,7 +280,7 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
     const float r = p.steer4[bird].w;
     const Walk w = make_walk(mk3(q.x, q.y, q.z), r, sg);
     uint32_t c = 0;
-    walk_cells(w, sg, p, g.n_slots, bird, [&](uint32_t t, f3, float d2) {
+    walk_cells<1, 8>(w, sg, p, g.n_slots, bird, 0, [&](uint32_t t, f3, float d2) {
         if (walk == 1 && d2 > w.r2) return;
         if (c < cap) ids[c] = __float_as_uint(p.spos[t].w);
         ++c;
@@ -219,25 +288,39 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
     *count = c;
 }
 
-template <int BLOCK>
+template <int BLOCK, int G>
 static void launch_decide_block(const Geometry &g, const Pools &p, float time, uint32_t chunk, hipStream_t s) {
     const uint32_t blocks = (g.n_slots + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_decide<BLOCK>, dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s, g, p, time,
+    hipLaunchKernelGGL((k_decide<BLOCK, G>), dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s, g, p, time,
                        chunk);
 }
 
-hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
-                         uint32_t birds_per_thread, hipStream_t s) {
-    if (g.n_slots == 0) return hipSuccess;
-    const uint32_t chunk = block * birds_per_thread;
-    switch (block) {
-        case 64: launch_decide_block<64>(g, p, time, chunk, s); break;
-        case 128: launch_decide_block<128>(g, p, time, chunk, s); break;
-        case 256: launch_decide_block<256>(g, p, time, chunk, s); break;
-        case 512: launch_decide_block<512>(g, p, time, chunk, s); break;
-        case 1024: launch_decide_block<1024>(g, p, time, chunk, s); break;
+template <int BLOCK>
+static hipError_t launch_decide_group(const Geometry &g, const Pools &p, float time, uint32_t chunk, uint32_t group,
+                                      hipStream_t s) {
+    switch (group) {
+        case 1: launch_decide_block<BLOCK, 1>(g, p, time, chunk, s); break;
+        case 2: launch_decide_block<BLOCK, 2>(g, p, time, chunk, s); break;
+        case 4: launch_decide_block<BLOCK, 4>(g, p, time, chunk, s); break;
+        case 8: launch_decide_block<BLOCK, 8>(g, p, time, chunk, s); break;
         default: return hipErrorInvalidValue;
     }
+    return hipSuccess;
+}
+
+hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
+                         uint32_t birds_per_thread, uint32_t group, hipStream_t s) {
+    if (g.n_slots == 0) return hipSuccess;
+    const uint32_t chunk = block * birds_per_thread;
+    hipError_t e;
+    switch (block) {
+        case 64: e = launch_decide_group<64>(g, p, time, chunk, group, s); break;
+        case 128: e = launch_decide_group<128>(g, p, time, chunk, group, s); break;
+        case 256: e = launch_decide_group<256>(g, p, time, chunk, group, s); break;
+        case 512: e = launch_decide_group<512>(g, p, time, chunk, group, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
@@ -249,7 +332,7 @@ hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, i
 
 hipError_t decide_kernel_attributes(const char *name, hipFuncAttributes *out) {
     if (!__builtin_strcmp(name, "decide"))
-        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_decide<256>));
+        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_decide<256, 1>));
     return hipErrorInvalidValue;
 }
 

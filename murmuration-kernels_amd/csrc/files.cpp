@@ -53,6 +53,34 @@ fh_status require_device() {
     return FH_OK;
 }
 
+fh_status resolve_params(const fh_params *in, fh_params *out, const char *who) {
+    fh_params prm;
+    fh_params_default(&prm);
+    if (in) {
+        if (in->cell != 0.0f) prm.cell = in->cell;
+        prm.buckets = in->buckets;
+        if (in->decide_block) prm.decide_block = in->decide_block;
+        if (in->birds_per_thread) prm.birds_per_thread = in->birds_per_thread;
+        if (in->decide_group) prm.decide_group = in->decide_group;
+        prm.flags = in->flags;
+    }
+    if (!(prm.cell >= 0.5f) || !(prm.cell <= 1000.0f))
+        return set_error(FH_ERR_INVALID, "%s: cell %g outside [0.5, 1000] m", who, prm.cell);
+    if (prm.buckets && (prm.buckets & (prm.buckets - 1)))
+        return set_error(FH_ERR_INVALID, "%s: buckets %u is not a power of two", who, prm.buckets);
+    if (prm.buckets > (1u << 28)) return set_error(FH_ERR_INVALID, "%s: buckets %u > 2^28", who, prm.buckets);
+    const uint32_t blk = prm.decide_block;
+    if (blk != 64 && blk != 128 && blk != 256 && blk != 512)
+        return set_error(FH_ERR_INVALID, "%s: decide_block %u not in {64,128,256,512}", who, blk);
+    if (prm.birds_per_thread > 64 || blk * prm.birds_per_thread > 16384)
+        return set_error(FH_ERR_INVALID, "%s: decide chunk %u x %u too large", who, blk, prm.birds_per_thread);
+    const uint32_t grp = prm.decide_group;
+    if (grp != 1 && grp != 2 && grp != 4 && grp != 8)
+        return set_error(FH_ERR_INVALID, "%s: decide_group %u not in {1,2,4,8}", who, grp);
+    *out = prm;
+    return FH_OK;
+}
+
 // SplitMix64 (Steele, Lea, Flood 2014), as crates/flock-ref/src/state.rs:58-73.
 struct SplitMix {
     uint64_t s;
@@ -276,6 +304,7 @@ fh_status fh_params_default(fh_params *p) {
     p->decide_block = 256;
     p->birds_per_thread = 8;
     p->flags = 0;
+    p->decide_group = 1;
     return FH_OK;
 }
 

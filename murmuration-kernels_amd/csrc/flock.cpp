@@ -79,7 +79,8 @@ static fh_status do_scatter(fh_flock *h, bool relative) {
 }
 
 static fh_status do_decide(fh_flock *h, float time_arg) {
-    FH_HIP(launch_decide(h->geo, h->P, time_arg, h->prm.decide_block, h->prm.birds_per_thread, h->stream));
+    FH_HIP(launch_decide(h->geo, h->P, time_arg, h->prm.decide_block, h->prm.birds_per_thread,
+                         h->prm.decide_group, h->stream));
     h->stage = Stage::Decided;
     return FH_OK;
 }
@@ -156,26 +157,10 @@ fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
     if (!out) return set_error(FH_ERR_INVALID, "fh_create: NULL out");
     *out = nullptr;
     fh_params prm;
-    fh_params_default(&prm);
-    if (params) {
-        if (params->cell != 0.0f) prm.cell = params->cell;
-        prm.buckets = params->buckets;
-        if (params->decide_block) prm.decide_block = params->decide_block;
-        if (params->birds_per_thread) prm.birds_per_thread = params->birds_per_thread;
-        prm.flags = params->flags;
-    }
-    if (!(prm.cell >= 0.5f) || !(prm.cell <= 1000.0f))
-        return set_error(FH_ERR_INVALID, "fh_create: cell %g outside [0.5, 1000] m", prm.cell);
-    if (prm.buckets && (prm.buckets & (prm.buckets - 1)))
-        return set_error(FH_ERR_INVALID, "fh_create: buckets %u is not a power of two", prm.buckets);
-    if (prm.buckets > (1u << 28)) return set_error(FH_ERR_INVALID, "fh_create: buckets %u > 2^28", prm.buckets);
-    const uint32_t blk = prm.decide_block;
-    if (blk != 64 && blk != 128 && blk != 256 && blk != 512 && blk != 1024)
-        return set_error(FH_ERR_INVALID, "fh_create: decide_block %u not in {64,128,256,512,1024}", blk);
-    if (prm.birds_per_thread > 64 || blk * prm.birds_per_thread > 16384)
-        return set_error(FH_ERR_INVALID, "fh_create: decide chunk %u x %u too large", blk, prm.birds_per_thread);
+    fh_status st = resolve_params(params, &prm, "fh_create");
+    if (st) return st;
     if (n > (1u << 30)) return set_error(FH_ERR_INVALID, "fh_create: n = %u too large", n);
-    fh_status st = require_device();
+    st = require_device();
     if (st) return st;
 
     fh_flock *h = new fh_flock();

@@ -28,6 +28,9 @@ fh_status hip_fail(hipError_t e, const char *what);
 // Checks that device 0 is a usable gfx950 GPU (once per process).
 fh_status require_device();
 
+// Fills defaults into `in` (NULL = all defaults) and validates the result.
+fh_status resolve_params(const fh_params *in, fh_params *out, const char *who);
+
 // .flk record words (crates/flock-ref/src/state.rs:4-7).
 enum Word { PX = 0, PY, PZ, VX, VY, VZ, ROLL, SPIN, RADIUS, SX, SY, SZ, NEXT, NB, CEN };
 

@@ -47,7 +47,8 @@ class Params(C.Structure):
         ("decide_block", C.c_uint32),
         ("birds_per_thread", C.c_uint32),
         ("flags", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("decide_group", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
     ]
 
 
@@ -236,10 +237,10 @@ class Flock:
     """One flock on the GPU with the reference pass API (cuda/flock.cu:452-470)."""
 
     def __init__(self, n: int, *, cell: float = 0.0, buckets: int = 0, decide_block: int = 0,
-                 birds_per_thread: int = 0, flags: int = 0):
+                 birds_per_thread: int = 0, flags: int = 0, decide_group: int = 0):
         p = Params()
         p.cell, p.buckets, p.decide_block = cell, buckets, decide_block
-        p.birds_per_thread, p.flags = birds_per_thread, flags
+        p.birds_per_thread, p.flags, p.decide_group = birds_per_thread, flags, decide_group
         h = C.c_void_p()
         _check(lib().fh_create(n, C.addressof(p), C.addressof(h)))
         self._h = h
@@ -343,10 +344,11 @@ class Batch:
     """Many independent flocks stepped together, one launch per stage."""
 
     def __init__(self, bird_capacity: int, bucket_capacity: int, flock_capacity: int, *, cell: float = 0.0,
-                 buckets: int = 0, decide_block: int = 0, birds_per_thread: int = 0, flags: int = 0):
+                 buckets: int = 0, decide_block: int = 0, birds_per_thread: int = 0, flags: int = 0,
+                 decide_group: int = 0):
         p = Params()
         p.cell, p.buckets, p.decide_block = cell, buckets, decide_block
-        p.birds_per_thread, p.flags = birds_per_thread, flags
+        p.birds_per_thread, p.flags, p.decide_group = birds_per_thread, flags, decide_group
         h = C.c_void_p()
         _check(lib().fh_batch_create(bird_capacity, bucket_capacity, flock_capacity, C.addressof(p),
                                      C.addressof(h)))
@@ -467,7 +469,7 @@ def server_config(**kw) -> ServerConfig:
     c = ServerConfig()
     _check(lib().fh_server_config_default(C.addressof(c)))
     for k, v in kw.items():
-        if k in ("cell", "buckets", "decide_block", "birds_per_thread", "flags"):
+        if k in ("cell", "buckets", "decide_block", "birds_per_thread", "flags", "decide_group"):
             setattr(c.params, k, v)
         else:
             setattr(c, k, v)

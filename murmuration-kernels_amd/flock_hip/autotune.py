@@ -1,8 +1,9 @@
 """Autotuner for the batched and plain steps.
 
-Picks the grid cell edge, the decide workgroup size and the birds each decide
-lane scans (birds_per_thread) by timing short runs of the real workload:
-a coordinate search (birds_per_thread, then workgroup size, then cell edge),
+Picks the grid cell edge, the decide workgroup size, the birds each decide
+lane scans (birds_per_thread) and the lanes that share one bird's walk
+(decide_group) by timing short runs of the real workload: a coordinate
+search (group, birds_per_thread, workgroup size, then cell edge),
 each candidate timed over `steps` steps after `warmup` untimed ones. Results
 are cached in a JSON file keyed by the workload's signature.
 
@@ -22,6 +23,7 @@ from . import FLAG_GRAPH, Batch, Flock
 
 BPT = (2, 4, 8, 16)
 BLOCKS = (64, 128, 256, 512)
+GROUPS = (1, 2, 4, 8)
 CELLS = (4.0, 3.0, 5.0, 6.0)
 
 
@@ -79,7 +81,7 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
         return cache[key]
     timer = (lambda prm: _time_batch(starts, prm, flags, warmup, steps)) if kind == "batch" else \
         (lambda prm: _time_plain(starts[0], prm, flags, warmup, steps))
-    best = {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8}
+    best = {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1}
     trials = []
 
     def run(prm):
@@ -88,7 +90,8 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
         return t
 
     best_t = run(best)
-    for name, values in (("birds_per_thread", BPT), ("decide_block", BLOCKS), ("cell", cells)):
+    for name, values in (("decide_group", GROUPS), ("birds_per_thread", BPT), ("decide_block", BLOCKS),
+                         ("cell", cells)):
         for v in values:
             if v == best[name]:
                 continue

@@ -224,6 +224,39 @@ def test_c2_cube_1m_one_step_parity(fh, fr):
         assert gpu[i, 13].view(np.uint32) == len(want)
 
 
+@pytest.mark.parametrize("cell,group,block,bpt", [(4.0, 4, 256, 8), (6.0, 2, 128, 4), (3.0, 8, 64, 16),
+                                                  (5.0, 1, 512, 2)])
+def test_c2_one_step_parity_at_tuned_params(fh, fr, cell, group, block, bpt):
+    # Tunables change the hash keys (cell) and the summation grouping (group),
+    # never the neighbour sets: one step at 1M birds stays within 1e-4 of the
+    # oracle's grid walk at the reference cell size.
+    w0 = cube(fh, 1_000_000, 2)
+    with fh.Flock.from_words(w0, cell=cell, decide_group=group, decide_block=block, birds_per_thread=bpt,
+                             flags=fh.FLAG_CANONICAL) as f:
+        f.step(1)
+        gpu, t = f.download()
+    cpu, tc = fr.step_grid(w0, 0.0, 1, threads=THREADS)
+    d = fr.diff(cpu, gpu)
+    print("tuned 1-step", cell, group, d)
+    assert t == tc and d["nb_mismatch"] == 0
+    assert rel_err(gpu[:, 0:6], cpu[:, 0:6]) <= 1e-4
+
+
+@pytest.mark.parametrize("group", [2, 4, 8])
+def test_grouped_decide_matches_brute_force(fh, fr, group):
+    w = fh.gen_reference(4000, 6)
+    t = np.float32(0.0) + DT
+    with fh.Flock.from_words(w, decide_group=group) as f:
+        build_grid(f)
+        f.decide(t)
+        got, _ = f.download()
+    for i in np.nonzero(t >= w[:, 12])[0]:
+        d = fr.decide(w, int(i))
+        assert got[i, 13].view(np.uint32) == d["neighbours"], i
+        assert rel_err(got[i, 9:12], d["force"]) <= 1e-4, i
+        assert abs(got[i, 14] - d["centrality"]) <= 1e-5
+
+
 def test_step_is_deterministic_in_canonical_mode(fh):
     w0 = cube(fh, 50_000, 3)
     outs = []
