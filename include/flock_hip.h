@@ -58,7 +58,10 @@ typedef struct fh_params {
     uint32_t decide_group;     /* lanes sharing one bird's walk: 1, 2, 4 or 8; 0 -> 1. With 1 the
                                   sums follow the walk order exactly; with more, each lane sums
                                   every G-th cell and the partial sums are added */
-    uint32_t reserved[2];
+    uint32_t bucket_shift;     /* when buckets == 0: buckets_for(n) >> bucket_shift (at least 64),
+                                  0..8; a smaller table stays in L2 at the price of more hash
+                                  collisions. Keys equal the reference's only at shift 0 */
+    uint32_t reserved;
 } fh_params;
 
 /* Within a bucket, order birds by ascending index (a stable counting sort):

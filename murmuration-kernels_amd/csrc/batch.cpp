@@ -348,7 +348,7 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
     if (st) return st;
     if (!flock_id || (n && !words) || n == 0)
         return set_error(FH_ERR_INVALID, "fh_batch_add: need n > 0, words and flock_id");
-    const uint32_t buckets = b->prm.buckets ? b->prm.buckets : buckets_for(n);
+    const uint32_t buckets = flock_buckets(n, b->prm);
     const uint32_t ntile = (n + TILE - 1) / TILE;
     const uint32_t nunit = (buckets + SCAN_TILE - 1) / SCAN_TILE;
     // find a free segment slot
@@ -493,7 +493,9 @@ fh_status fh_batch_info(const fh_batch *b, uint32_t *live_flocks, uint32_t *live
 }
 
 size_t fh_flock_bytes(uint32_t n, const fh_params *params) {
-    const uint32_t buckets = (params && params->buckets) ? params->buckets : buckets_for(n);
+    fh_params prm;
+    if (resolve_params(params, &prm, "fh_flock_bytes")) fh_params_default(&prm);
+    const uint32_t buckets = flock_buckets(n, prm);
     const size_t birds = (size_t)((n + TILE - 1) / TILE) * TILE;
     const size_t bk = (size_t)((buckets + SCAN_TILE - 1) / SCAN_TILE) * SCAN_TILE;
     return birds * batch_bytes_per_bird() + bk * batch_bytes_per_bucket();

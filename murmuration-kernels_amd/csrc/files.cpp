@@ -62,8 +62,10 @@ fh_status resolve_params(const fh_params *in, fh_params *out, const char *who) {
         if (in->decide_block) prm.decide_block = in->decide_block;
         if (in->birds_per_thread) prm.birds_per_thread = in->birds_per_thread;
         if (in->decide_group) prm.decide_group = in->decide_group;
+        prm.bucket_shift = in->bucket_shift;
         prm.flags = in->flags;
     }
+    if (prm.bucket_shift > 8) return set_error(FH_ERR_INVALID, "%s: bucket_shift %u > 8", who, prm.bucket_shift);
     if (!(prm.cell >= 0.5f) || !(prm.cell <= 1000.0f))
         return set_error(FH_ERR_INVALID, "%s: cell %g outside [0.5, 1000] m", who, prm.cell);
     if (prm.buckets && (prm.buckets & (prm.buckets - 1)))

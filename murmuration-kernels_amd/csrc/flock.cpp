@@ -166,7 +166,7 @@ fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
     fh_flock *h = new fh_flock();
     h->n = n;
     h->prm = prm;
-    h->buckets = prm.buckets ? prm.buckets : buckets_for(n);
+    h->buckets = flock_buckets(n, prm);
     const uint32_t ntiles = (h->buckets + SCAN_TILE - 1) / SCAN_TILE;
     h->padded_buckets = ntiles * SCAN_TILE;
 

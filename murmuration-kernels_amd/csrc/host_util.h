@@ -31,6 +31,17 @@ fh_status require_device();
 // Fills defaults into `in` (NULL = all defaults) and validates the result.
 fh_status resolve_params(const fh_params *in, fh_params *out, const char *who);
 
+// Buckets of a flock of n birds under resolved params: the override, or
+// buckets_for(n) >> bucket_shift, at least 64.
+inline uint32_t flock_buckets(uint32_t n, const fh_params &p) {
+    if (p.buckets) return p.buckets;
+    uint32_t b = 1;
+    while ((uint64_t)b < (uint64_t)n * 2) b <<= 1;
+    b = b < (1u << 16) ? (1u << 16) : (b > (1u << 22) ? (1u << 22) : b);
+    b >>= p.bucket_shift;
+    return b < 64u ? 64u : b;
+}
+
 // .flk record words (crates/flock-ref/src/state.rs:4-7).
 enum Word { PX = 0, PY, PZ, VX, VY, VZ, ROLL, SPIN, RADIUS, SX, SY, SZ, NEXT, NB, CEN };
 

@@ -48,7 +48,8 @@ class Params(C.Structure):
         ("birds_per_thread", C.c_uint32),
         ("flags", C.c_uint32),
         ("decide_group", C.c_uint32),
-        ("reserved", C.c_uint32 * 2),
+        ("bucket_shift", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -237,10 +238,11 @@ class Flock:
     """One flock on the GPU with the reference pass API (cuda/flock.cu:452-470)."""
 
     def __init__(self, n: int, *, cell: float = 0.0, buckets: int = 0, decide_block: int = 0,
-                 birds_per_thread: int = 0, flags: int = 0, decide_group: int = 0):
+                 birds_per_thread: int = 0, flags: int = 0, decide_group: int = 0, bucket_shift: int = 0):
         p = Params()
         p.cell, p.buckets, p.decide_block = cell, buckets, decide_block
         p.birds_per_thread, p.flags, p.decide_group = birds_per_thread, flags, decide_group
+        p.bucket_shift = bucket_shift
         h = C.c_void_p()
         _check(lib().fh_create(n, C.addressof(p), C.addressof(h)))
         self._h = h
@@ -333,10 +335,10 @@ class Flock:
         return ids[: min(cnt.value, cap)].copy() if cnt.value <= cap else ids.copy()
 
 
-def flock_bytes(n: int, buckets: int = 0) -> int:
+def flock_bytes(n: int, buckets: int = 0, bucket_shift: int = 0) -> int:
     """Device bytes one flock of n birds occupies in a batch (admission unit)."""
     p = Params()
-    p.buckets = buckets
+    p.buckets, p.bucket_shift = buckets, bucket_shift
     return int(lib().fh_flock_bytes(n, C.addressof(p)))
 
 
@@ -345,10 +347,11 @@ class Batch:
 
     def __init__(self, bird_capacity: int, bucket_capacity: int, flock_capacity: int, *, cell: float = 0.0,
                  buckets: int = 0, decide_block: int = 0, birds_per_thread: int = 0, flags: int = 0,
-                 decide_group: int = 0):
+                 decide_group: int = 0, bucket_shift: int = 0):
         p = Params()
         p.cell, p.buckets, p.decide_block = cell, buckets, decide_block
         p.birds_per_thread, p.flags, p.decide_group = birds_per_thread, flags, decide_group
+        p.bucket_shift = bucket_shift
         h = C.c_void_p()
         _check(lib().fh_batch_create(bird_capacity, bucket_capacity, flock_capacity, C.addressof(p),
                                      C.addressof(h)))
@@ -469,7 +472,7 @@ def server_config(**kw) -> ServerConfig:
     c = ServerConfig()
     _check(lib().fh_server_config_default(C.addressof(c)))
     for k, v in kw.items():
-        if k in ("cell", "buckets", "decide_block", "birds_per_thread", "flags", "decide_group"):
+        if k in ("cell", "buckets", "decide_block", "birds_per_thread", "flags", "decide_group", "bucket_shift"):
             setattr(c.params, k, v)
         else:
             setattr(c, k, v)

@@ -3,7 +3,8 @@
 Picks the grid cell edge, the decide workgroup size, the birds each decide
 lane scans (birds_per_thread) and the lanes that share one bird's walk
 (decide_group) by timing short runs of the real workload: a coordinate
-search (group, birds_per_thread, workgroup size, then cell edge),
+search (group, birds_per_thread, workgroup size, cell edge, bucket count,
+then cell edge and group again),
 each candidate timed over `steps` steps after `warmup` untimed ones. Results
 are cached in a JSON file keyed by the workload's signature.
 
@@ -24,7 +25,8 @@ from . import FLAG_GRAPH, Batch, Flock
 BPT = (2, 4, 8, 16)
 BLOCKS = (64, 128, 256, 512)
 GROUPS = (1, 2, 4, 8)
-CELLS = (4.0, 3.0, 5.0, 6.0)
+CELLS = (4.0, 3.0, 5.0, 6.0, 8.0, 10.0, 12.0)
+SHIFTS = (0, 1, 2, 3, 4)
 
 
 def _signature(kind: str, sizes, steps: int, flags: int) -> str:
@@ -81,7 +83,7 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
         return cache[key]
     timer = (lambda prm: _time_batch(starts, prm, flags, warmup, steps)) if kind == "batch" else \
         (lambda prm: _time_plain(starts[0], prm, flags, warmup, steps))
-    best = {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1}
+    best = {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0}
     trials = []
 
     def run(prm):
@@ -90,12 +92,16 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
         return t
 
     best_t = run(best)
+    seen = {tuple(sorted(best.items()))}
     for name, values in (("decide_group", GROUPS), ("birds_per_thread", BPT), ("decide_block", BLOCKS),
-                         ("cell", cells)):
+                         ("cell", cells), ("bucket_shift", SHIFTS), ("cell", cells), ("decide_group", GROUPS)):
         for v in values:
             if v == best[name]:
                 continue
             cand = {**best, name: v}
+            if tuple(sorted(cand.items())) in seen:
+                continue
+            seen.add(tuple(sorted(cand.items())))
             t = run(cand)
             if t < best_t:
                 best, best_t = cand, t

@@ -224,15 +224,16 @@ def test_c2_cube_1m_one_step_parity(fh, fr):
         assert gpu[i, 13].view(np.uint32) == len(want)
 
 
-@pytest.mark.parametrize("cell,group,block,bpt", [(4.0, 4, 256, 8), (6.0, 2, 128, 4), (3.0, 8, 64, 16),
-                                                  (5.0, 1, 512, 2)])
-def test_c2_one_step_parity_at_tuned_params(fh, fr, cell, group, block, bpt):
+@pytest.mark.parametrize("cell,group,block,bpt,shift", [(4.0, 4, 256, 8, 0), (6.0, 2, 128, 4, 1),
+                                                        (3.0, 8, 64, 16, 0), (5.0, 1, 512, 2, 2),
+                                                        (8.0, 4, 128, 8, 3), (12.0, 2, 256, 4, 4)])
+def test_c2_one_step_parity_at_tuned_params(fh, fr, cell, group, block, bpt, shift):
     # Tunables change the hash keys (cell) and the summation grouping (group),
     # never the neighbour sets: one step at 1M birds stays within 1e-4 of the
     # oracle's grid walk at the reference cell size.
     w0 = cube(fh, 1_000_000, 2)
     with fh.Flock.from_words(w0, cell=cell, decide_group=group, decide_block=block, birds_per_thread=bpt,
-                             flags=fh.FLAG_CANONICAL) as f:
+                             bucket_shift=shift, flags=fh.FLAG_CANONICAL) as f:
         f.step(1)
         gpu, t = f.download()
     cpu, tc = fr.step_grid(w0, 0.0, 1, threads=THREADS)
