@@ -593,6 +593,17 @@ void fr_decide_grid(const float *words, uint32_t n, uint32_t i, float z0, float 
     finish_decision(&a, me, z0, out);
 }
 
+void fr_decide_grid_many(const float *words, uint32_t n, const uint32_t *birds, uint32_t count, float z0,
+                         float cell, uint32_t buckets, const uint32_t *offsets, const uint32_t *sorted,
+                         float *out, int threads) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads > 0 ? threads : 1)
+#endif
+    for (int64_t k = 0; k < (int64_t)count; ++k)
+        fr_decide_grid(words, n, birds[k], z0, cell, buckets, offsets, sorted, out + (size_t)k * 6);
+    (void)threads;
+}
+
 void fr_step_grid(float *words, uint32_t n, float *time, uint32_t steps, float cell,
                   uint32_t buckets, int threads) {
     float *dec = (float *)malloc((size_t)(n ? n : 1) * 6 * sizeof(float));

@@ -124,6 +124,11 @@ uint32_t fr_neighbours_brute(const float *words, uint32_t n, uint32_t i, int k,
 void fr_decide_grid(const float *words, uint32_t n, uint32_t i, float z0, float cell,
                     uint32_t buckets, const uint32_t *offsets, const uint32_t *sorted,
                     float out[6]);
+/* fr_decide_grid for `count` birds (threads > 1: in parallel); out is
+ * count x 6 in the fr_decide layout. */
+void fr_decide_grid_many(const float *words, uint32_t n, const uint32_t *birds, uint32_t count, float z0,
+                         float cell, uint32_t buckets, const uint32_t *offsets, const uint32_t *sorted,
+                         float *out, int threads);
 /* Flock::step using fr_decide_grid for the decisions (threads as fr_step). */
 void fr_step_grid(float *words, uint32_t n, float *time, uint32_t steps, float cell,
                   uint32_t buckets, int threads);

@@ -1,0 +1,68 @@
+"""Clustered flock: Gaussian clumps with thousands of birds per 4 m cell.
+
+Parity on sampled due birds against the oracle's grid walk over the whole
+flock (the same neighbour semantics as brute force, pinned in
+test_oracle.py), and against brute force for a few of them."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+DT = np.float32(0.005)
+
+
+def rel_err(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)), initial=0.0))
+
+
+@pytest.mark.parametrize("n,cell,flags", [(1_000_000, 4.0, 0), (1_000_000, 2.0, 8)])
+def test_clumped_decide_matches_the_oracle(fh, fr, n, cell, flags):
+    w = fh.gen_clumps(n, 8, 5.0, 1000.0, 3)
+    t = np.float32(0.0) + DT
+    with fh.Flock.from_words(w, cell=cell, flags=flags, decide_group=4) as f:
+        f.hash_count()
+        g = f.read_grid(offsets=False, sorted=False)
+        assert int(g.counts.max()) > 1000  # dense buckets really occur
+        f.scan()
+        f.scatter()
+        f.decide(t)
+        got, _ = f.download()
+    due = np.nonzero(t >= w[:, 12])[0]
+    sample = np.sort(np.random.default_rng(0).choice(due, 2000, replace=False))
+    keys, counts, offsets, srt = fr.grid(w)
+    d = fr.decide_grid_many(w, sample, offsets, srt, fr.buckets_for(n), threads=THREADS)
+    assert np.array_equal(got[sample, 13].view(np.uint32), d["neighbours"])
+    assert rel_err(got[sample, 9:12], d["force"]) <= 1e-4
+    assert rel_err(got[sample, 14], d["centrality"]) <= 1e-4
+    for k, i in enumerate(sample[:6]):
+        assert fr.decide(w, int(i))["neighbours"] == d["neighbours"][k]
+
+
+def test_clumped_step_parity_10m_sample(fh, fr):
+    # The configured workload, 10M birds: one step, checked on 50,000 due birds
+    # sampled from the whole flock.
+    n = 10_000_000
+    w = fh.gen_clumps(n, 8, 5.0, 1000.0, 1)
+    with fh.Flock.from_words(w, cell=2.0, decide_group=4) as f:
+        f.step(1)
+        gpu, tg = f.download()
+    due = np.nonzero(np.float32(0.0) + DT >= w[:, 12])[0]
+    sample = np.sort(np.random.default_rng(1).choice(due, 50_000, replace=False))
+    keys, counts, offsets, srt = fr.grid(w)
+    d = fr.decide_grid_many(w, sample, offsets, srt, fr.buckets_for(n), threads=THREADS)
+    assert np.array_equal(gpu[sample, 13].view(np.uint32), d["neighbours"])
+    assert rel_err(gpu[sample, 9:12], d["force"]) <= 1e-4
+    # positions and velocities after the step follow step_bird of the decided state
+    for k in range(0, 2000):
+        i = sample[k]
+        r = w[i].copy()
+        r[9:12] = d["force"][k]
+        r[8] = d["radius"][k]
+        r[12] += np.float32(0.05)
+        want = fr.step_bird(r)
+        assert rel_err(gpu[i, 0:6], want[0:6]) <= 1e-4, i
