@@ -87,7 +87,7 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--slots", type=int, default=64)
-    ap.add_argument("--load", type=float, default=0.7, help="offered load / batched throughput")
+    ap.add_argument("--load", type=float, default=0.6, help="offered load / measured server capacity")
     ap.add_argument("--copy-cal", action="store_true",
                     help="only run a 256 MiB device-to-device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -118,7 +118,8 @@ def main() -> int:
     if not args.no_tune:
         from flock_hip import autotune
 
-        tuned = autotune.tune([words], "plain", flags=args.flags, warmup=3, steps=8)
+        # trials time the same window as the timed run (radii grow each decision)
+        tuned = autotune.tune([words], "plain", flags=args.flags, warmup=args.warmup, steps=10)
         prm = tuned["params"]
 
     def timed_run(params: dict) -> tuple[float, int]:
@@ -303,42 +304,67 @@ def batched_leg(args) -> dict:
     }
 
 
-def served_leg(args, batched: dict) -> dict:
-    """A Poisson stream of requests (birds log-uniform 2e3..5e5, steps uniform
-    50..500, own seeds) through the continuous-batching server: admission by
-    HBM bytes against a 200 GB budget and a slot cap, state resident on the
-    device. The rate is set to `load` x the batched throughput."""
-    import flock_hip as fh
-    from flock_hip import workloads
-
-    mean_bird_steps = 90_200 * 275.0  # E[birds] x E[steps] of the request mix
-    rate = args.load * batched["batched_bird_steps_per_s"] / mean_bird_steps
-    req = workloads.stream(args.requests, rate, seed=args.seed)
-    t0 = time.perf_counter()
-    prm = batched["autotuned"]
+def _replay(fh, req, args, prm) -> dict:
     res, st = fh.server_replay(req["birds"], req["steps"], req["seeds"], req["arrival_s"],
                                max_active=args.slots, hbm_budget_bytes=200e9, steps_per_iter=8,
                                flags=fh.FLAG_CANONICAL, **prm)
-    wall = time.perf_counter() - t0
     ok = [r for r in res if r["status"] == 0]
     lat = np.array([r["done_s"] - r["arrival_s"] for r in ok])
     span = max(r["done_s"] for r in ok) - min(r["arrival_s"] for r in ok)
-    total_bs = float(np.sum(req["birds"].astype(np.float64) * req["steps"]))
+    done = np.array([r["birds"] * r["steps"] for r in ok], np.float64)
     return {
-        "workload": f"{args.requests} requests, Poisson {rate:.1f}/s, birds log-uniform 2e3..5e5, "
-                    f"steps uniform 50..500, slot cap {args.slots}, 200 GB HBM budget "
-                    f"({(fh.flock_bytes(512) - fh.flock_bytes(256)) // 256} B per bird, birds padded to 256, "
-                    f"+ 8 B per bucket)",
-        "rate_per_s": rate,
+        "requests": len(req["birds"]),
         "completed": len(ok),
-        "bird_steps_per_s": total_bs / span,
+        "bird_steps_per_s": float(done.sum()) / span,
         "bird_steps_per_busy_s": st["bird_steps"] / st["busy_s"],
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "peak_active": st["peak_active"],
         "peak_hbm_bytes": st["peak_bytes"],
         "iterations": st["iterations"],
-        "leg_seconds": wall,
+    }
+
+
+def served_leg(args, batched: dict) -> dict:
+    """Flock-simulation requests (birds log-uniform 2e3..5e5, steps uniform
+    50..500, own seeds) through the continuous-batching server: admission by
+    HBM bytes against a 200 GB budget and a slot cap, state resident on the
+    device, active flocks packed into the batched step.
+
+    1. capacity: 500 requests arriving within 0.1 s (the server saturated);
+    2. latency: the 2000-request Poisson stream at `load` x that capacity.
+    Tunables are autotuned on request-sized flocks warmed for 250 steps (the
+    mean request length), where the adaptive radii have grown."""
+    import flock_hip as fh
+    from flock_hip import autotune, workloads
+
+    t0 = time.perf_counter()
+    warm_req = workloads.stream(32, 1.0, seed=args.seed + 7)
+    warm = []
+    with fh.Batch(40 << 20, 64 << 20, 64, flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL) as b:
+        ids = [b.add(fh.gen_request(int(n), int(s))) for n, s in zip(warm_req["birds"], warm_req["seeds"])]
+        b.step(250)
+        warm = [b.download(i)[0] for i in ids]
+    tuned = autotune.tune(warm, "batch", flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, warmup=1, steps=4,
+                          cells=(4.0, 6.0, 8.0, 10.0, 12.0))
+    prm = tuned["params"]
+    del warm
+    sat = workloads.stream(500, 5000.0, seed=args.seed + 1)
+    capacity = _replay(fh, sat, args, prm)
+    mean_bird_steps = 90_200 * 275.0  # E[birds] x E[steps] of the request mix
+    rate = args.load * capacity["bird_steps_per_s"] / mean_bird_steps
+    req = workloads.stream(args.requests, rate, seed=args.seed)
+    stream = _replay(fh, req, args, prm)
+    return {
+        "workload": f"{args.requests} requests, Poisson {rate:.1f}/s ({args.load:.0%} of the measured capacity), "
+                    f"birds log-uniform 2e3..5e5, steps uniform 50..500, slot cap {args.slots}, 200 GB HBM budget "
+                    f"({(fh.flock_bytes(512) - fh.flock_bytes(256)) // 256} B per bird, birds padded to 256, "
+                    f"+ 8 B per bucket)",
+        "rate_per_s": rate,
+        **stream,
+        "capacity": capacity,
+        "autotuned": prm,
+        "leg_seconds": time.perf_counter() - t0,
     }
 
 

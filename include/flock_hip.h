@@ -75,6 +75,9 @@ typedef struct fh_params {
 /* Wave-aggregated histogram atomics: lanes of a wave that share a bucket
  * issue one atomic (pays off when many birds share a cell). */
 #define FH_FLAG_WAVE_AGG 8u
+/* Never use decide's tiled path (cells shared by a workgroup's due birds
+ * staged in LDS); its results are bit-identical, this is for checking. */
+#define FH_FLAG_NO_TILE 16u
 
 /* Per-pass GPU time of fh_step, mean ms per timed step (hipEvents on the
  * handle's stream), in the rows of the reference report

@@ -115,7 +115,7 @@ static fh_status batch_step_launches(fh_batch *b, hipEvent_t *ev) {
     FH_HIP(launch_scatter(b->geo, b->P, (b->prm.flags & FH_FLAG_CANONICAL) != 0, true, b->stream));
     if (ev) FH_HIP(hipEventRecord(ev[3], b->stream));
     FH_HIP(launch_decide(b->geo, b->P, 0.0f, b->prm.decide_block, b->prm.birds_per_thread, b->prm.decide_group,
-                         b->stream));
+                         !(b->prm.flags & FH_FLAG_NO_TILE), b->stream));
     if (ev) FH_HIP(hipEventRecord(ev[4], b->stream));
     FH_HIP(launch_integrate(b->geo, b->P, true, (b->prm.flags & FH_FLAG_WAVE_AGG) != 0, b->stream));
     b->counts_dirty = true;

@@ -136,37 +136,44 @@ struct Acc {
     uint32_t n_all, n_star, n_align, n_g;
 };
 
-template <int G, int PF>
-__device__ __forceinline__ Acc walk_sums(const Pools &p, const Seg &sg, uint32_t n_slots, uint32_t me, f3 pos,
-                                         f3 f_i, float r, int32_t first) {
-    Acc a;
+__device__ __forceinline__ void acc_zero(Acc &a) {
     a.sep = mk3(0.f, 0.f, 0.f);
     a.coh = a.sep;
     a.align = a.sep;
     a.csum = a.sep;
     a.n_all = a.n_star = a.n_align = a.n_g = 0;
-    const Walk w = make_walk(pos, r, sg);
-    walk_cells<G, PF>(w, sg, p, n_slots, me, first, [&](uint32_t t, f3 d, float d2) {
-        const float dist = sqrtf(d2);
-        const f3 dhat = d / fmaxf(dist, 1e-6f);
-        // walk 2 (model.rs:104-115): csum += d / max(sqrt(d2), 1e-6)
-        a.csum = a.csum + dhat;
-        ++a.n_g;
-        if (d2 > w.r2) return;
-        // walk 1 (model.rs:67-93)
-        ++a.n_all;
-        const float g = dist <= R_H ? 1.0f : expf(-((dist - R_H) * (dist - R_H)) / SEP_DENOM);
-        a.sep = a.sep - dhat * g;
-        if (dist > R_H) {
-            a.coh = a.coh + dhat;
-            ++a.n_star;
-        }
-        if (dot3(f_i, dhat) >= BLIND_COS) {
-            const float4 vj = p.svel[t];
-            a.align = a.align + forward_of(mk3(vj.x, vj.y, vj.z));
-            ++a.n_align;
-        }
-    });
+}
+
+// One candidate at slot t with offset d and d^2 <= 4r^2.
+__device__ __forceinline__ void accumulate(Acc &a, const Pools &p, f3 f_i, float r2, uint32_t t, f3 d, float d2) {
+    const float dist = sqrtf(d2);
+    const f3 dhat = d / fmaxf(dist, 1e-6f);
+    // walk 2 (model.rs:104-115): csum += d / max(sqrt(d2), 1e-6)
+    a.csum = a.csum + dhat;
+    ++a.n_g;
+    if (d2 > r2) return;
+    // walk 1 (model.rs:67-93)
+    ++a.n_all;
+    const float g = dist <= R_H ? 1.0f : expf(-((dist - R_H) * (dist - R_H)) / SEP_DENOM);
+    a.sep = a.sep - dhat * g;
+    if (dist > R_H) {
+        a.coh = a.coh + dhat;
+        ++a.n_star;
+    }
+    if (dot3(f_i, dhat) >= BLIND_COS) {
+        const float4 vj = p.svel[t];
+        a.align = a.align + forward_of(mk3(vj.x, vj.y, vj.z));
+        ++a.n_align;
+    }
+}
+
+template <int G, int PF>
+__device__ __forceinline__ Acc walk_sums(const Pools &p, const Seg &sg, uint32_t n_slots, uint32_t me, const Walk &w,
+                                         f3 f_i, int32_t first) {
+    Acc a;
+    acc_zero(a);
+    walk_cells<G, PF>(w, sg, p, n_slots, me, first,
+                      [&](uint32_t t, f3 d, float d2) { accumulate(a, p, f_i, w.r2, t, d, d2); });
     return a;
 }
 
@@ -210,14 +217,32 @@ __device__ __forceinline__ Decision finish(const Acc &a, f3 pos, f3 vel, f3 f_i,
     return dcs;
 }
 
+// Cells of the union box the tiled path may walk, and how much larger than
+// the largest single box the union may be, before the per-lane walk is used.
+constexpr int64_t TILED_MAX_CELLS = 4096;
+constexpr int64_t TILED_MAX_RATIO = 8;
+constexpr uint32_t NO_BIRD = 0xffffffffu;
+
 // BLOCK lanes per workgroup, G lanes per due bird. With G = 1 each bird's sums
 // follow the walk order exactly (the order of cuda/flock.cu:209-238); with
 // G > 1 lane j sums cells j, j+G, ... and the G partial sums are added.
+//
+// Tiled path (G = 1): when the due birds a workgroup holds at once share
+// their neighbourhood (a dense cell: they sit together in slot order), the
+// workgroup walks the union of their boxes in z-y-x order. For each cell some
+// lane needs, the bucket's candidates are staged in LDS BLOCK at a time with
+// coalesced loads and every lane that needs the cell reads them from LDS
+// (broadcast). A lane sees the same (cell, slot) sequence as in its own walk,
+// so its sums are bit-identical to the per-lane path.
 template <int BLOCK, int G>
-__global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float time, uint32_t chunk) {
+__global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float time, uint32_t chunk,
+                                                  int allow_tiled) {
     constexpr int PF = G >= 4 ? 4 : 8;
     extern __shared__ uint32_t due_slots[];
     __shared__ uint32_t n_due;
+    __shared__ int32_t u_lo[3], u_hi[3];
+    __shared__ uint32_t u_seg_lo, u_seg_hi, u_vol, u_any;
+    __shared__ float4 tile[BLOCK];
     if (threadIdx.x == 0) n_due = 0;
     __syncthreads();
     const uint32_t start = blockIdx.x * chunk;
@@ -232,16 +257,13 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
     for (uint32_t base = 0; base < nd; base += groups) {  // same trip count for every lane
         const uint32_t k = base + threadIdx.x / G;
         bool active = k < nd;
-        uint32_t me = 0;
+        uint32_t me = 0, si = 0;
         f3 pos = mk3(0.f, 0.f, 0.f), vel = pos, f_i = pos;
         float r = 0.0f;
-        const Seg *sg = &g.segs[0];
         Acc a;
-        a.sep = mk3(0.f, 0.f, 0.f);
-        a.coh = a.sep;
-        a.align = a.sep;
-        a.csum = a.sep;
-        a.n_all = a.n_star = a.n_align = a.n_g = 0;
+        acc_zero(a);
+        Walk w;
+        w.ok = false;
         if (active) {
             const uint32_t s = due_slots[k];
             const float4 q = p.spos[s];
@@ -250,15 +272,89 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
                 if (lane_g == 0) atomicOr(p.err, ERR_SLOT);
                 active = false;
             } else {
-                sg = &g.segs[g.tile_seg ? g.tile_seg[me / TILE] : 0u];
+                si = g.tile_seg ? g.tile_seg[me / TILE] : 0u;
                 const float4 v = p.svel[s];
                 pos = mk3(q.x, q.y, q.z);
                 vel = mk3(v.x, v.y, v.z);
                 f_i = forward_of(vel);
                 r = p.steer4[me].w;
-                a = walk_sums<G, PF>(p, *sg, g.n_slots, me, pos, f_i, r, (int32_t)lane_g);
+                w = make_walk(pos, r, g.segs[si]);
             }
         }
+        bool tiled = false;
+        if (G == 1) {
+            if (threadIdx.x == 0) {
+                u_lo[0] = u_lo[1] = u_lo[2] = 2147483647;
+                u_hi[0] = u_hi[1] = u_hi[2] = -2147483647 - 1;
+                u_seg_lo = 0xffffffffu;
+                u_seg_hi = 0u;
+                u_vol = 0u;
+                u_any = 0u;
+            }
+            __syncthreads();
+            if (active && w.ok) {
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    atomicMin(&u_lo[ax], w.lo[ax]);
+                    atomicMax(&u_hi[ax], w.hi[ax]);
+                }
+                atomicMin(&u_seg_lo, si);
+                atomicMax(&u_seg_hi, si);
+                const int64_t vol = (int64_t)(w.hi[0] - w.lo[0] + 1) * (w.hi[1] - w.lo[1] + 1) *
+                                    (w.hi[2] - w.lo[2] + 1);
+                atomicMax(&u_vol, (uint32_t)min<int64_t>(vol, 0x7fffffff));
+                u_any = 1u;
+            }
+            __syncthreads();
+            const int64_t ex = (int64_t)u_hi[0] - u_lo[0] + 1, ey = (int64_t)u_hi[1] - u_lo[1] + 1,
+                          ez = (int64_t)u_hi[2] - u_lo[2] + 1;
+            tiled = allow_tiled && u_any && u_seg_lo == u_seg_hi && ex <= TILED_MAX_CELLS && ey <= TILED_MAX_CELLS &&
+                    ez <= TILED_MAX_CELLS && ex * ey * ez <= TILED_MAX_CELLS &&
+                    ex * ey * ez <= TILED_MAX_RATIO * (int64_t)u_vol;
+        }
+        if (tiled) {
+            const Seg &sg = g.segs[u_seg_lo];
+            const int32_t lx = u_lo[0], ly = u_lo[1], lz = u_lo[2];
+            const int32_t hx = u_hi[0], hy = u_hi[1], hz = u_hi[2];
+            for (int32_t cz = lz; cz <= hz; ++cz)
+                for (int32_t cy = ly; cy <= hy; ++cy)
+                    for (int32_t cx = lx; cx <= hx; ++cx) {
+                        const bool need = active && w.ok && cx >= w.lo[0] && cx <= w.hi[0] && cy >= w.lo[1] &&
+                                          cy <= w.hi[1] && cz >= w.lo[2] && cz <= w.hi[2] &&
+                                          cell_meets_sphere(cx, cy, cz, sg.cell, w.p, w.c2);
+                        if (!__syncthreads_or(need)) continue;
+                        const uint32_t b = sg.bucket_base + (bucket_hash(cx, cy, cz) & sg.mask);
+                        const uint32_t s0 = p.offsets[b];
+                        const uint32_t s1 = min(p.offsets[b + 1], g.n_slots);
+                        for (uint32_t t0 = s0; t0 < s1; t0 += BLOCK) {
+                            const uint32_t t = t0 + threadIdx.x;
+                            if (t < s1) {
+                                const float4 c = p.spos[t];
+                                const bool in_cell = cell_coord(c.x, sg.inv_cell) == cx &&
+                                                     cell_coord(c.y, sg.inv_cell) == cy &&
+                                                     cell_coord(c.z, sg.inv_cell) == cz;
+                                tile[threadIdx.x] = make_float4(c.x, c.y, c.z, in_cell ? c.w : __uint_as_float(NO_BIRD));
+                            }
+                            __syncthreads();
+                            const uint32_t cnt = min((uint32_t)BLOCK, s1 - t0);
+                            if (need) {
+                                for (uint32_t j = 0; j < cnt; ++j) {
+                                    const float4 c = tile[j];
+                                    const uint32_t id = __float_as_uint(c.w);
+                                    if (id == NO_BIRD || id == me) continue;
+                                    const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
+                                    const float d2 = dot3(d, d);
+                                    if (d2 > w.c2) continue;
+                                    accumulate(a, p, f_i, w.r2, t0 + j, d, d2);
+                                }
+                            }
+                            __syncthreads();
+                        }
+                    }
+        } else if (active) {
+            a = walk_sums<G, PF>(p, g.segs[si], g.n_slots, me, w, f_i, (int32_t)lane_g);
+        }
+        if (G == 1) __syncthreads();  // the union box is reset by the next iteration
         if (G > 1) reduce_group<G>(a);
         if (active && lane_g == 0) {
             const Decision d = finish(a, pos, vel, f_i, r);
@@ -289,35 +385,36 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
 }
 
 template <int BLOCK, int G>
-static void launch_decide_block(const Geometry &g, const Pools &p, float time, uint32_t chunk, hipStream_t s) {
+static void launch_decide_block(const Geometry &g, const Pools &p, float time, uint32_t chunk, int tiled,
+                                hipStream_t s) {
     const uint32_t blocks = (g.n_slots + chunk - 1) / chunk;
     hipLaunchKernelGGL((k_decide<BLOCK, G>), dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s, g, p, time,
-                       chunk);
+                       chunk, tiled);
 }
 
 template <int BLOCK>
 static hipError_t launch_decide_group(const Geometry &g, const Pools &p, float time, uint32_t chunk, uint32_t group,
-                                      hipStream_t s) {
+                                      int tiled, hipStream_t s) {
     switch (group) {
-        case 1: launch_decide_block<BLOCK, 1>(g, p, time, chunk, s); break;
-        case 2: launch_decide_block<BLOCK, 2>(g, p, time, chunk, s); break;
-        case 4: launch_decide_block<BLOCK, 4>(g, p, time, chunk, s); break;
-        case 8: launch_decide_block<BLOCK, 8>(g, p, time, chunk, s); break;
+        case 1: launch_decide_block<BLOCK, 1>(g, p, time, chunk, tiled, s); break;
+        case 2: launch_decide_block<BLOCK, 2>(g, p, time, chunk, tiled, s); break;
+        case 4: launch_decide_block<BLOCK, 4>(g, p, time, chunk, tiled, s); break;
+        case 8: launch_decide_block<BLOCK, 8>(g, p, time, chunk, tiled, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipSuccess;
 }
 
 hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
-                         uint32_t birds_per_thread, uint32_t group, hipStream_t s) {
+                         uint32_t birds_per_thread, uint32_t group, bool tiled, hipStream_t s) {
     if (g.n_slots == 0) return hipSuccess;
     const uint32_t chunk = block * birds_per_thread;
     hipError_t e;
     switch (block) {
-        case 64: e = launch_decide_group<64>(g, p, time, chunk, group, s); break;
-        case 128: e = launch_decide_group<128>(g, p, time, chunk, group, s); break;
-        case 256: e = launch_decide_group<256>(g, p, time, chunk, group, s); break;
-        case 512: e = launch_decide_group<512>(g, p, time, chunk, group, s); break;
+        case 64: e = launch_decide_group<64>(g, p, time, chunk, group, tiled ? 1 : 0, s); break;
+        case 128: e = launch_decide_group<128>(g, p, time, chunk, group, tiled ? 1 : 0, s); break;
+        case 256: e = launch_decide_group<256>(g, p, time, chunk, group, tiled ? 1 : 0, s); break;
+        case 512: e = launch_decide_group<512>(g, p, time, chunk, group, tiled ? 1 : 0, s); break;
         default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;

@@ -80,7 +80,7 @@ static fh_status do_scatter(fh_flock *h, bool relative) {
 
 static fh_status do_decide(fh_flock *h, float time_arg) {
     FH_HIP(launch_decide(h->geo, h->P, time_arg, h->prm.decide_block, h->prm.birds_per_thread,
-                         h->prm.decide_group, h->stream));
+                         h->prm.decide_group, !(h->prm.flags & FH_FLAG_NO_TILE), h->stream));
     h->stage = Stage::Decided;
     return FH_OK;
 }

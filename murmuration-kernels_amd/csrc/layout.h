@@ -80,7 +80,7 @@ hipError_t launch_scan(const Geometry &g, const Pools &p, hipStream_t s);
 hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, bool relative_time,
                           hipStream_t s);
 hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
-                         uint32_t birds_per_thread, uint32_t group, hipStream_t s);
+                         uint32_t birds_per_thread, uint32_t group, bool tiled, hipStream_t s);
 hipError_t launch_integrate(const Geometry &g, const Pools &p, bool hash, bool wave_agg,
                             hipStream_t s);
 hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, int walk,
