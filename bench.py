@@ -23,6 +23,12 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+_T0 = time.perf_counter()
+
+
+def log(msg: str) -> None:
+    """One progress line on stderr (the JSON result goes to stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def step_bytes(n: int, buckets: int, due: int) -> dict:
@@ -119,7 +125,8 @@ def main() -> int:
         from flock_hip import autotune
 
         # trials time the same window as the timed run (radii grow each decision)
-        tuned = autotune.tune([words], "plain", flags=args.flags, warmup=args.warmup, steps=10)
+        log(f"plain leg: autotune at {n} birds")
+        tuned = autotune.tune([words], "plain", flags=args.flags, warmup=args.warmup, steps=10, progress=log)
         prm = tuned["params"]
 
     def timed_run(params: dict) -> tuple[float, int]:
@@ -135,6 +142,7 @@ def main() -> int:
 
     ms_per_step, buckets = timed_run(prm)
     default_ms = timed_run({})[0] if prm else ms_per_step
+    log(f"plain leg: {ms_per_step:.4f} ms/step tuned, {default_ms:.4f} ms/step at the default tunables")
 
     # Per-pass breakdown with hipEvents on the library's stream (same start).
     with fh.Flock.from_words(words, flags=args.flags & ~fh.FLAG_GRAPH, **prm) as f:
@@ -184,16 +192,23 @@ def main() -> int:
                      "bytes_per_launch": nbytes["decide"]},
     }
     if not args.no_cpu:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(words, n, args.cpu_sample)
     del words
     if not args.no_reference:
+        log("reference-protocol leg")
         out["reference_protocol"] = reference_leg(args)
     if not args.no_batched:
+        log("batched leg")
         out["batched"] = batched_leg(args)
+        log(f"batched leg: speedup {out['batched']['speedup']:.2f}x over plain per-flock stepping")
         if not args.no_served:
+            log("served leg")
             out["served"] = served_leg(args, out["batched"])
     if not args.no_clumps:
+        log("clumps leg")
         out["clumps"] = clumps_leg(args)
+    log("done")
     print(json.dumps(out), flush=True)
     return 0
 
@@ -231,7 +246,7 @@ def clumps_leg(args) -> dict:
     n = args.clump_birds
     w = workloads.clumps(n, 8, 5.0, 1000.0, seed=args.seed)
     tuned = autotune.tune([w], "plain", flags=fh.FLAG_GRAPH, warmup=1, steps=2,
-                          cells=(4.0, 1.0, 2.0, 3.0))
+                          cells=(4.0, 1.0, 2.0, 3.0), progress=log)
     prm = tuned["params"]
     with fh.Flock.from_words(w, flags=fh.FLAG_GRAPH, **prm) as f:
         f.step(args.clump_warmup)
@@ -267,7 +282,8 @@ def batched_leg(args) -> dict:
     starts = workloads.mix(args.flocks, seed=args.seed)
     sizes = [int(w.shape[0]) for w in starts]
     total = sum(sizes)
-    tuned = autotune.tune(starts, "batch", flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, warmup=2, steps=5)
+    tuned = autotune.tune(starts, "batch", flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, warmup=2, steps=5,
+                          progress=log)
     prm = tuned["params"]
     buckets = sum(autotune._buckets(s) for s in sizes)
     with fh.Batch(total + 256 * len(starts), buckets + 4096 * len(starts), len(starts),
@@ -345,12 +361,14 @@ def served_leg(args, batched: dict) -> dict:
         ids = [b.add(fh.gen_request(int(n), int(s))) for n, s in zip(warm_req["birds"], warm_req["seeds"])]
         b.step(250)
         warm = [b.download(i)[0] for i in ids]
+    log("served leg: autotune on 32 warmed request flocks")
     tuned = autotune.tune(warm, "batch", flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, warmup=1, steps=4,
-                          cells=(4.0, 6.0, 8.0, 10.0, 12.0))
+                          cells=(4.0, 6.0, 8.0, 10.0, 12.0), progress=log)
     prm = tuned["params"]
     del warm
     sat = workloads.stream(500, 5000.0, seed=args.seed + 1)
     capacity = _replay(fh, sat, args, prm)
+    log(f"served leg: capacity {capacity['bird_steps_per_s']:.3e} bird-steps/s")
     mean_bird_steps = 90_200 * 275.0  # E[birds] x E[steps] of the request mix
     rate = args.load * capacity["bird_steps_per_s"] / mean_bird_steps
     req = workloads.stream(args.requests, rate, seed=args.seed)

@@ -278,6 +278,7 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
     size_t o_counts = a.take<uint32_t>(nb), o_offsets = a.take<uint32_t>(nb + 1);
     size_t o_tsum = a.take<uint32_t>(b->bucket_cap_units + 1), o_tpre = a.take<uint32_t>(b->bucket_cap_units + 1);
     size_t o_err = a.take<uint32_t>(4), o_due = a.take<uint32_t>(4);
+    size_t o_bigc = a.take<uint32_t>(4), o_big = a.take<uint32_t>(big_list_cap((uint32_t)nn));
     size_t o_segs = a.take<Seg>(flock_capacity), o_tseg = a.take<uint32_t>(tile_cap ? tile_cap : 1);
     hipError_t e = hipMalloc(&b->arena, a.used);
     if (e != hipSuccess) {
@@ -304,6 +305,9 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
     b->P.tile_prefix = reinterpret_cast<uint32_t *>(base + o_tpre);
     b->P.err = reinterpret_cast<uint32_t *>(base + o_err);
     b->P.due_count = reinterpret_cast<uint32_t *>(base + o_due);
+    b->P.big_count = reinterpret_cast<uint32_t *>(base + o_bigc);
+    b->P.big = reinterpret_cast<uint32_t *>(base + o_big);
+    b->P.big_cap = big_list_cap((uint32_t)nn);
     b->d_segs = reinterpret_cast<Seg *>(base + o_segs);
     b->d_tile_seg = reinterpret_cast<uint32_t *>(base + o_tseg);
     b->tile_seg.assign(tile_cap, NO_SEG);

@@ -181,6 +181,7 @@ fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
     size_t o_offsets = a.take<uint32_t>((size_t)h->padded_buckets + 1);
     size_t o_tsum = a.take<uint32_t>(ntiles), o_tpre = a.take<uint32_t>(ntiles);
     size_t o_err = a.take<uint32_t>(4), o_due = a.take<uint32_t>(4), o_seg = a.take<Seg>(1);
+    size_t o_bigc = a.take<uint32_t>(4), o_big = a.take<uint32_t>(big_list_cap((uint32_t)nn));
     hipError_t e = hipMalloc(&h->arena, a.used);
     if (e != hipSuccess) {
         delete h;
@@ -205,6 +206,9 @@ fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
     h->P.tile_prefix = reinterpret_cast<uint32_t *>(base + o_tpre);
     h->P.err = reinterpret_cast<uint32_t *>(base + o_err);
     h->P.due_count = reinterpret_cast<uint32_t *>(base + o_due);
+    h->P.big_count = reinterpret_cast<uint32_t *>(base + o_bigc);
+    h->P.big = reinterpret_cast<uint32_t *>(base + o_big);
+    h->P.big_cap = big_list_cap((uint32_t)nn);
     h->d_seg = reinterpret_cast<Seg *>(base + o_seg);
 
     h->seg.bird_base = 0;

@@ -118,8 +118,12 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t *__restrict_
 __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *__restrict__ tile_sums,
                                                      uint32_t *__restrict__ tile_prefix, uint32_t ntiles,
                                                      uint32_t *__restrict__ offsets_end,
-                                                     uint32_t *__restrict__ due_count) {
-    if (threadIdx.x == 0) *due_count = 0;  // a new grid starts a new step
+                                                     uint32_t *__restrict__ due_count,
+                                                     uint32_t *__restrict__ big_count) {
+    if (threadIdx.x == 0) {  // a new grid starts a new step
+        *due_count = 0;
+        *big_count = 0;
+    }
     uint32_t carry = 0;
     for (uint32_t base = 0; base < ntiles; base += 4096u) {
         uint32_t v[4], sum = 0;
@@ -199,10 +203,23 @@ __global__ __launch_bounds__(TILE) void k_scatter(Geometry g, Pools p, int canon
     p.snext[s] = relative ? q.w - sg.time : q.w;
 }
 
-// Canonical order: each bird moves to bucket start + number of birds of its
-// bucket with a smaller index, so a bucket lists its birds in ascending order
-// (the order of a stable counting sort over bird index). Buckets larger than
-// CANON_CAP keep the arrival order and raise ERR_CANON.
+// The sorted copies of `bird` at slot c (bird < g.n_birds, c < g.n_slots).
+__device__ __forceinline__ void write_sorted(const Geometry &g, const Pools &p, uint32_t c, uint32_t bird,
+                                             int relative) {
+    const float4 q = p.pos4[bird];
+    const float4 v = p.vel4[bird];
+    float t = 0.0f;
+    if (relative) t = g.segs[g.tile_seg ? g.tile_seg[bird / TILE] : 0u].time;
+    p.spos[c] = make_float4(q.x, q.y, q.z, __uint_as_float(bird));
+    p.svel[c] = make_float4(v.x, v.y, v.z, 0.0f);
+    p.snext[c] = relative ? q.w - t : q.w;
+}
+
+// Canonical order: each bucket lists its birds in ascending bird index (the
+// order of a stable counting sort over bird index). A bird of a bucket with
+// at most CANON_CAP birds moves to bucket start + number of birds of its
+// bucket with a smaller index. Larger buckets are put on the big-bucket list
+// (by their first slot) and left to k_canonical_big.
 __global__ __launch_bounds__(256) void k_canonical(Geometry g, Pools p, int relative) {
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
     if (s >= g.n_slots) return;
@@ -213,20 +230,73 @@ __global__ __launch_bounds__(256) void k_canonical(Geometry g, Pools p, int rela
     }
     const uint32_t k = p.key[bird];
     const uint32_t o0 = p.offsets[k], o1 = min(p.offsets[k + 1], g.n_slots);
-    uint32_t c = s;
-    if (o1 - o0 <= CANON_CAP && o0 <= s && s < o1) {
-        c = o0;
-        for (uint32_t t = o0; t < o1; ++t) c += p.sorted_tmp[t] < bird;
-    } else if (s == o0) {
-        atomicOr(p.err, ERR_CANON);
+    if (!(o0 <= s && s < o1)) {  // inconsistent grid: keep the slot
+        atomicOr(p.err, ERR_SLOT);
+        write_sorted(g, p, s, bird, relative);
+        return;
     }
-    const float4 q = p.pos4[bird];
-    const float4 v = p.vel4[bird];
-    float t = 0.0f;
-    if (relative) t = g.segs[g.tile_seg ? g.tile_seg[bird / TILE] : 0u].time;
-    p.spos[c] = make_float4(q.x, q.y, q.z, __uint_as_float(bird));
-    p.svel[c] = make_float4(v.x, v.y, v.z, 0.0f);
-    p.snext[c] = relative ? q.w - t : q.w;
+    if (o1 - o0 > CANON_CAP) {
+        if (s == o0) {
+            // Listed buckets are disjoint slot ranges of more than CANON_CAP
+            // slots each, so big_cap entries always suffice.
+            const uint32_t i = atomicAdd(p.big_count, 1u);
+            if (i < p.big_cap) p.big[i] = k;
+            else atomicOr(p.err, ERR_SLOT);
+        }
+        return;
+    }
+    uint32_t c = o0;
+    for (uint32_t t = o0; t < o1; ++t) c += p.sorted_tmp[t] < bird;
+    write_sorted(g, p, c, bird, relative);
+}
+
+// Canonical order of the big buckets: one workgroup per listed bucket (a
+// grid-stride loop over the list) loads the bucket's bird indices into LDS,
+// bitonic-sorts them and writes the sorted copies in that order. Buckets of
+// more than CANON_BIG_CAP birds keep the arrival order and raise ERR_CANON.
+constexpr uint32_t BIG_BLOCK = 1024;
+__global__ __launch_bounds__(BIG_BLOCK) void k_canonical_big(Geometry g, Pools p, int relative) {
+    __shared__ uint32_t ids[CANON_BIG_CAP];
+    const uint32_t nbig = min(*p.big_count, p.big_cap);
+    for (uint32_t i = blockIdx.x; i < nbig; i += gridDim.x) {
+        const uint32_t k = p.big[i];
+        const uint32_t o0 = p.offsets[k], o1 = min(p.offsets[k + 1], g.n_slots);
+        const uint32_t m = o1 > o0 ? o1 - o0 : 0u;
+        if (m > CANON_BIG_CAP) {
+            if (threadIdx.x == 0) atomicOr(p.err, ERR_CANON);
+            for (uint32_t j = threadIdx.x; j < m; j += BIG_BLOCK) {
+                const uint32_t bird = p.sorted_tmp[o0 + j];
+                if (bird < g.n_birds) write_sorted(g, p, o0 + j, bird, relative);
+                else atomicOr(p.err, ERR_SLOT);
+            }
+            continue;
+        }
+        uint32_t size = 2u;
+        while (size < m) size <<= 1;
+        for (uint32_t j = threadIdx.x; j < size; j += BIG_BLOCK) ids[j] = j < m ? p.sorted_tmp[o0 + j] : 0xffffffffu;
+        __syncthreads();
+        for (uint32_t span = 2u; span <= size; span <<= 1) {
+            for (uint32_t stride = span >> 1; stride > 0u; stride >>= 1) {
+                // pair j = (lo, lo + stride) where lo is j with a zero bit inserted at log2(stride)
+                for (uint32_t j = threadIdx.x; j < (size >> 1); j += BIG_BLOCK) {
+                    const uint32_t lo = 2u * j - (j & (stride - 1u));
+                    const uint32_t a = ids[lo], b = ids[lo + stride];
+                    const bool ascending = (lo & span) == 0u;
+                    if ((a > b) == ascending) {
+                        ids[lo] = b;
+                        ids[lo + stride] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t j = threadIdx.x; j < m; j += BIG_BLOCK) {
+            const uint32_t bird = ids[j];
+            if (bird < g.n_birds) write_sorted(g, p, o0 + j, bird, relative);
+            else atomicOr(p.err, ERR_SLOT);
+        }
+        __syncthreads();  // ids is refilled for the next bucket
+    }
 }
 
 // Fill kernel (used instead of hipMemsetAsync so a captured graph holds only
@@ -269,7 +339,7 @@ hipError_t launch_scan(const Geometry &g, const Pools &p, hipStream_t s) {
     const uint32_t ntiles = (g.n_buckets + SCAN_TILE - 1) / SCAN_TILE;
     hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_sums);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, p.tile_sums, p.tile_prefix, ntiles,
-                       p.offsets + (size_t)ntiles * SCAN_TILE, p.due_count);
+                       p.offsets + (size_t)ntiles * SCAN_TILE, p.due_count, p.big_count);
     hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_prefix, p.offsets);
     return hipGetLastError();
 }
@@ -279,9 +349,13 @@ hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, boo
     if (g.n_birds == 0) return hipSuccess;
     hipLaunchKernelGGL(k_scatter, dim3(launch_tiles(g)), dim3(TILE), 0, s, g, p, canonical ? 1 : 0,
                        relative_time ? 1 : 0);
-    if (canonical && g.n_slots)
+    if (canonical && g.n_slots) {
         hipLaunchKernelGGL(k_canonical, dim3((g.n_slots + 255u) / 256u), dim3(256), 0, s, g, p,
                            relative_time ? 1 : 0);
+        // the list length is known only on the device: a fixed grid walks it
+        const uint32_t blocks = min(big_list_cap(g.n_slots), 256u);
+        hipLaunchKernelGGL(k_canonical_big, dim3(blocks), dim3(BIG_BLOCK), 0, s, g, p, relative_time ? 1 : 0);
+    }
     return hipGetLastError();
 }
 

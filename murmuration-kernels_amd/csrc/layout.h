@@ -30,12 +30,17 @@ namespace fh {
 
 constexpr uint32_t TILE = 256;        // birds per workgroup of the bird-order passes
 constexpr uint32_t SCAN_TILE = 4096;  // buckets per workgroup of the scan (256 lanes x 16)
-constexpr uint32_t CANON_CAP = 1024;  // largest bucket the canonical pass reorders
+constexpr uint32_t CANON_CAP = 1024;  // largest bucket the per-slot canonical pass reorders
+constexpr uint32_t CANON_BIG_CAP = 16384;  // largest bucket the LDS sort of big buckets reorders
 constexpr uint32_t NO_SEG = 0xffffffffu;
+
+// Entries of the big-bucket list for `slots` sorted birds: every listed
+// bucket holds more than CANON_CAP of them.
+constexpr uint32_t big_list_cap(uint32_t slots) { return slots / (CANON_CAP + 1u) + 1u; }
 
 // Error bits a kernel may raise in the device error word.
 constexpr uint32_t ERR_SLOT = 1u;     // a scatter slot fell outside [0, n_slots)
-constexpr uint32_t ERR_CANON = 2u;    // a bucket exceeded CANON_CAP: left in arrival order
+constexpr uint32_t ERR_CANON = 2u;    // a bucket exceeded CANON_BIG_CAP: left in arrival order
 constexpr uint32_t ERR_NBCAP = 4u;    // neighbour list longer than the caller's buffer
 
 struct Seg {
@@ -60,6 +65,9 @@ struct Pools {
     uint32_t *sorted_tmp;  // canonical mode: slot -> bird before reordering
     uint32_t *err;         // device error word (ERR_* bits)
     uint32_t *due_count;   // birds that decided in the current step (zeroed by the scan)
+    uint32_t *big;         // canonical mode: buckets with more than CANON_CAP birds
+    uint32_t *big_count;   // entries of `big` this step (zeroed by the scan)
+    uint32_t big_cap;      // capacity of `big`: big_list_cap(bird capacity)
 };
 
 // Launch geometry of one build/step over the pools.

@@ -73,9 +73,10 @@ def _buckets(n: int) -> int:
 
 
 def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, steps: int = 8,
-         cache_path: str | None = None, cells=CELLS) -> dict:
+         cache_path: str | None = None, cells=CELLS, progress=None) -> dict:
     """Returns {"params": {...}, "ms_per_step": x, "trials": [...]} for the
-    batch of `starts` (kind "batch") or the single flock starts[0] ("plain")."""
+    batch of `starts` (kind "batch") or the single flock starts[0] ("plain").
+    `progress`, if given, is called with one line of text per trial."""
     sizes = [int(w.shape[0]) for w in starts]
     key = _signature(kind, sizes, steps, flags)
     cache = _load(cache_path) if cache_path else {}
@@ -89,6 +90,8 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
     def run(prm):
         t = timer(prm)
         trials.append({**prm, "ms": t * 1e3})
+        if progress:
+            progress(f"{kind} trial {len(trials)}: {prm} {t * 1e3:.4f} ms/step")
         return t
 
     best_t = run(best)

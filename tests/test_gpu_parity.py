@@ -76,6 +76,31 @@ def test_canonical_order_is_the_stable_sort(fh, fr, n):
         assert np.array_equal(g.sorted, srt)
 
 
+def test_canonical_order_in_big_buckets(fh, fr):
+    # Cells of 1500, 5000 and 12000 birds take the LDS sort of big buckets
+    # (bit-exact stable order); 17000 birds exceed its 16384 cap and keep the
+    # arrival order (set equality). Bird indices of a cell are interleaved.
+    rng = np.random.default_rng(4)
+    n = 40_000
+    w = fh.gen_reference(n, 5)
+    sizes = [1500, 5000, 12000, 17000]
+    idx = rng.permutation(n)
+    at = 0
+    for c, m in enumerate(sizes):
+        pick = idx[at:at + m]
+        w[pick, 0:3] = (rng.random((m, 3)) * 3.9 + np.array([0.05 + 40.0 * c, 0.05, 100.05])).astype(np.float32)
+        at += m
+    with fh.Flock.from_words(w, flags=fh.FLAG_CANONICAL) as f:
+        g = build_grid(f)
+        _, counts, offsets, srt = fr.grid(w, f.buckets)
+    assert np.array_equal(g.offsets, offsets)
+    big = np.nonzero(counts > 16384)[0]
+    assert len(big) == 1 and np.any(counts[counts <= 16384] >= 12000)
+    s, e = offsets[big[0]], offsets[big[0] + 1]
+    assert set(g.sorted[s:e].tolist()) == set(srt[s:e].tolist())
+    assert np.array_equal(np.delete(g.sorted, np.arange(s, e)), np.delete(srt, np.arange(s, e)))
+
+
 def test_dense_cell_and_hash_collisions(fh, fr):
     # 3000 birds inside one 4 m cell plus a spread, hashed into only 64 buckets.
     rng = np.random.default_rng(1)
