@@ -94,6 +94,7 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--load", type=float, default=0.6, help="offered load / measured server capacity")
+    ap.add_argument("--plain-sample", type=int, default=32, help="served requests also stepped alone")
     ap.add_argument("--copy-cal", action="store_true",
                     help="only run a 256 MiB device-to-device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -120,19 +121,19 @@ def main() -> int:
     dev = fh.device_info()
 
     # Tunables of the plain step, picked by the autotuner on this workload.
-    prm, tuned = {}, None
+    prm, tuned, flags = {}, None, args.flags
     if not args.no_tune:
         from flock_hip import autotune
 
         # trials time the same window as the timed run (radii grow each decision)
         log(f"plain leg: autotune at {n} birds")
         tuned = autotune.tune([words], "plain", flags=args.flags, warmup=args.warmup, steps=10, progress=log)
-        prm = tuned["params"]
+        prm, flags = tuned["params"], tuned["flags"]
 
-    def timed_run(params: dict) -> tuple[float, int]:
+    def timed_run(params: dict, flags: int) -> tuple[float, int]:
         """Fused step replayed from a hipGraph; wall clock over K steps,
         synchronised on both sides, inputs resident in HBM."""
-        with fh.Flock.from_words(words, flags=args.flags, **params) as f:
+        with fh.Flock.from_words(words, flags=flags, **params) as f:
             f.step(args.warmup)
             f.sync()
             t0 = time.perf_counter()
@@ -140,12 +141,12 @@ def main() -> int:
             f.sync()
             return (time.perf_counter() - t0) * 1000.0 / args.steps, f.buckets
 
-    ms_per_step, buckets = timed_run(prm)
-    default_ms = timed_run({})[0] if prm else ms_per_step
+    ms_per_step, buckets = timed_run(prm, flags)
+    default_ms = timed_run({}, args.flags)[0] if prm else ms_per_step
     log(f"plain leg: {ms_per_step:.4f} ms/step tuned, {default_ms:.4f} ms/step at the default tunables")
 
     # Per-pass breakdown with hipEvents on the library's stream (same start).
-    with fh.Flock.from_words(words, flags=args.flags & ~fh.FLAG_GRAPH, **prm) as f:
+    with fh.Flock.from_words(words, flags=flags & ~fh.FLAG_GRAPH, **prm) as f:
         f.step(args.warmup)
         passes = f.step(args.steps, 0, timing=True)
     due_last = int(passes["due_last"])  # birds that decided in the last timed step
@@ -174,7 +175,7 @@ def main() -> int:
                         f"isotropic headings 10+-1 m/s, seed {args.seed}",
             "birds": n,
             "buckets": buckets,
-            "flags": args.flags,
+            "flags": flags,
             "params": {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, **prm},
             "autotuned": tuned is not None,
             "device": dev["name"],
@@ -246,16 +247,16 @@ def clumps_leg(args) -> dict:
     n = args.clump_birds
     w = workloads.clumps(n, 8, 5.0, 1000.0, seed=args.seed)
     tuned = autotune.tune([w], "plain", flags=fh.FLAG_GRAPH, warmup=1, steps=2,
-                          cells=(4.0, 1.0, 2.0, 3.0), progress=log)
-    prm = tuned["params"]
-    with fh.Flock.from_words(w, flags=fh.FLAG_GRAPH, **prm) as f:
+                          cells=(4.0, 1.0, 2.0, 3.0), progress=log, thorough=False)
+    prm, flags = tuned["params"], tuned["flags"]
+    with fh.Flock.from_words(w, flags=flags, **prm) as f:
         f.step(args.clump_warmup)
         f.sync()
         tt = time.perf_counter()
         f.step(args.clump_steps)
         f.sync()
         wall = time.perf_counter() - tt
-    with fh.Flock.from_words(w, **prm) as f:
+    with fh.Flock.from_words(w, flags=flags & ~fh.FLAG_GRAPH, **prm) as f:
         f.step(args.clump_warmup)
         passes = f.step(args.clump_steps, 0, timing=True)
     return {
@@ -266,6 +267,7 @@ def clumps_leg(args) -> dict:
         "passes_ms": {k: passes[k] for k in ("grid_scan", "grid_scatter", "decide", "integrate", "step_total")},
         "due_birds_last_step": passes["due_last"],
         "autotuned": prm,
+        "flags": flags,
         "autotune_trials": len(tuned["trials"]),
         "leg_seconds": time.perf_counter() - t0,
     }
@@ -283,11 +285,10 @@ def batched_leg(args) -> dict:
     sizes = [int(w.shape[0]) for w in starts]
     total = sum(sizes)
     tuned = autotune.tune(starts, "batch", flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, warmup=2, steps=5,
-                          progress=log)
-    prm = tuned["params"]
+                          progress=log, thorough=False)
+    prm, flags = tuned["params"], tuned["flags"]
     buckets = sum(autotune._buckets(s) for s in sizes)
-    with fh.Batch(total + 256 * len(starts), buckets + 4096 * len(starts), len(starts),
-                  flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, **prm) as b:
+    with fh.Batch(total + 256 * len(starts), buckets + 4096 * len(starts), len(starts), flags=flags, **prm) as b:
         for w in starts:
             b.add(w)
         b.step(3)
@@ -315,15 +316,16 @@ def batched_leg(args) -> dict:
         "batched_ms_per_step": batched_s * 1e3 / args.batch_steps,
         "plain_ms_per_step": plain_s * 1e3 / args.batch_steps,
         "autotuned": prm,
+        "flags": flags,
         "autotune_trials": len(tuned["trials"]),
         "leg_seconds": time.perf_counter() - t0,
     }
 
 
-def _replay(fh, req, args, prm) -> dict:
+def _replay(fh, req, args, prm, flags) -> dict:
     res, st = fh.server_replay(req["birds"], req["steps"], req["seeds"], req["arrival_s"],
                                max_active=args.slots, hbm_budget_bytes=200e9, steps_per_iter=8,
-                               flags=fh.FLAG_CANONICAL, **prm)
+                               flags=flags, **prm)
     ok = [r for r in res if r["status"] == 0]
     lat = np.array([r["done_s"] - r["arrival_s"] for r in ok])
     span = max(r["done_s"] for r in ok) - min(r["arrival_s"] for r in ok)
@@ -341,6 +343,22 @@ def _replay(fh, req, args, prm) -> dict:
     }
 
 
+def _plain_requests(fh, req, count: int, prm: dict, flags: int) -> dict:
+    """The first `count` requests of the stream, each stepped alone, one after
+    another (one flock per launch sequence, hipGraph): upload, steps and
+    download timed; start states generated outside the timed region."""
+    total_s, bird_steps = 0.0, 0.0
+    for b, st, seed in list(zip(req["birds"], req["steps"], req["seeds"]))[:count]:
+        w = fh.gen_request(int(b), int(seed))
+        t0 = time.perf_counter()
+        with fh.Flock.from_words(w, flags=flags | fh.FLAG_GRAPH, **prm) as f:
+            f.step(int(st))
+            f.download()
+        total_s += time.perf_counter() - t0
+        bird_steps += float(b) * float(st)
+    return {"requests": count, "bird_steps_per_s": bird_steps / total_s, "seconds": total_s}
+
+
 def served_leg(args, batched: dict) -> dict:
     """Flock-simulation requests (birds log-uniform 2e3..5e5, steps uniform
     50..500, own seeds) through the continuous-batching server: admission by
@@ -356,23 +374,25 @@ def served_leg(args, batched: dict) -> dict:
 
     t0 = time.perf_counter()
     warm_req = workloads.stream(32, 1.0, seed=args.seed + 7)
-    warm = []
     with fh.Batch(40 << 20, 64 << 20, 64, flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL) as b:
         ids = [b.add(fh.gen_request(int(n), int(s))) for n, s in zip(warm_req["birds"], warm_req["seeds"])]
         b.step(250)
-        warm = [b.download(i)[0] for i in ids]
+        warm = [b.download(i) for i in ids]
     log("served leg: autotune on 32 warmed request flocks")
-    tuned = autotune.tune(warm, "batch", flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, warmup=1, steps=4,
+    tuned = autotune.tune([w for w, _ in warm], "batch", flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, warmup=1, steps=4,
+                          times=[t for _, t in warm],
                           cells=(4.0, 6.0, 8.0, 10.0, 12.0), progress=log)
-    prm = tuned["params"]
+    prm, flags = tuned["params"], tuned["flags"] & ~fh.FLAG_GRAPH
     del warm
     sat = workloads.stream(500, 5000.0, seed=args.seed + 1)
-    capacity = _replay(fh, sat, args, prm)
+    capacity = _replay(fh, sat, args, prm, flags)
     log(f"served leg: capacity {capacity['bird_steps_per_s']:.3e} bird-steps/s")
     mean_bird_steps = 90_200 * 275.0  # E[birds] x E[steps] of the request mix
     rate = args.load * capacity["bird_steps_per_s"] / mean_bird_steps
     req = workloads.stream(args.requests, rate, seed=args.seed)
-    stream = _replay(fh, req, args, prm)
+    stream = _replay(fh, req, args, prm, flags)
+    log(f"served leg: stream p50 {stream['p50_latency_s']:.3f} s, p99 {stream['p99_latency_s']:.3f} s")
+    plain = _plain_requests(fh, req, args.plain_sample, prm, flags)
     return {
         "workload": f"{args.requests} requests, Poisson {rate:.1f}/s ({args.load:.0%} of the measured capacity), "
                     f"birds log-uniform 2e3..5e5, steps uniform 50..500, slot cap {args.slots}, 200 GB HBM budget "
@@ -381,7 +401,10 @@ def served_leg(args, batched: dict) -> dict:
         "rate_per_s": rate,
         **stream,
         "capacity": capacity,
+        "plain": plain,
+        "capacity_vs_plain": capacity["bird_steps_per_s"] / plain["bird_steps_per_s"],
         "autotuned": prm,
+        "flags": flags,
         "leg_seconds": time.perf_counter() - t0,
     }
 

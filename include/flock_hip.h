@@ -78,6 +78,12 @@ typedef struct fh_params {
 /* Never use decide's tiled path (cells shared by a workgroup's due birds
  * staged in LDS); its results are bit-identical, this is for checking. */
 #define FH_FLAG_NO_TILE 16u
+/* decide tests the bucket occupancy bitmap written by the scan before it
+ * reads a bucket's range, so empty buckets cost a lookup in a small table
+ * instead of one in the offsets table (pays off when most buckets are
+ * empty and the offsets table is far larger than the cache); results are
+ * bit-identical. */
+#define FH_FLAG_OCC 32u
 
 /* Per-pass GPU time of fh_step, mean ms per timed step (hipEvents on the
  * handle's stream), in the rows of the reference report
@@ -223,7 +229,7 @@ typedef struct fh_result {
     int32_t status;            /* FH_OK or the error that ended the request */
     float final_time;          /* flock clock at completion */
     double arrival_s, admit_s, done_s;  /* seconds since the server started */
-    uint64_t checksum;         /* FNV-1a 64 of the final .flk records */
+    uint64_t checksum;         /* FNV-1a 64 over the 32-bit words of the final .flk records */
 } fh_result;
 
 typedef struct fh_server_stats {

@@ -77,16 +77,18 @@ static void refresh_geometry(fh_batch *b) {
     b->graph_valid = false;
 }
 
-static fh_status push_tables(fh_batch *b) {
+// Uploads the segment table and the tile -> segment entries [t_lo, t_hi).
+static fh_status push_tables(fh_batch *b, uint32_t t_lo, uint32_t t_hi) {
     // The device advances each clock by DT per step; the host mirror in
     // times[] does the same f32 additions, so it is the current value.
     for (size_t f = 0; f < b->segs.size(); ++f) b->segs[f].time = b->times[f];
     if (!b->segs.empty())
         FH_HIP(hipMemcpyAsync(b->d_segs, b->segs.data(), b->segs.size() * sizeof(Seg), hipMemcpyHostToDevice,
                               b->stream));
-    if (!b->tile_seg.empty())
-        FH_HIP(hipMemcpyAsync(b->d_tile_seg, b->tile_seg.data(), b->tile_seg.size() * 4, hipMemcpyHostToDevice,
-                              b->stream));
+    t_hi = std::min<uint32_t>(t_hi, (uint32_t)b->tile_seg.size());
+    if (t_lo < t_hi)
+        FH_HIP(hipMemcpyAsync(b->d_tile_seg + t_lo, b->tile_seg.data() + t_lo, (size_t)(t_hi - t_lo) * 4,
+                              hipMemcpyHostToDevice, b->stream));
     FH_HIP(hipStreamSynchronize(b->stream));
     return FH_OK;
 }
@@ -115,7 +117,7 @@ static fh_status batch_step_launches(fh_batch *b, hipEvent_t *ev) {
     FH_HIP(launch_scatter(b->geo, b->P, (b->prm.flags & FH_FLAG_CANONICAL) != 0, true, b->stream));
     if (ev) FH_HIP(hipEventRecord(ev[3], b->stream));
     FH_HIP(launch_decide(b->geo, b->P, 0.0f, b->prm.decide_block, b->prm.birds_per_thread, b->prm.decide_group,
-                         !(b->prm.flags & FH_FLAG_NO_TILE), b->stream));
+                         walk_flags(b->prm.flags), b->stream));
     if (ev) FH_HIP(hipEventRecord(ev[4], b->stream));
     FH_HIP(launch_integrate(b->geo, b->P, true, (b->prm.flags & FH_FLAG_WAVE_AGG) != 0, b->stream));
     b->counts_dirty = true;
@@ -279,6 +281,7 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
     size_t o_tsum = a.take<uint32_t>(b->bucket_cap_units + 1), o_tpre = a.take<uint32_t>(b->bucket_cap_units + 1);
     size_t o_err = a.take<uint32_t>(4), o_due = a.take<uint32_t>(4);
     size_t o_bigc = a.take<uint32_t>(4), o_big = a.take<uint32_t>(big_list_cap((uint32_t)nn));
+    size_t o_occ = a.take<uint32_t>(nb / 32);
     size_t o_segs = a.take<Seg>(flock_capacity), o_tseg = a.take<uint32_t>(tile_cap ? tile_cap : 1);
     hipError_t e = hipMalloc(&b->arena, a.used);
     if (e != hipSuccess) {
@@ -308,6 +311,7 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
     b->P.big_count = reinterpret_cast<uint32_t *>(base + o_bigc);
     b->P.big = reinterpret_cast<uint32_t *>(base + o_big);
     b->P.big_cap = big_list_cap((uint32_t)nn);
+    b->P.occ = reinterpret_cast<uint32_t *>(base + o_occ);
     b->d_segs = reinterpret_cast<Seg *>(base + o_segs);
     b->d_tile_seg = reinterpret_cast<uint32_t *>(base + o_tseg);
     b->tile_seg.assign(tile_cap, NO_SEG);
@@ -324,7 +328,7 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
         delete b;
         return hip_fail(e, "hipMemsetAsync(batch arena)");
     }
-    if ((st = push_tables(b))) {
+    if ((st = push_tables(b, 0, tile_cap))) {
         (void)hipStreamDestroy(b->stream);
         (void)hipFree(b->arena);
         delete b;
@@ -407,7 +411,7 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
     FH_HIP(hipMemcpyAsync(b->P.spin + o, spin.data(), n * 4, hipMemcpyHostToDevice, b->stream));
     FH_HIP(hipMemcpyAsync(b->P.cen + o, cen.data(), n * 4, hipMemcpyHostToDevice, b->stream));
     FH_HIP(hipMemcpyAsync(b->P.nb + o, nbv.data(), n * 4, hipMemcpyHostToDevice, b->stream));
-    if ((st = push_tables(b))) return st;
+    if ((st = push_tables(b, t0, t0 + ntile))) return st;
     // Count the new flock's birds into the histogram if the others are already
     // counted; otherwise the next step hashes everything.
     refresh_geometry(b);
@@ -435,7 +439,7 @@ fh_status fh_batch_remove(fh_batch *b, uint32_t flock_id) {
     b->tiles.release(t0);
     b->bucket_units.release(s.bucket_base / SCAN_TILE);
     s.live = 0;
-    if ((st = push_tables(b))) return st;
+    if ((st = push_tables(b, t0, t0 + ntile))) return st;
     refresh_geometry(b);
     return FH_OK;
 }
@@ -502,7 +506,7 @@ size_t fh_flock_bytes(uint32_t n, const fh_params *params) {
     const uint32_t buckets = flock_buckets(n, prm);
     const size_t birds = (size_t)((n + TILE - 1) / TILE) * TILE;
     const size_t bk = (size_t)((buckets + SCAN_TILE - 1) / SCAN_TILE) * SCAN_TILE;
-    return birds * batch_bytes_per_bird() + bk * batch_bytes_per_bucket();
+    return birds * batch_bytes_per_bird() + bk * batch_bytes_per_bucket() + bk / 8 /*occupancy bits*/;
 }
 
 }  // extern "C"

@@ -57,16 +57,22 @@ __device__ __forceinline__ Walk make_walk(f3 p, float r, const Seg &sg) {
     return w;
 }
 
+__device__ __forceinline__ bool bucket_occupied(const Pools &p, uint32_t b) {
+    return (p.occ[b >> 5] >> (b & 31u)) & 1u;
+}
+
 // Cells of the box are visited in z-y-x order (x fastest), as the nested
 // loops of cuda/flock.cu:209-211. A lane visits every G-th cell starting at
 // `first` (G = 1: every cell). Its cells are taken PF at a time: the bucket
 // ranges of all PF cells are loaded together, then the first candidate of
 // each non-empty bucket, so a lane has up to 2 * PF independent loads in
-// flight instead of a chain of dependent ones. Within a lane the visit order
+// flight instead of a chain of dependent ones. With `use_occ` the occupancy
+// bits of the PF buckets are read first (a small, cache-resident table) and
+// empty buckets never touch the offsets table. Within a lane the visit order
 // is the walk order.
 template <int G, int PF, class Visit>
 __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const Pools &p, uint32_t n_slots,
-                                           uint32_t me, int32_t first, Visit &&visit) {
+                                           uint32_t me, int32_t first, bool use_occ, Visit &&visit) {
     if (!w.ok) return;
     const int32_t nx = w.hi[0] - w.lo[0] + 1, ny = w.hi[1] - w.lo[1] + 1, nz = w.hi[2] - w.lo[2] + 1;
     const int32_t ncell = nx * ny * nz;
@@ -84,7 +90,8 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
     advance(first);
     for (int32_t base = first; base < ncell; base += G * PF) {
         int32_t cx[PF], cy[PF], cz[PF];
-        uint32_t s0[PF], s1[PF];
+        uint32_t s0[PF], s1[PF], bk[PF];
+        bool want[PF];
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             cx[k] = x;
@@ -92,11 +99,8 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
             cz[k] = z;
             s0[k] = 0u;
             s1[k] = 0u;
-            if (base + k * G < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2)) {
-                const uint32_t b = sg.bucket_base + (bucket_hash(x, y, z) & sg.mask);
-                s0[k] = p.offsets[b];
-                s1[k] = p.offsets[b + 1];
-            }
+            want[k] = base + k * G < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2);
+            bk[k] = sg.bucket_base + (bucket_hash(x, y, z) & sg.mask);
 #pragma unroll
             for (int s = 0; s < G; ++s)
                 if (++x > w.hi[0]) {
@@ -107,6 +111,19 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
                     }
                 }
         }
+        if (use_occ) {
+            uint32_t word[PF];
+#pragma unroll
+            for (int k = 0; k < PF; ++k) word[k] = want[k] ? p.occ[bk[k] >> 5] : 0u;
+#pragma unroll
+            for (int k = 0; k < PF; ++k) want[k] = (word[k] >> (bk[k] & 31u)) & 1u;
+        }
+#pragma unroll
+        for (int k = 0; k < PF; ++k)
+            if (want[k]) {
+                s0[k] = p.offsets[bk[k]];
+                s1[k] = p.offsets[bk[k] + 1];
+            }
         float4 head[PF];
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
@@ -169,10 +186,10 @@ __device__ __forceinline__ void accumulate(Acc &a, const Pools &p, f3 f_i, float
 
 template <int G, int PF>
 __device__ __forceinline__ Acc walk_sums(const Pools &p, const Seg &sg, uint32_t n_slots, uint32_t me, const Walk &w,
-                                         f3 f_i, int32_t first) {
+                                         f3 f_i, int32_t first, bool use_occ) {
     Acc a;
     acc_zero(a);
-    walk_cells<G, PF>(w, sg, p, n_slots, me, first,
+    walk_cells<G, PF>(w, sg, p, n_slots, me, first, use_occ,
                       [&](uint32_t t, f3 d, float d2) { accumulate(a, p, f_i, w.r2, t, d, d2); });
     return a;
 }
@@ -236,7 +253,8 @@ constexpr uint32_t NO_BIRD = 0xffffffffu;
 // so its sums are bit-identical to the per-lane path.
 template <int BLOCK, int G>
 __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float time, uint32_t chunk,
-                                                  int allow_tiled) {
+                                                  int walk_flags) {
+    const bool allow_tiled = (walk_flags & WALK_TILED) != 0, use_occ = (walk_flags & WALK_OCC) != 0;
     constexpr int PF = G >= 4 ? 4 : 8;
     extern __shared__ uint32_t due_slots[];
     __shared__ uint32_t n_due;
@@ -273,11 +291,11 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
                 active = false;
             } else {
                 si = g.tile_seg ? g.tile_seg[me / TILE] : 0u;
-                const float4 v = p.svel[s];
+                const float4 v = p.svel[s];  // w: the bird's radius
                 pos = mk3(q.x, q.y, q.z);
                 vel = mk3(v.x, v.y, v.z);
                 f_i = forward_of(vel);
-                r = p.steer4[me].w;
+                r = v.w;
                 w = make_walk(pos, r, g.segs[si]);
             }
         }
@@ -324,6 +342,7 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
                                           cell_meets_sphere(cx, cy, cz, sg.cell, w.p, w.c2);
                         if (!__syncthreads_or(need)) continue;
                         const uint32_t b = sg.bucket_base + (bucket_hash(cx, cy, cz) & sg.mask);
+                        if (use_occ && !bucket_occupied(p, b)) continue;  // same b for every lane
                         const uint32_t s0 = p.offsets[b];
                         const uint32_t s1 = min(p.offsets[b + 1], g.n_slots);
                         for (uint32_t t0 = s0; t0 < s1; t0 += BLOCK) {
@@ -352,7 +371,7 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
                         }
                     }
         } else if (active) {
-            a = walk_sums<G, PF>(p, g.segs[si], g.n_slots, me, w, f_i, (int32_t)lane_g);
+            a = walk_sums<G, PF>(p, g.segs[si], g.n_slots, me, w, f_i, (int32_t)lane_g, use_occ);
         }
         if (G == 1) __syncthreads();  // the union box is reset by the next iteration
         if (G > 1) reduce_group<G>(a);
@@ -376,7 +395,7 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
     const float r = p.steer4[bird].w;
     const Walk w = make_walk(mk3(q.x, q.y, q.z), r, sg);
     uint32_t c = 0;
-    walk_cells<1, 8>(w, sg, p, g.n_slots, bird, 0, [&](uint32_t t, f3, float d2) {
+    walk_cells<1, 8>(w, sg, p, g.n_slots, bird, 0, true, [&](uint32_t t, f3, float d2) {
         if (walk == 1 && d2 > w.r2) return;
         if (c < cap) ids[c] = __float_as_uint(p.spos[t].w);
         ++c;
@@ -385,36 +404,36 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
 }
 
 template <int BLOCK, int G>
-static void launch_decide_block(const Geometry &g, const Pools &p, float time, uint32_t chunk, int tiled,
+static void launch_decide_block(const Geometry &g, const Pools &p, float time, uint32_t chunk, int walk_flags,
                                 hipStream_t s) {
     const uint32_t blocks = (g.n_slots + chunk - 1) / chunk;
     hipLaunchKernelGGL((k_decide<BLOCK, G>), dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s, g, p, time,
-                       chunk, tiled);
+                       chunk, walk_flags);
 }
 
 template <int BLOCK>
 static hipError_t launch_decide_group(const Geometry &g, const Pools &p, float time, uint32_t chunk, uint32_t group,
-                                      int tiled, hipStream_t s) {
+                                      int walk_flags, hipStream_t s) {
     switch (group) {
-        case 1: launch_decide_block<BLOCK, 1>(g, p, time, chunk, tiled, s); break;
-        case 2: launch_decide_block<BLOCK, 2>(g, p, time, chunk, tiled, s); break;
-        case 4: launch_decide_block<BLOCK, 4>(g, p, time, chunk, tiled, s); break;
-        case 8: launch_decide_block<BLOCK, 8>(g, p, time, chunk, tiled, s); break;
+        case 1: launch_decide_block<BLOCK, 1>(g, p, time, chunk, walk_flags, s); break;
+        case 2: launch_decide_block<BLOCK, 2>(g, p, time, chunk, walk_flags, s); break;
+        case 4: launch_decide_block<BLOCK, 4>(g, p, time, chunk, walk_flags, s); break;
+        case 8: launch_decide_block<BLOCK, 8>(g, p, time, chunk, walk_flags, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipSuccess;
 }
 
 hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
-                         uint32_t birds_per_thread, uint32_t group, bool tiled, hipStream_t s) {
+                         uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s) {
     if (g.n_slots == 0) return hipSuccess;
     const uint32_t chunk = block * birds_per_thread;
     hipError_t e;
     switch (block) {
-        case 64: e = launch_decide_group<64>(g, p, time, chunk, group, tiled ? 1 : 0, s); break;
-        case 128: e = launch_decide_group<128>(g, p, time, chunk, group, tiled ? 1 : 0, s); break;
-        case 256: e = launch_decide_group<256>(g, p, time, chunk, group, tiled ? 1 : 0, s); break;
-        case 512: e = launch_decide_group<512>(g, p, time, chunk, group, tiled ? 1 : 0, s); break;
+        case 64: e = launch_decide_group<64>(g, p, time, chunk, group, walk_flags, s); break;
+        case 128: e = launch_decide_group<128>(g, p, time, chunk, group, walk_flags, s); break;
+        case 256: e = launch_decide_group<256>(g, p, time, chunk, group, walk_flags, s); break;
+        case 512: e = launch_decide_group<512>(g, p, time, chunk, group, walk_flags, s); break;
         default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;

@@ -80,7 +80,7 @@ static fh_status do_scatter(fh_flock *h, bool relative) {
 
 static fh_status do_decide(fh_flock *h, float time_arg) {
     FH_HIP(launch_decide(h->geo, h->P, time_arg, h->prm.decide_block, h->prm.birds_per_thread,
-                         h->prm.decide_group, !(h->prm.flags & FH_FLAG_NO_TILE), h->stream));
+                         h->prm.decide_group, walk_flags(h->prm.flags), h->stream));
     h->stage = Stage::Decided;
     return FH_OK;
 }
@@ -182,6 +182,7 @@ fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
     size_t o_tsum = a.take<uint32_t>(ntiles), o_tpre = a.take<uint32_t>(ntiles);
     size_t o_err = a.take<uint32_t>(4), o_due = a.take<uint32_t>(4), o_seg = a.take<Seg>(1);
     size_t o_bigc = a.take<uint32_t>(4), o_big = a.take<uint32_t>(big_list_cap((uint32_t)nn));
+    size_t o_occ = a.take<uint32_t>(h->padded_buckets / 32);
     hipError_t e = hipMalloc(&h->arena, a.used);
     if (e != hipSuccess) {
         delete h;
@@ -209,6 +210,7 @@ fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
     h->P.big_count = reinterpret_cast<uint32_t *>(base + o_bigc);
     h->P.big = reinterpret_cast<uint32_t *>(base + o_big);
     h->P.big_cap = big_list_cap((uint32_t)nn);
+    h->P.occ = reinterpret_cast<uint32_t *>(base + o_occ);
     h->d_seg = reinterpret_cast<Seg *>(base + o_seg);
 
     h->seg.bird_base = 0;

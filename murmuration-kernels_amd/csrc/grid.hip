@@ -147,10 +147,12 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *__restrict_
 }
 
 // Scan, level 3: offsets of one tile = its prefix + local exclusive scan.
-// The counts are zeroed for the next histogram as they are consumed.
+// The counts are zeroed for the next histogram as they are consumed, and the
+// occupancy bitmap records which buckets hold a bird.
 __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *__restrict__ counts,
                                                     const uint32_t *__restrict__ tile_prefix,
-                                                    uint32_t *__restrict__ offsets) {
+                                                    uint32_t *__restrict__ offsets,
+                                                    uint32_t *__restrict__ occ) {
     const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * 16u;
     uint4 *c4 = reinterpret_cast<uint4 *>(counts + base);
     uint32_t c[16];
@@ -159,9 +161,15 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *__restrict__ count
         uint4 v = c4[k];
         c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w;
     }
-    uint32_t sum = 0;
+    uint32_t sum = 0, bits = 0;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) sum += c[k];
+    for (int k = 0; k < 16; ++k) {
+        sum += c[k];
+        bits |= (c[k] != 0u ? 1u : 0u) << k;
+    }
+    // lanes 2i and 2i+1 hold the 32 buckets of one bitmap word
+    const uint32_t hi = __shfl_xor(bits, 1);
+    if ((threadIdx.x & 1u) == 0u) occ[base / 32u] = bits | (hi << 16);
     uint32_t total;
     uint32_t run = tile_prefix[blockIdx.x] + block_excl_scan<4>(sum, &total);
     uint32_t o[16];
@@ -199,7 +207,7 @@ __global__ __launch_bounds__(TILE) void k_scatter(Geometry g, Pools p, int canon
     const float4 q = p.pos4[bird];
     const float4 v = p.vel4[bird];
     p.spos[s] = make_float4(q.x, q.y, q.z, __uint_as_float(bird));
-    p.svel[s] = make_float4(v.x, v.y, v.z, 0.0f);
+    p.svel[s] = make_float4(v.x, v.y, v.z, p.steer4[bird].w);
     p.snext[s] = relative ? q.w - sg.time : q.w;
 }
 
@@ -211,7 +219,7 @@ __device__ __forceinline__ void write_sorted(const Geometry &g, const Pools &p, 
     float t = 0.0f;
     if (relative) t = g.segs[g.tile_seg ? g.tile_seg[bird / TILE] : 0u].time;
     p.spos[c] = make_float4(q.x, q.y, q.z, __uint_as_float(bird));
-    p.svel[c] = make_float4(v.x, v.y, v.z, 0.0f);
+    p.svel[c] = make_float4(v.x, v.y, v.z, p.steer4[bird].w);
     p.snext[c] = relative ? q.w - t : q.w;
 }
 
@@ -340,7 +348,7 @@ hipError_t launch_scan(const Geometry &g, const Pools &p, hipStream_t s) {
     hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_sums);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, p.tile_sums, p.tile_prefix, ntiles,
                        p.offsets + (size_t)ntiles * SCAN_TILE, p.due_count, p.big_count);
-    hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_prefix, p.offsets);
+    hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_prefix, p.offsets, p.occ);
     return hipGetLastError();
 }
 

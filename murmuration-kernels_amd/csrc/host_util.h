@@ -42,6 +42,11 @@ inline uint32_t flock_buckets(uint32_t n, const fh_params &p) {
     return b < 64u ? 64u : b;
 }
 
+// decide's walk switches (WALK_* of layout.h) for the FH_FLAG_* flags.
+inline int walk_flags(uint32_t flags) {
+    return ((flags & FH_FLAG_NO_TILE) ? 0 : 1) | ((flags & FH_FLAG_OCC) ? 2 : 0);
+}
+
 // .flk record words (crates/flock-ref/src/state.rs:4-7).
 enum Word { PX = 0, PY, PZ, VX, VY, VZ, ROLL, SPIN, RADIUS, SX, SY, SZ, NEXT, NB, CEN };
 

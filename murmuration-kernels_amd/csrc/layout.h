@@ -12,8 +12,9 @@
 //   rank[bird]       arrival order of the bird inside its bucket
 //   counts[bucket]   histogram, zeroed again by the scan
 //   offsets[bucket]  exclusive scan, buckets + 1 entries
+//   occ[bucket / 32] bit (bucket % 32) set when the bucket holds a bird (scan)
 // and the scatter writes the sorted copies decide walks, in slot order:
-//   spos[slot] = (px, py, pz, bird index bits), svel[slot] = (vx, vy, vz, 0),
+//   spos[slot] = (px, py, pz, bird index bits), svel[slot] = (vx, vy, vz, radius),
 //   snext[slot] = next_decision (or next_decision - time, see Seg::time).
 //
 // Several flocks can share one set of pools ("segments"): flock f owns birds
@@ -60,6 +61,7 @@ struct Pools {
     uint32_t *nb;
     uint32_t *key, *rank;
     uint32_t *counts, *offsets, *tile_sums, *tile_prefix;
+    uint32_t *occ;         // occupancy bitmap of the buckets (one bit each)
     float4 *spos, *svel;
     float *snext;
     uint32_t *sorted_tmp;  // canonical mode: slot -> bird before reordering
@@ -87,8 +89,11 @@ hipError_t launch_hash_count(const Geometry &g, const Pools &p, bool wave_agg, h
 hipError_t launch_scan(const Geometry &g, const Pools &p, hipStream_t s);
 hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, bool relative_time,
                           hipStream_t s);
+// Walk switches of decide (bit set = enabled).
+constexpr int WALK_TILED = 1;  // dense neighbourhoods through LDS
+constexpr int WALK_OCC = 2;    // skip empty buckets by the occupancy bitmap
 hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
-                         uint32_t birds_per_thread, uint32_t group, bool tiled, hipStream_t s);
+                         uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s);
 hipError_t launch_integrate(const Geometry &g, const Pools &p, bool hash, bool wave_agg,
                             hipStream_t s);
 hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, int walk,

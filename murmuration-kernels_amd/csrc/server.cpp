@@ -3,7 +3,8 @@
 // (fh_batch), admits waiting requests whenever slots and HBM bytes free up,
 // advances all active flocks together a few steps per iteration, and retires
 // each flock when its step count is reached. Generation of a request's start
-// state runs on a small host pool so it overlaps the device work.
+// state and the checksum of its final state run on a small host pool, so they
+// overlap the device work.
 //
 // No counterpart in the reference (one flock per process, cuda/flock.cu:374).
 #include <algorithm>
@@ -34,15 +35,23 @@ struct Job {
     uint32_t left = 0;         // steps still to run
 };
 
-uint64_t fnv1a(const float *w, size_t count) {
+// FNV-1a 64 over the 32-bit words of the records (one xor and one multiply
+// per word).
+uint64_t fnv1a_words(const float *w, size_t count) {
     uint64_t h = 1469598103934665603ull;
-    const unsigned char *p = reinterpret_cast<const unsigned char *>(w);
-    for (size_t i = 0; i < count * 4; ++i) {
+    const uint32_t *p = reinterpret_cast<const uint32_t *>(w);
+    for (size_t i = 0; i < count; ++i) {
         h ^= p[i];
         h *= 1099511628211ull;
     }
     return h;
 }
+
+// A retired flock whose checksum is still to be computed.
+struct Finish {
+    fh_result res;
+    std::vector<float> words;
+};
 
 }  // namespace
 
@@ -54,6 +63,8 @@ struct fh_server {
     std::condition_variable cv_work, cv_prep, cv_done;
     std::deque<uint64_t> queue;                  // submitted, in arrival order
     std::deque<uint64_t> prep_queue;             // waiting for generation
+    std::deque<Finish> finish_queue;             // retired, checksum pending
+    uint32_t finishing = 0;                      // finish tasks not yet in `done`
     std::unordered_map<uint64_t, Job> jobs;
     std::vector<uint64_t> active;                // in the batch
     std::deque<fh_result> done;
@@ -79,8 +90,21 @@ static double flock_bytes_d(const fh_server *s, uint32_t n) { return (double)fh_
 static void prep_loop(fh_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
-        s->cv_prep.wait(lk, [&] { return s->stop || !s->prep_queue.empty(); });
+        s->cv_prep.wait(lk, [&] { return s->stop || !s->prep_queue.empty() || !s->finish_queue.empty(); });
         if (s->stop) return;
+        if (!s->finish_queue.empty()) {  // results first: they end a request's latency
+            Finish f = std::move(s->finish_queue.front());
+            s->finish_queue.pop_front();
+            lk.unlock();
+            f.res.checksum = fnv1a_words(f.words.data(), f.words.size());
+            lk.lock();
+            f.res.done_s = s->now();
+            if (s->cfg.keep_states) s->kept[f.res.id] = fh_server::Kept{std::move(f.words), f.res.final_time};
+            s->done.push_back(f.res);
+            s->finishing -= 1;
+            s->cv_done.notify_all();
+            continue;
+        }
         const uint64_t id = s->prep_queue.front();
         s->prep_queue.pop_front();
         auto it = s->jobs.find(id);
@@ -179,15 +203,13 @@ static void worker_loop(fh_server *s) {
                 break;
             }
             j.res.final_time = t;
-            j.res.checksum = fnv1a(out.data(), out.size());
-            j.res.done_s = s->now();
             j.res.status = FH_OK;
             s->bytes_active -= flock_bytes_d(s, j.res.birds);
-            if (s->cfg.keep_states) s->kept[j.res.id] = fh_server::Kept{std::move(out), t};
-            s->done.push_back(j.res);
+            s->finish_queue.push_back(Finish{j.res, std::move(out)});
+            s->finishing += 1;
+            s->cv_prep.notify_one();
             s->jobs.erase(s->active[i]);
             s->active.erase(s->active.begin() + (long)i);
-            s->cv_done.notify_all();
         }
     }
     s->cv_done.notify_all();
@@ -297,7 +319,7 @@ fh_status fh_server_drain(fh_server *s, double timeout_s) {
     if (!s) return set_error(FH_ERR_INVALID, "fh_server_drain: NULL server");
     std::unique_lock<std::mutex> lk(s->mu);
     const auto until = Clock::now() + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(timeout_s));
-    while (s->fatal == FH_OK && (!s->queue.empty() || !s->active.empty())) {
+    while (s->fatal == FH_OK && (!s->queue.empty() || !s->active.empty() || s->finishing)) {
         if (s->cv_done.wait_until(lk, until) == std::cv_status::timeout)
             return set_error(FH_ERR_STATE, "fh_server_drain: timed out with %zu queued, %zu active",
                              s->queue.size(), s->active.size());

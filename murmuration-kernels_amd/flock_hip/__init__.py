@@ -25,6 +25,7 @@ FLAG_UNFUSED = 2
 FLAG_GRAPH = 4
 FLAG_WAVE_AGG = 8
 FLAG_NO_TILE = 16
+FLAG_OCC = 32
 
 OK = 0
 ERR_INVALID, ERR_FLK, ERR_NOMEM, ERR_HIP, ERR_BUDGET, ERR_STATE, ERR_IO, ERR_NODEVICE = range(1, 9)
@@ -550,9 +551,9 @@ def server_replay(birds, steps, seeds, arrival_s, **kw) -> tuple[list[dict], dic
 
 
 def state_checksum(words: np.ndarray) -> int:
-    """FNV-1a 64 of .flk records, as fh_result.checksum."""
+    """FNV-1a 64 over the 32-bit words of .flk records, as fh_result.checksum."""
     h = 1469598103934665603
-    for byte in np.ascontiguousarray(words, np.float32).tobytes():
-        h ^= byte
+    for word in np.ascontiguousarray(words, np.float32).view(np.uint32).ravel().tolist():
+        h ^= word
         h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
     return h

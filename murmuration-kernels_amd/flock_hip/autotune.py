@@ -1,12 +1,12 @@
 """Autotuner for the batched and plain steps.
 
-Picks the grid cell edge, the decide workgroup size, the birds each decide
-lane scans (birds_per_thread) and the lanes that share one bird's walk
-(decide_group) by timing short runs of the real workload: a coordinate
-search (group, birds_per_thread, workgroup size, cell edge, bucket count,
-then cell edge and group again),
-each candidate timed over `steps` steps after `warmup` untimed ones. Results
-are cached in a JSON file keyed by the workload's signature.
+Picks the grid cell edge, the bucket count (bucket_shift), the decide
+workgroup size, the birds each decide workgroup scans per lane
+(birds_per_thread), the lanes that share one bird's walk (decide_group) and
+whether decide tests the bucket occupancy bitmap, by timing short runs of
+the real workload. Each candidate is timed over `steps` steps after `warmup`
+untimed ones. Results can be cached in a JSON file keyed by the workload's
+signature.
 
 Changing the cell edge changes the hash keys (the keys match the reference's
 only at 4 m) but not the neighbour sets: the walk always covers the 2r box.
@@ -20,7 +20,7 @@ import time
 
 import numpy as np
 
-from . import FLAG_GRAPH, Batch, Flock
+from . import FLAG_GRAPH, FLAG_OCC, Batch, Flock
 
 BPT = (2, 4, 8, 16)
 BLOCKS = (64, 128, 256, 512)
@@ -42,12 +42,12 @@ def _load(path):
         return {}
 
 
-def _time_batch(starts, prm: dict, flags: int, warmup: int, steps: int) -> float:
+def _time_batch(starts, times, prm: dict, flags: int, warmup: int, steps: int) -> float:
     total = sum(int(w.shape[0]) for w in starts)
     buckets = sum(_buckets(int(w.shape[0])) for w in starts)
     with Batch(total + 256 * len(starts), buckets + 4096 * len(starts), len(starts), flags=flags, **prm) as b:
-        for w in starts:
-            b.add(w)
+        for w, t in zip(starts, times):
+            b.add(w, t)
         b.step(warmup)
         b.step(1)  # sync point after warm-up
         t0 = time.perf_counter()
@@ -55,8 +55,8 @@ def _time_batch(starts, prm: dict, flags: int, warmup: int, steps: int) -> float
         return (time.perf_counter() - t0) / steps
 
 
-def _time_plain(words, prm: dict, flags: int, warmup: int, steps: int) -> float:
-    with Flock.from_words(words, flags=flags, **prm) as f:
+def _time_plain(words, time0: float, prm: dict, flags: int, warmup: int, steps: int) -> float:
+    with Flock.from_words(words, time0, flags=flags, **prm) as f:
         f.step(warmup)
         f.sync()
         t0 = time.perf_counter()
@@ -73,42 +73,75 @@ def _buckets(n: int) -> int:
 
 
 def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, steps: int = 8,
-         cache_path: str | None = None, cells=CELLS, progress=None) -> dict:
-    """Returns {"params": {...}, "ms_per_step": x, "trials": [...]} for the
-    batch of `starts` (kind "batch") or the single flock starts[0] ("plain").
-    `progress`, if given, is called with one line of text per trial."""
+         cache_path: str | None = None, cells=CELLS, progress=None, thorough: bool = True, times=None) -> dict:
+    """Returns {"params": {...}, "flags": f, "ms_per_step": x, "trials": [...]}
+    for the batch of `starts` (kind "batch") or the single flock starts[0]
+    ("plain"). `flags` is the caller's flags plus the switches the search
+    turned on (FLAG_OCC). With `thorough` the launch shape is searched
+    exhaustively (group x birds_per_thread x block) before and after the
+    cell and bucket searches; otherwise one coordinate sweep per tunable.
+    `times` are the flocks' clocks (default 0), e.g. of states downloaded
+    after some steps. `progress`, if given, is called with one line of text
+    per trial."""
+    times = [0.0] * len(starts) if times is None else [float(t) for t in times]
     sizes = [int(w.shape[0]) for w in starts]
-    key = _signature(kind, sizes, steps, flags)
+    key = _signature(kind, sizes, steps, flags) + (":t" if thorough else ":q")
     cache = _load(cache_path) if cache_path else {}
     if key in cache:
         return cache[key]
-    timer = (lambda prm: _time_batch(starts, prm, flags, warmup, steps)) if kind == "batch" else \
-        (lambda prm: _time_plain(starts[0], prm, flags, warmup, steps))
-    best = {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0}
+
+    def timer(prm):
+        prm = dict(prm)
+        f = flags | (FLAG_OCC if prm.pop("occ") else 0)
+        if kind == "batch":
+            return _time_batch(starts, times, prm, f, warmup, steps)
+        return _time_plain(starts[0], times[0], prm, f, warmup, steps)
+
+    best = {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0,
+            "occ": 0}
     trials = []
+    seen = {}
 
     def run(prm):
+        k = tuple(sorted(prm.items()))
+        if k in seen:
+            return seen[k]
         t = timer(prm)
+        seen[k] = t
         trials.append({**prm, "ms": t * 1e3})
         if progress:
             progress(f"{kind} trial {len(trials)}: {prm} {t * 1e3:.4f} ms/step")
         return t
 
     best_t = run(best)
-    seen = {tuple(sorted(best.items()))}
-    for name, values in (("decide_group", GROUPS), ("birds_per_thread", BPT), ("decide_block", BLOCKS),
-                         ("cell", cells), ("bucket_shift", SHIFTS), ("cell", cells), ("decide_group", GROUPS)):
-        for v in values:
-            if v == best[name]:
-                continue
-            cand = {**best, name: v}
-            if tuple(sorted(cand.items())) in seen:
-                continue
-            seen.add(tuple(sorted(cand.items())))
+
+    def sweep(names_values):
+        """Tries every combination of the listed tunables around the best."""
+        nonlocal best, best_t
+        combos = [{}]
+        for name, values in names_values:
+            combos = [{**c, name: v} for c in combos for v in values]
+        for c in combos:
+            cand = {**best, **c}
             t = run(cand)
             if t < best_t:
                 best, best_t = cand, t
-    out = {"params": best, "ms_per_step": best_t * 1e3, "trials": trials, "signature": key}
+
+    shape = [("decide_group", GROUPS), ("birds_per_thread", BPT), ("decide_block", BLOCKS)]
+    if thorough:
+        sweep(shape)
+    else:
+        for nv in shape:
+            sweep([nv])
+    for nv in (("cell", cells), ("bucket_shift", SHIFTS), ("occ", (0, 1)), ("cell", cells)):
+        sweep([nv])
+    if thorough:
+        sweep(shape[:2])
+    else:
+        sweep([shape[0]])
+    params = {k: v for k, v in best.items() if k != "occ"}
+    out = {"params": params, "flags": flags | (FLAG_OCC if best["occ"] else 0), "ms_per_step": best_t * 1e3,
+           "trials": trials, "signature": key}
     if cache_path:
         cache[key] = out
         os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)

@@ -50,7 +50,7 @@ def test_tiled_decide_is_bit_identical_to_the_lane_walk(fh, cell, block, bpt):
     # Dense clumps take the tiled path; it must give the per-lane walk's bits.
     w = fh.gen_clumps(300_000, 4, 4.0, 200.0, 5)
     outs = []
-    for flags in (fh.FLAG_CANONICAL, fh.FLAG_CANONICAL | fh.FLAG_NO_TILE):
+    for flags in (fh.FLAG_CANONICAL | fh.FLAG_OCC, fh.FLAG_CANONICAL | fh.FLAG_NO_TILE):
         with fh.Flock.from_words(w, cell=cell, decide_block=block, birds_per_thread=bpt, flags=flags) as f:
             f.step(3)
             outs.append(f.download()[0])

@@ -143,11 +143,11 @@ def test_neighbour_sets_match_brute_force(fh, fr, n, radius):
                 assert ties_ok(w, int(i), walk, got, want), (i, walk)
 
 
-@pytest.mark.parametrize("canonical", [False, True])
-def test_decide_matches_brute_force(fh, fr, canonical):
+@pytest.mark.parametrize("flags", [0, 1, 32])  # arrival order, canonical, occupancy skip
+def test_decide_matches_brute_force(fh, fr, flags):
     w = fh.gen_reference(4000, 6)
     t = np.float32(0.0) + DT
-    with fh.Flock.from_words(w, flags=fh.FLAG_CANONICAL if canonical else 0) as f:
+    with fh.Flock.from_words(w, flags=flags) as f:
         build_grid(f)
         f.decide(t)
         got, _ = f.download()
@@ -289,6 +289,19 @@ def test_step_is_deterministic_in_canonical_mode(fh):
     for _ in range(2):
         with fh.Flock.from_words(w0, flags=fh.FLAG_CANONICAL) as f:
             f.step(25)
+            outs.append(f.download()[0])
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("n,cell,group,shift", [(200_000, 4.0, 1, 0), (200_000, 6.0, 4, 0), (100_000, 4.0, 8, 3)])
+def test_occupancy_skip_is_bit_identical(fh, n, cell, group, shift):
+    # Skipping empty buckets through the scan's occupancy bitmap must not
+    # change a single bit (a wrong bit would drop a neighbour).
+    w0 = cube(fh, n, 6)
+    outs = []
+    for flags in (fh.FLAG_CANONICAL | fh.FLAG_OCC, fh.FLAG_CANONICAL):
+        with fh.Flock.from_words(w0, flags=flags, cell=cell, decide_group=group, bucket_shift=shift) as f:
+            f.step(12)
             outs.append(f.download()[0])
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 

@@ -41,6 +41,8 @@ def test_each_result_equals_the_flock_alone(fh):
                 want, tw = f.download()
             assert t == tw
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), rid
+            if b < 6000:
+                assert r["checksum"] == fh.state_checksum(want), rid
 
 
 def test_served_request_matches_the_oracle(fh, fr):
