@@ -217,19 +217,28 @@ def main() -> int:
 def reference_leg(args) -> dict:
     """The reference's own protocol (docs/SPEC.md:48-58): Flock::scatter at
     density 0.5 birds/m^3 (32 birds per 4 m cell), +x heading, seed 1, 20
-    warm-up + 20 timed steps, mean GPU ms per pass from events; the rows of
-    docs/results/rtx4000ada-2026-10-21/bench-table-4000ada.txt."""
+    warm-up + 20 timed steps, mean GPU ms per pass from events (histogram as
+    its own pass); the rows of
+    docs/results/rtx4000ada-2026-10-21/bench-table-4000ada.txt. Each size is
+    run at the default launch shape and at the autotuned one (same grid)."""
     import flock_hip as fh
+    from flock_hip import autotune
 
     rows = {}
     for n in (100_000, 1_000_000):
         w = fh.gen_reference(n, 1)
-        with fh.Flock.from_words(w, flags=fh.FLAG_UNFUSED) as f:
-            f.step(20)
-            ms = f.step(20, 0, timing=True)
-        rows[str(n)] = {k: ms[k] for k in ("grid_histogram", "grid_scan", "grid_scatter", "decide", "integrate",
-                                           "step_total")}
-        rows[str(n)]["bird_steps_per_s"] = n / (ms["step_total"] * 1e-3)
+        # the reference grid (4 m cells, buckets_for(n) buckets); launch shape autotuned
+        tuned = autotune.tune([w], "plain", flags=fh.FLAG_UNFUSED, warmup=20, steps=5, cells=(4.0,), shifts=(0,),
+                              thorough=False, progress=log)
+        for name, prm, flags in (("default", {}, fh.FLAG_UNFUSED), ("tuned", tuned["params"], tuned["flags"])):
+            with fh.Flock.from_words(w, flags=flags, **prm) as f:
+                f.step(20)
+                ms = f.step(20, 0, timing=True)
+            row = {k: ms[k] for k in ("grid_histogram", "grid_scan", "grid_scatter", "decide", "integrate",
+                                      "step_total")}
+            row["bird_steps_per_s"] = n / (ms["step_total"] * 1e-3)
+            row["params"] = prm
+            rows[f"{n}_{name}"] = row
     rows["published_rtx4000ada_decide_ms"] = {"100000": 1.24, "1000000": 9.46}
     rows["published_rtx4000ada_step_total_ms"] = {"100000": 1.276, "1000000": 9.979}
     return rows

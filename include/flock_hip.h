@@ -55,9 +55,11 @@ typedef struct fh_params {
     uint32_t decide_block;     /* decide workgroup size: 64, 128, 256 or 512; 0 -> 256 */
     uint32_t birds_per_thread; /* decide scans block * this sorted slots per workgroup; 0 -> 8 */
     uint32_t flags;            /* FH_FLAG_* */
-    uint32_t decide_group;     /* lanes sharing one bird's walk: 1, 2, 4 or 8; 0 -> 1. With 1 the
-                                  sums follow the walk order exactly; with more, each lane sums
-                                  every G-th cell and the partial sums are added */
+    uint32_t decide_group;     /* lanes sharing one bird's walk: 1, 2, 4, 8, 16 or 64; 0 -> 1.
+                                  With 1 the sums follow the walk order exactly; with 2..8 each
+                                  lane sums every G-th cell, with 16 and 64 every lane walks
+                                  every cell and sums every G-th candidate of each bucket (for
+                                  dense cells); the G partial sums are then added */
     uint32_t bucket_shift;     /* when buckets == 0: buckets_for(n) >> bucket_shift (at least 64),
                                   0..8; a smaller table stays in L2 at the price of more hash
                                   collisions. Keys equal the reference's only at shift 0 */

@@ -147,6 +147,69 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
     }
 }
 
+// Candidate-split walk for dense cells: the G lanes of a group walk the same
+// cells in walk order, and lane j takes candidates s0 + j, s0 + j + G, ... of
+// each bucket, so one bucket's candidates are read with coalesced loads and
+// spread over the group. PF cells' bucket ranges are loaded together (every
+// lane of the group loads the same words).
+template <int G, int PF, class Visit>
+__device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, const Pools &p, uint32_t n_slots,
+                                                 uint32_t me, uint32_t lane, bool use_occ, Visit &&visit) {
+    if (!w.ok) return;
+    const int32_t nx = w.hi[0] - w.lo[0] + 1, ny = w.hi[1] - w.lo[1] + 1, nz = w.hi[2] - w.lo[2] + 1;
+    const int32_t ncell = nx * ny * nz;
+    int32_t x = w.lo[0], y = w.lo[1], z = w.lo[2];
+    for (int32_t base = 0; base < ncell; base += PF) {
+        int32_t cx[PF], cy[PF], cz[PF];
+        uint32_t s0[PF], s1[PF], bk[PF];
+        bool want[PF];
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            cx[k] = x;
+            cy[k] = y;
+            cz[k] = z;
+            s0[k] = 0u;
+            s1[k] = 0u;
+            want[k] = base + k < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2);
+            bk[k] = sg.bucket_base + (bucket_hash(x, y, z) & sg.mask);
+            if (++x > w.hi[0]) {
+                x = w.lo[0];
+                if (++y > w.hi[1]) {
+                    y = w.lo[1];
+                    ++z;
+                }
+            }
+        }
+        if (use_occ) {
+            uint32_t word[PF];
+#pragma unroll
+            for (int k = 0; k < PF; ++k) word[k] = want[k] ? p.occ[bk[k] >> 5] : 0u;
+#pragma unroll
+            for (int k = 0; k < PF; ++k) want[k] = (word[k] >> (bk[k] & 31u)) & 1u;
+        }
+#pragma unroll
+        for (int k = 0; k < PF; ++k)
+            if (want[k]) {
+                s0[k] = p.offsets[bk[k]];
+                s1[k] = min(p.offsets[bk[k] + 1], n_slots);
+            }
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            for (uint32_t t = s0[k] + lane; t < s1[k]; t += G) {
+                const float4 c = p.spos[t];
+                if (__float_as_uint(c.w) == me) continue;
+                if (cell_coord(c.x, sg.inv_cell) != cx[k] || cell_coord(c.y, sg.inv_cell) != cy[k] ||
+                    cell_coord(c.z, sg.inv_cell) != cz[k])
+                    continue;
+                const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
+                const float d2 = dot3(d, d);
+                if (d2 > w.c2) continue;
+                visit(t, d, d2);
+            }
+        }
+    }
+}
+
 // The running sums of one bird's walk (model.rs:62-115).
 struct Acc {
     f3 sep, coh, align, csum;
@@ -189,8 +252,11 @@ __device__ __forceinline__ Acc walk_sums(const Pools &p, const Seg &sg, uint32_t
                                          f3 f_i, int32_t first, bool use_occ) {
     Acc a;
     acc_zero(a);
-    walk_cells<G, PF>(w, sg, p, n_slots, me, first, use_occ,
-                      [&](uint32_t t, f3 d, float d2) { accumulate(a, p, f_i, w.r2, t, d, d2); });
+    auto visit = [&](uint32_t t, f3 d, float d2) { accumulate(a, p, f_i, w.r2, t, d, d2); };
+    if constexpr (G <= 8)
+        walk_cells<G, PF>(w, sg, p, n_slots, me, first, use_occ, visit);
+    else
+        walk_cells_split<G, PF>(w, sg, p, n_slots, me, (uint32_t)first, use_occ, visit);
     return a;
 }
 
@@ -242,7 +308,8 @@ constexpr uint32_t NO_BIRD = 0xffffffffu;
 
 // BLOCK lanes per workgroup, G lanes per due bird. With G = 1 each bird's sums
 // follow the walk order exactly (the order of cuda/flock.cu:209-238); with
-// G > 1 lane j sums cells j, j+G, ... and the G partial sums are added.
+// G = 2..8 lane j sums cells j, j+G, ...; with G = 16 or 64 lane j sums
+// candidates j, j+G, ... of every bucket; the G partial sums are then added.
 //
 // Tiled path (G = 1): when the due birds a workgroup holds at once share
 // their neighbourhood (a dense cell: they sit together in slot order), the
@@ -255,7 +322,7 @@ template <int BLOCK, int G>
 __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float time, uint32_t chunk,
                                                   int walk_flags) {
     const bool allow_tiled = (walk_flags & WALK_TILED) != 0, use_occ = (walk_flags & WALK_OCC) != 0;
-    constexpr int PF = G >= 4 ? 4 : 8;
+    constexpr int PF = G >= 16 ? 8 : (G >= 4 ? 4 : 8);
     extern __shared__ uint32_t due_slots[];
     __shared__ uint32_t n_due;
     __shared__ int32_t u_lo[3], u_hi[3];
@@ -419,6 +486,8 @@ static hipError_t launch_decide_group(const Geometry &g, const Pools &p, float t
         case 2: launch_decide_block<BLOCK, 2>(g, p, time, chunk, walk_flags, s); break;
         case 4: launch_decide_block<BLOCK, 4>(g, p, time, chunk, walk_flags, s); break;
         case 8: launch_decide_block<BLOCK, 8>(g, p, time, chunk, walk_flags, s); break;
+        case 16: launch_decide_block<BLOCK, 16>(g, p, time, chunk, walk_flags, s); break;
+        case 64: launch_decide_block<BLOCK, 64>(g, p, time, chunk, walk_flags, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipSuccess;

@@ -77,8 +77,8 @@ fh_status resolve_params(const fh_params *in, fh_params *out, const char *who) {
     if (prm.birds_per_thread > 64 || blk * prm.birds_per_thread > 16384)
         return set_error(FH_ERR_INVALID, "%s: decide chunk %u x %u too large", who, blk, prm.birds_per_thread);
     const uint32_t grp = prm.decide_group;
-    if (grp != 1 && grp != 2 && grp != 4 && grp != 8)
-        return set_error(FH_ERR_INVALID, "%s: decide_group %u not in {1,2,4,8}", who, grp);
+    if (grp != 1 && grp != 2 && grp != 4 && grp != 8 && grp != 16 && grp != 64)
+        return set_error(FH_ERR_INVALID, "%s: decide_group %u not in {1,2,4,8,16,64}", who, grp);
     *out = prm;
     return FH_OK;
 }

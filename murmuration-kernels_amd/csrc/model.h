@@ -92,10 +92,18 @@ FH_HD float radius_step(float r, uint32_t n) {
 // cell_of (cuda/flock.cu:77-79) with Rust `as i32` saturation, NaN -> 0.
 FH_HD int32_t cell_coord(float p, float inv_cell) {
     float f = floorf(p * inv_cell);
+#if defined(__HIP_DEVICE_COMPILE__)
+    // v_cvt_i32_f32 clamps out-of-range values to INT32_MIN / INT32_MAX and
+    // converts NaN to 0: the same saturation, in one instruction.
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(f));
+    return r;
+#else
     if (!(f == f)) return 0;
     if (f >= 2147483647.0f) return 2147483647;
     if (f <= -2147483648.0f) return (int32_t)(-2147483647 - 1);
     return (int32_t)f;
+#endif
 }
 // bucket_of (cuda/flock.cu:80-83): wrapping u32 multiply-xor hash.
 FH_HD uint32_t bucket_hash(int32_t cx, int32_t cy, int32_t cz) {

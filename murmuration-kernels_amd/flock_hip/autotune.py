@@ -22,9 +22,9 @@ import numpy as np
 
 from . import FLAG_GRAPH, FLAG_OCC, Batch, Flock
 
-BPT = (2, 4, 8, 16)
+BPT = (1, 2, 4, 8, 16)
 BLOCKS = (64, 128, 256, 512)
-GROUPS = (1, 2, 4, 8)
+GROUPS = (1, 2, 4, 8, 16, 64)
 CELLS = (4.0, 3.0, 5.0, 6.0, 8.0, 10.0, 12.0)
 SHIFTS = (0, 1, 2, 3, 4)
 
@@ -73,7 +73,8 @@ def _buckets(n: int) -> int:
 
 
 def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, steps: int = 8,
-         cache_path: str | None = None, cells=CELLS, progress=None, thorough: bool = True, times=None) -> dict:
+         cache_path: str | None = None, cells=CELLS, progress=None, thorough: bool = True, times=None,
+         shifts=SHIFTS) -> dict:
     """Returns {"params": {...}, "flags": f, "ms_per_step": x, "trials": [...]}
     for the batch of `starts` (kind "batch") or the single flock starts[0]
     ("plain"). `flags` is the caller's flags plus the switches the search
@@ -97,7 +98,7 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
             return _time_batch(starts, times, prm, f, warmup, steps)
         return _time_plain(starts[0], times[0], prm, f, warmup, steps)
 
-    best = {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0,
+    best = {"cell": float(cells[0]), "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0,
             "occ": 0}
     trials = []
     seen = {}
@@ -133,7 +134,7 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
     else:
         for nv in shape:
             sweep([nv])
-    for nv in (("cell", cells), ("bucket_shift", SHIFTS), ("occ", (0, 1)), ("cell", cells)):
+    for nv in (("cell", cells), ("bucket_shift", shifts), ("occ", (0, 1)), ("cell", cells)):
         sweep([nv])
     if thorough:
         sweep(shape[:2])

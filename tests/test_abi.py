@@ -129,3 +129,13 @@ def test_device_calls_fail_loudly_without_a_gpu(fh):
     with pytest.raises(fh.FlockHipError) as e:
         fh.Flock(100)
     assert e.value.status == fh.ERR_NODEVICE
+
+
+@pytest.mark.parametrize("kw", [dict(decide_group=32), dict(decide_group=3), dict(decide_block=100),
+                                dict(birds_per_thread=65), dict(bucket_shift=9), dict(buckets=1000)])
+def test_bad_params_are_rejected_before_the_device(fh, kw):
+    # Parameter validation runs before any device call, so it reports
+    # FH_ERR_INVALID on any machine.
+    with pytest.raises(fh.FlockHipError) as e:
+        fh.Flock(100, **kw)
+    assert e.value.status == fh.ERR_INVALID, kw

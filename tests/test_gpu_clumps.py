@@ -20,11 +20,12 @@ def rel_err(a, b):
     return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)), initial=0.0))
 
 
-@pytest.mark.parametrize("n,cell,flags", [(1_000_000, 4.0, 0), (1_000_000, 2.0, 8)])
-def test_clumped_decide_matches_the_oracle(fh, fr, n, cell, flags):
+@pytest.mark.parametrize("n,cell,flags,group", [(1_000_000, 4.0, 0, 4), (1_000_000, 2.0, 8, 4),
+                                               (1_000_000, 1.0, 32, 16), (1_000_000, 2.0, 0, 64)])
+def test_clumped_decide_matches_the_oracle(fh, fr, n, cell, flags, group):
     w = fh.gen_clumps(n, 8, 5.0, 1000.0, 3)
     t = np.float32(0.0) + DT
-    with fh.Flock.from_words(w, cell=cell, flags=flags, decide_group=4) as f:
+    with fh.Flock.from_words(w, cell=cell, flags=flags, decide_group=group) as f:
         f.hash_count()
         g = f.read_grid(offsets=False, sorted=False)
         # dense buckets really occur: the peak density of a 125k-bird clump of

@@ -66,6 +66,23 @@ def test_grid_bit_exact(fh, fr, n, kind):
         assert per_bucket_sets_equal(g.sorted, offsets, srt)
 
 
+def test_keys_at_extreme_positions(fh, fr):
+    # cell_of saturates like Rust `as i32` (NaN -> 0) on the device too: NaN,
+    # infinities, values far beyond 2^31 cells and values at the i32 edges.
+    w = fh.gen_reference(2000, 7)
+    vals = np.array([np.nan, np.inf, -np.inf, 1e30, -1e30, 8.6e9, -8.6e9, 8.589934e9, -8.589935e9, -0.0,
+                     3.9999998, 4.0, -1e-30, -4.0], np.float32)
+    rng = np.random.default_rng(3)
+    for i in range(700):
+        w[i, rng.integers(0, 3)] = vals[i % len(vals)]
+    with fh.Flock.from_words(w) as f:
+        f.hash_count()
+        raw = f.read_grid(offsets=False, sorted=False)
+        keys, counts, _, _ = fr.grid(w, f.buckets)
+    assert np.array_equal(raw.keys, keys)
+    assert np.array_equal(raw.counts, counts)
+
+
 @pytest.mark.parametrize("n", [3000, 200_000])
 def test_canonical_order_is_the_stable_sort(fh, fr, n):
     w = fh.gen_reference(n, 2)
@@ -251,7 +268,8 @@ def test_c2_cube_1m_one_step_parity(fh, fr):
 
 @pytest.mark.parametrize("cell,group,block,bpt,shift", [(4.0, 4, 256, 8, 0), (6.0, 2, 128, 4, 1),
                                                         (3.0, 8, 64, 16, 0), (5.0, 1, 512, 2, 2),
-                                                        (8.0, 4, 128, 8, 3), (12.0, 2, 256, 4, 4)])
+                                                        (8.0, 4, 128, 8, 3), (12.0, 2, 256, 4, 4),
+                                                        (4.0, 16, 256, 8, 0), (2.0, 64, 128, 4, 1)])
 def test_c2_one_step_parity_at_tuned_params(fh, fr, cell, group, block, bpt, shift):
     # Tunables change the hash keys (cell) and the summation grouping (group),
     # never the neighbour sets: one step at 1M birds stays within 1e-4 of the
@@ -268,11 +286,11 @@ def test_c2_one_step_parity_at_tuned_params(fh, fr, cell, group, block, bpt, shi
     assert rel_err(gpu[:, 0:6], cpu[:, 0:6]) <= 1e-4
 
 
-@pytest.mark.parametrize("group", [2, 4, 8])
-def test_grouped_decide_matches_brute_force(fh, fr, group):
+@pytest.mark.parametrize("group,block", [(2, 256), (4, 256), (8, 256), (16, 256), (64, 64), (64, 512)])
+def test_grouped_decide_matches_brute_force(fh, fr, group, block):
     w = fh.gen_reference(4000, 6)
     t = np.float32(0.0) + DT
-    with fh.Flock.from_words(w, decide_group=group) as f:
+    with fh.Flock.from_words(w, decide_group=group, decide_block=block) as f:
         build_grid(f)
         f.decide(t)
         got, _ = f.download()
