@@ -104,14 +104,14 @@ static fh_status batch_hash(fh_batch *b) {
 // One batched step: every live flock's clock += DT on the device, then the
 // passes over all flocks (relative due test, fused histogram).
 static fh_status batch_step_launches(fh_batch *b, hipEvent_t *ev) {
-    FH_HIP(launch_set_time(b->d_segs, (uint32_t)b->segs.size(), 0.0f, DT, b->stream));
     if (ev) FH_HIP(hipEventRecord(ev[0], b->stream));
     if (!b->hashed) {
         fh_status st = batch_hash(b);
         if (st) return st;
     }
     if (ev) FH_HIP(hipEventRecord(ev[1], b->stream));
-    FH_HIP(launch_scan(b->geo, b->P, b->stream));
+    // the scan also advances every live flock's device clock by DT
+    FH_HIP(launch_scan(b->geo, b->P, b->d_segs, (uint32_t)b->segs.size(), DT, b->stream));
     b->counts_dirty = false;
     if (ev) FH_HIP(hipEventRecord(ev[2], b->stream));
     FH_HIP(launch_scatter(b->geo, b->P, (b->prm.flags & FH_FLAG_CANONICAL) != 0, true, b->stream));
@@ -235,7 +235,7 @@ fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *o
 }
 
 size_t batch_bytes_per_bird() {
-    // pos4, vel4, steer4, spos, svel (16 B each) + spin, cen, nb, key, rank,
+    // pos4, vel4, steer4 (16 B each), sbird (32 B) + spin, cen, nb, key, rank,
     // snext, sorted_tmp (4 B each)
     return 5 * 16 + 7 * 4;
 }
@@ -273,7 +273,7 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
 
     Arena a;
     size_t o_pos = a.take<float4>(nn), o_vel = a.take<float4>(nn), o_steer = a.take<float4>(nn);
-    size_t o_spos = a.take<float4>(nn), o_svel = a.take<float4>(nn);
+    size_t o_sbird = a.take<float4>(2 * nn);
     size_t o_spin = a.take<float>(nn), o_cen = a.take<float>(nn), o_nb = a.take<uint32_t>(nn);
     size_t o_key = a.take<uint32_t>(nn), o_rank = a.take<uint32_t>(nn), o_snext = a.take<float>(nn);
     size_t o_tmp = a.take<uint32_t>(nn);
@@ -293,8 +293,7 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
     b->P.pos4 = reinterpret_cast<float4 *>(base + o_pos);
     b->P.vel4 = reinterpret_cast<float4 *>(base + o_vel);
     b->P.steer4 = reinterpret_cast<float4 *>(base + o_steer);
-    b->P.spos = reinterpret_cast<float4 *>(base + o_spos);
-    b->P.svel = reinterpret_cast<float4 *>(base + o_svel);
+    b->P.sbird = reinterpret_cast<float4 *>(base + o_sbird);
     b->P.spin = reinterpret_cast<float *>(base + o_spin);
     b->P.cen = reinterpret_cast<float *>(base + o_cen);
     b->P.nb = reinterpret_cast<uint32_t *>(base + o_nb);

@@ -128,12 +128,12 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             s1[k] = min(s1[k], n_slots);
-            head[k] = s0[k] < s1[k] ? p.spos[s0[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            head[k] = s0[k] < s1[k] ? p.sbird[2u * s0[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             for (uint32_t t = s0[k]; t < s1[k]; ++t) {
-                const float4 c = t == s0[k] ? head[k] : p.spos[t];
+                const float4 c = t == s0[k] ? head[k] : p.sbird[2u * t];
                 if (__float_as_uint(c.w) == me) continue;
                 if (cell_coord(c.x, sg.inv_cell) != cx[k] || cell_coord(c.y, sg.inv_cell) != cy[k] ||
                     cell_coord(c.z, sg.inv_cell) != cz[k])
@@ -151,7 +151,9 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
 // cells in walk order, and lane j takes candidates s0 + j, s0 + j + G, ... of
 // each bucket, so one bucket's candidates are read with coalesced loads and
 // spread over the group. PF cells' bucket ranges are loaded together (every
-// lane of the group loads the same words).
+// lane of the group loads the same words); a lane then loads UNROLL of its
+// candidates (position and velocity) before it tests the first, so one
+// memory latency covers UNROLL candidates. `visit(d, d2, velocity)`.
 template <int G, int PF, class Visit>
 __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, const Pools &p, uint32_t n_slots,
                                                  uint32_t me, uint32_t lane, bool use_occ, Visit &&visit) {
@@ -193,18 +195,32 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
                 s0[k] = p.offsets[bk[k]];
                 s1[k] = min(p.offsets[bk[k] + 1], n_slots);
             }
+        constexpr int UNROLL = 4;
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
-            for (uint32_t t = s0[k] + lane; t < s1[k]; t += G) {
-                const float4 c = p.spos[t];
-                if (__float_as_uint(c.w) == me) continue;
-                if (cell_coord(c.x, sg.inv_cell) != cx[k] || cell_coord(c.y, sg.inv_cell) != cy[k] ||
-                    cell_coord(c.z, sg.inv_cell) != cz[k])
-                    continue;
-                const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
-                const float d2 = dot3(d, d);
-                if (d2 > w.c2) continue;
-                visit(t, d, d2);
+            for (uint32_t t0 = s0[k] + lane; t0 < s1[k]; t0 += UNROLL * G) {
+                float4 c[UNROLL], v[UNROLL];
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) {
+                    const uint32_t t = t0 + (uint32_t)u * G;
+                    c[u] = make_float4(0.f, 0.f, 0.f, __uint_as_float(me));
+                    v[u] = c[u];
+                    if (t < s1[k]) {
+                        c[u] = p.sbird[2u * t];
+                        v[u] = p.sbird[2u * t + 1u];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) {
+                    if (__float_as_uint(c[u].w) == me) continue;  // self, or past the bucket's end
+                    if (cell_coord(c[u].x, sg.inv_cell) != cx[k] || cell_coord(c[u].y, sg.inv_cell) != cy[k] ||
+                        cell_coord(c[u].z, sg.inv_cell) != cz[k])
+                        continue;
+                    const f3 d = mk3(c[u].x - w.p.x, c[u].y - w.p.y, c[u].z - w.p.z);
+                    const float d2 = dot3(d, d);
+                    if (d2 > w.c2) continue;
+                    visit(d, d2, v[u]);
+                }
             }
         }
     }
@@ -224,8 +240,10 @@ __device__ __forceinline__ void acc_zero(Acc &a) {
     a.n_all = a.n_star = a.n_align = a.n_g = 0;
 }
 
-// One candidate at slot t with offset d and d^2 <= 4r^2.
-__device__ __forceinline__ void accumulate(Acc &a, const Pools &p, f3 f_i, float r2, uint32_t t, f3 d, float d2) {
+// One candidate with offset d and d^2 <= 4r^2; `velocity()` gives its
+// velocity (read only for aligned neighbours within r).
+template <class Velocity>
+__device__ __forceinline__ void accumulate(Acc &a, f3 f_i, float r2, f3 d, float d2, Velocity &&velocity) {
     const float dist = sqrtf(d2);
     const f3 dhat = d / fmaxf(dist, 1e-6f);
     // walk 2 (model.rs:104-115): csum += d / max(sqrt(d2), 1e-6)
@@ -241,7 +259,7 @@ __device__ __forceinline__ void accumulate(Acc &a, const Pools &p, f3 f_i, float
         ++a.n_star;
     }
     if (dot3(f_i, dhat) >= BLIND_COS) {
-        const float4 vj = p.svel[t];
+        const float4 vj = velocity();
         a.align = a.align + forward_of(mk3(vj.x, vj.y, vj.z));
         ++a.n_align;
     }
@@ -252,11 +270,14 @@ __device__ __forceinline__ Acc walk_sums(const Pools &p, const Seg &sg, uint32_t
                                          f3 f_i, int32_t first, bool use_occ) {
     Acc a;
     acc_zero(a);
-    auto visit = [&](uint32_t t, f3 d, float d2) { accumulate(a, p, f_i, w.r2, t, d, d2); };
     if constexpr (G <= 8)
-        walk_cells<G, PF>(w, sg, p, n_slots, me, first, use_occ, visit);
+        walk_cells<G, PF>(w, sg, p, n_slots, me, first, use_occ, [&](uint32_t t, f3 d, float d2) {
+            accumulate(a, f_i, w.r2, d, d2, [&] { return p.sbird[2u * t + 1u]; });
+        });
     else
-        walk_cells_split<G, PF>(w, sg, p, n_slots, me, (uint32_t)first, use_occ, visit);
+        walk_cells_split<G, PF>(w, sg, p, n_slots, me, (uint32_t)first, use_occ, [&](f3 d, float d2, float4 vj) {
+            accumulate(a, f_i, w.r2, d, d2, [&] { return vj; });
+        });
     return a;
 }
 
@@ -351,14 +372,14 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
         w.ok = false;
         if (active) {
             const uint32_t s = due_slots[k];
-            const float4 q = p.spos[s];
+            const float4 q = p.sbird[2u * s];
             me = __float_as_uint(q.w);
             if (me >= g.n_birds) {
                 if (lane_g == 0) atomicOr(p.err, ERR_SLOT);
                 active = false;
             } else {
                 si = g.tile_seg ? g.tile_seg[me / TILE] : 0u;
-                const float4 v = p.svel[s];  // w: the bird's radius
+                const float4 v = p.sbird[2u * s + 1u];  // w: the bird's radius
                 pos = mk3(q.x, q.y, q.z);
                 vel = mk3(v.x, v.y, v.z);
                 f_i = forward_of(vel);
@@ -415,7 +436,7 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
                         for (uint32_t t0 = s0; t0 < s1; t0 += BLOCK) {
                             const uint32_t t = t0 + threadIdx.x;
                             if (t < s1) {
-                                const float4 c = p.spos[t];
+                                const float4 c = p.sbird[2u * t];
                                 const bool in_cell = cell_coord(c.x, sg.inv_cell) == cx &&
                                                      cell_coord(c.y, sg.inv_cell) == cy &&
                                                      cell_coord(c.z, sg.inv_cell) == cz;
@@ -431,7 +452,7 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
                                     const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
                                     const float d2 = dot3(d, d);
                                     if (d2 > w.c2) continue;
-                                    accumulate(a, p, f_i, w.r2, t0 + j, d, d2);
+                                    accumulate(a, f_i, w.r2, d, d2, [&] { return p.sbird[2u * (t0 + j) + 1u]; });
                                 }
                             }
                             __syncthreads();
@@ -464,7 +485,7 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
     uint32_t c = 0;
     walk_cells<1, 8>(w, sg, p, g.n_slots, bird, 0, true, [&](uint32_t t, f3, float d2) {
         if (walk == 1 && d2 > w.r2) return;
-        if (c < cap) ids[c] = __float_as_uint(p.spos[t].w);
+        if (c < cap) ids[c] = __float_as_uint(p.sbird[2u * t].w);
         ++c;
     });
     *count = c;

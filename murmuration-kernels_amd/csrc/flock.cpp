@@ -64,8 +64,8 @@ static fh_status do_hash(fh_flock *h) {
     return FH_OK;
 }
 
-static fh_status do_scan(fh_flock *h) {
-    FH_HIP(launch_scan(h->geo, h->P, h->stream));
+static fh_status do_scan(fh_flock *h, float tick = 0.0f) {
+    FH_HIP(launch_scan(h->geo, h->P, h->d_seg, tick != 0.0f ? 1u : 0u, tick, h->stream));
     h->counts_dirty = false;
     h->stage = Stage::Scanned;
     return FH_OK;
@@ -100,14 +100,13 @@ static fh_status do_integrate(fh_flock *h, bool hash) {
 // One step after time += DT, in the relative-due form (scatter stores
 // next - time, decide tests <= 0) so the same launches replay in a graph.
 static fh_status step_launches(fh_flock *h, bool fused, hipEvent_t *ev) {
-    FH_HIP(launch_set_time(h->d_seg, 1, 0.0f, DT, h->stream));
     if (ev) FH_HIP(hipEventRecord(ev[0], h->stream));
     if (!fused || h->stage != Stage::Hashed) {
         fh_status st = do_hash(h);
         if (st) return st;
     }
     if (ev) FH_HIP(hipEventRecord(ev[1], h->stream));
-    fh_status st = do_scan(h);
+    fh_status st = do_scan(h, DT);  // the scan also advances the device clock by DT
     if (st) return st;
     if (ev) FH_HIP(hipEventRecord(ev[2], h->stream));
     if ((st = do_scatter(h, true))) return st;
@@ -173,7 +172,7 @@ fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
     Arena a;
     const size_t nn = n ? n : 1;
     size_t o_pos = a.take<float4>(nn), o_vel = a.take<float4>(nn), o_steer = a.take<float4>(nn);
-    size_t o_spos = a.take<float4>(nn), o_svel = a.take<float4>(nn);
+    size_t o_sbird = a.take<float4>(2 * nn);
     size_t o_spin = a.take<float>(nn), o_cen = a.take<float>(nn), o_nb = a.take<uint32_t>(nn);
     size_t o_key = a.take<uint32_t>(nn), o_rank = a.take<uint32_t>(nn), o_snext = a.take<float>(nn);
     size_t o_tmp = a.take<uint32_t>(nn);
@@ -192,8 +191,7 @@ fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
     h->P.pos4 = reinterpret_cast<float4 *>(base + o_pos);
     h->P.vel4 = reinterpret_cast<float4 *>(base + o_vel);
     h->P.steer4 = reinterpret_cast<float4 *>(base + o_steer);
-    h->P.spos = reinterpret_cast<float4 *>(base + o_spos);
-    h->P.svel = reinterpret_cast<float4 *>(base + o_svel);
+    h->P.sbird = reinterpret_cast<float4 *>(base + o_sbird);
     h->P.spin = reinterpret_cast<float *>(base + o_spin);
     h->P.cen = reinterpret_cast<float *>(base + o_cen);
     h->P.nb = reinterpret_cast<uint32_t *>(base + o_nb);
@@ -510,9 +508,9 @@ fh_status fh_read_grid(fh_flock *h, uint32_t *keys, uint32_t *counts, uint32_t *
     if (sorted) {
         if (h->stage != Stage::Scattered && h->stage != Stage::Decided)
             return set_error(FH_ERR_STATE, "fh_read_grid: sorted needs fh_scatter first");
-        std::vector<float4> sp(n);
-        if (n) FH_HIP(hipMemcpy(sp.data(), h->P.spos, (size_t)n * sizeof(float4), hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < n; ++i) std::memcpy(&sorted[i], &sp[i].w, 4);
+        std::vector<float4> sp((size_t)n * 2);
+        if (n) FH_HIP(hipMemcpy(sp.data(), h->P.sbird, (size_t)n * 2 * sizeof(float4), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) std::memcpy(&sorted[i], &sp[(size_t)i * 2].w, 4);
     }
     return read_error_word(h);
 }

@@ -99,9 +99,15 @@ __global__ __launch_bounds__(TILE) void k_hash_count(Geometry g, Pools p) {
     if (active) p.rank[bird] = r;
 }
 
-// Scan, level 1: the sum of each SCAN_TILE-bucket tile.
+// Scan, level 1: the sum of each SCAN_TILE-bucket tile. With nseg > 0,
+// workgroup 0 also advances the clock of every live segment by dt (the
+// step's time += DT, which the scatter after this scan reads).
 __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t *__restrict__ counts,
-                                                     uint32_t *__restrict__ tile_sums) {
+                                                     uint32_t *__restrict__ tile_sums, Seg *segs, uint32_t nseg,
+                                                     float dt) {
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < nseg; i += 256u)
+            if (segs[i].live) segs[i].time = segs[i].time + dt;
     const uint4 *c4 = reinterpret_cast<const uint4 *>(counts + (size_t)blockIdx.x * SCAN_TILE);
     uint32_t sum = 0;
 #pragma unroll
@@ -148,11 +154,30 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *__restrict_
 
 // Scan, level 3: offsets of one tile = its prefix + local exclusive scan.
 // The counts are zeroed for the next histogram as they are consumed, and the
-// occupancy bitmap records which buckets hold a bird.
+// occupancy bitmap records which buckets hold a bird. With MERGED (at most
+// SCAN_MERGE_TILES tiles) there is no level 2: each workgroup adds up the
+// sums of the tiles before it itself (tile_in = tile sums; the last one
+// writes the grand total and workgroup 0 zeroes the step counters);
+// otherwise tile_in holds the prefixes of level 2.
+constexpr uint32_t SCAN_MERGE_TILES = 2048;
+template <bool MERGED>
 __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *__restrict__ counts,
-                                                    const uint32_t *__restrict__ tile_prefix,
+                                                    const uint32_t *__restrict__ tile_in, uint32_t ntiles,
                                                     uint32_t *__restrict__ offsets,
-                                                    uint32_t *__restrict__ occ) {
+                                                    uint32_t *__restrict__ occ, uint32_t *__restrict__ due_count,
+                                                    uint32_t *__restrict__ big_count) {
+    uint32_t prefix;
+    if (MERGED) {
+        uint32_t part = 0;
+        for (uint32_t t = threadIdx.x; t < blockIdx.x; t += 256u) part += tile_in[t];
+        block_excl_scan<4>(part, &prefix);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {  // a new grid starts a new step
+            *due_count = 0;
+            *big_count = 0;
+        }
+    } else {
+        prefix = tile_in[blockIdx.x];
+    }
     const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * 16u;
     uint4 *c4 = reinterpret_cast<uint4 *>(counts + base);
     uint32_t c[16];
@@ -171,7 +196,8 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *__restrict__ count
     const uint32_t hi = __shfl_xor(bits, 1);
     if ((threadIdx.x & 1u) == 0u) occ[base / 32u] = bits | (hi << 16);
     uint32_t total;
-    uint32_t run = tile_prefix[blockIdx.x] + block_excl_scan<4>(sum, &total);
+    uint32_t run = prefix + block_excl_scan<4>(sum, &total);
+    if (MERGED && blockIdx.x == ntiles - 1u && threadIdx.x == 0) offsets[(size_t)ntiles * SCAN_TILE] = prefix + total;
     uint32_t o[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -206,8 +232,8 @@ __global__ __launch_bounds__(TILE) void k_scatter(Geometry g, Pools p, int canon
     }
     const float4 q = p.pos4[bird];
     const float4 v = p.vel4[bird];
-    p.spos[s] = make_float4(q.x, q.y, q.z, __uint_as_float(bird));
-    p.svel[s] = make_float4(v.x, v.y, v.z, p.steer4[bird].w);
+    p.sbird[2u * s] = make_float4(q.x, q.y, q.z, __uint_as_float(bird));
+    p.sbird[2u * s + 1u] = make_float4(v.x, v.y, v.z, p.steer4[bird].w);
     p.snext[s] = relative ? q.w - sg.time : q.w;
 }
 
@@ -218,8 +244,8 @@ __device__ __forceinline__ void write_sorted(const Geometry &g, const Pools &p, 
     const float4 v = p.vel4[bird];
     float t = 0.0f;
     if (relative) t = g.segs[g.tile_seg ? g.tile_seg[bird / TILE] : 0u].time;
-    p.spos[c] = make_float4(q.x, q.y, q.z, __uint_as_float(bird));
-    p.svel[c] = make_float4(v.x, v.y, v.z, p.steer4[bird].w);
+    p.sbird[2u * c] = make_float4(q.x, q.y, q.z, __uint_as_float(bird));
+    p.sbird[2u * c + 1u] = make_float4(v.x, v.y, v.z, p.steer4[bird].w);
     p.snext[c] = relative ? q.w - t : q.w;
 }
 
@@ -343,12 +369,23 @@ hipError_t launch_hash_count(const Geometry &g, const Pools &p, bool wave_agg, h
     return hipGetLastError();
 }
 
-hipError_t launch_scan(const Geometry &g, const Pools &p, hipStream_t s) {
+hipError_t launch_scan(const Geometry &g, const Pools &p, Seg *tick_segs, uint32_t tick_nseg, float dt,
+                       hipStream_t s) {
     const uint32_t ntiles = (g.n_buckets + SCAN_TILE - 1) / SCAN_TILE;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_sums);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, p.tile_sums, p.tile_prefix, ntiles,
-                       p.offsets + (size_t)ntiles * SCAN_TILE, p.due_count, p.big_count);
-    hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_prefix, p.offsets, p.occ);
+    if (ntiles == 0) {
+        if (tick_nseg) return launch_set_time(tick_segs, tick_nseg, 0.0f, dt, s);
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_sums, tick_segs, tick_nseg, dt);
+    if (ntiles <= SCAN_MERGE_TILES) {
+        hipLaunchKernelGGL(k_scan_apply<true>, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_sums, ntiles,
+                           p.offsets, p.occ, p.due_count, p.big_count);
+    } else {
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, p.tile_sums, p.tile_prefix, ntiles,
+                           p.offsets + (size_t)ntiles * SCAN_TILE, p.due_count, p.big_count);
+        hipLaunchKernelGGL(k_scan_apply<false>, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_prefix, ntiles,
+                           p.offsets, p.occ, p.due_count, p.big_count);
+    }
     return hipGetLastError();
 }
 
@@ -377,7 +414,7 @@ hipError_t grid_kernel_attributes(const char *name, hipFuncAttributes *out) {
     if (!__builtin_strcmp(name, "hash_count"))
         return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_hash_count<false>));
     if (!__builtin_strcmp(name, "scan"))
-        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_scan_apply));
+        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_scan_apply<true>));
     if (!__builtin_strcmp(name, "scatter"))
         return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_scatter));
     if (!__builtin_strcmp(name, "canonical"))

@@ -13,8 +13,10 @@
 //   counts[bucket]   histogram, zeroed again by the scan
 //   offsets[bucket]  exclusive scan, buckets + 1 entries
 //   occ[bucket / 32] bit (bucket % 32) set when the bucket holds a bird (scan)
-// and the scatter writes the sorted copies decide walks, in slot order:
-//   spos[slot] = (px, py, pz, bird index bits), svel[slot] = (vx, vy, vz, radius),
+// and the scatter writes the sorted copies decide walks, in slot order, as one
+// 32-byte record per slot (a candidate's velocity shares its position's line):
+//   sbird[2 slot]     = (px, py, pz, bird index bits)
+//   sbird[2 slot + 1] = (vx, vy, vz, radius)
 //   snext[slot] = next_decision (or next_decision - time, see Seg::time).
 //
 // Several flocks can share one set of pools ("segments"): flock f owns birds
@@ -62,7 +64,7 @@ struct Pools {
     uint32_t *key, *rank;
     uint32_t *counts, *offsets, *tile_sums, *tile_prefix;
     uint32_t *occ;         // occupancy bitmap of the buckets (one bit each)
-    float4 *spos, *svel;
+    float4 *sbird;         // sorted records, two float4 per slot (see above)
     float *snext;
     uint32_t *sorted_tmp;  // canonical mode: slot -> bird before reordering
     uint32_t *err;         // device error word (ERR_* bits)
@@ -86,7 +88,11 @@ struct Geometry {
 // Launchers (stream-ordered, no host synchronisation). Each returns the
 // hipError_t of its launch.
 hipError_t launch_hash_count(const Geometry &g, const Pools &p, bool wave_agg, hipStream_t s);
-hipError_t launch_scan(const Geometry &g, const Pools &p, hipStream_t s);
+// Exclusive scan of the counts into offsets (and the occupancy bitmap). With
+// tick_nseg > 0 it also advances the clocks of the live segments among
+// tick_segs[0, tick_nseg) by dt before any later launch reads them.
+hipError_t launch_scan(const Geometry &g, const Pools &p, Seg *tick_segs, uint32_t tick_nseg, float dt,
+                       hipStream_t s);
 hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, bool relative_time,
                           hipStream_t s);
 // Walk switches of decide (bit set = enabled).
