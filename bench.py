@@ -84,6 +84,9 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-reference", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="plain leg at the default tunables")
+    ap.add_argument("--params", default="",
+                    help="plain leg at these tunables instead of autotuning, e.g. "
+                         "'cell=4,decide_block=512,birds_per_thread=4,decide_group=4,bucket_shift=0,flags=4'")
     ap.add_argument("--no-clumps", action="store_true")
     ap.add_argument("--clump-birds", type=int, default=10_000_000)
     ap.add_argument("--clump-steps", type=int, default=20)
@@ -122,7 +125,14 @@ def main() -> int:
 
     # Tunables of the plain step, picked by the autotuner on this workload.
     prm, tuned, flags = {}, None, args.flags
-    if not args.no_tune:
+    if args.params:
+        for item in args.params.split(","):
+            k, v = item.split("=")
+            if k == "flags":
+                flags = int(v)
+            else:
+                prm[k] = float(v) if k == "cell" else int(v)
+    elif not args.no_tune:
         from flock_hip import autotune
 
         # trials time the same window as the timed run (radii grow each decision)
@@ -142,7 +152,7 @@ def main() -> int:
             return (time.perf_counter() - t0) * 1000.0 / args.steps, f.buckets
 
     ms_per_step, buckets = timed_run(prm, flags)
-    default_ms = timed_run({}, args.flags)[0] if prm else ms_per_step
+    default_ms = timed_run({}, args.flags)[0] if tuned else ms_per_step
     log(f"plain leg: {ms_per_step:.4f} ms/step tuned, {default_ms:.4f} ms/step at the default tunables")
 
     # Per-pass breakdown with hipEvents on the library's stream (same start).

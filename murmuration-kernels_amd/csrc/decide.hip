@@ -121,8 +121,9 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
 #pragma unroll
         for (int k = 0; k < PF; ++k)
             if (want[k]) {
-                s0[k] = p.offsets[bk[k]];
-                s1[k] = p.offsets[bk[k] + 1];
+                const uint2 r = bucket_range(p.offsets, bk[k]);
+                s0[k] = r.x;
+                s1[k] = r.y;
             }
         float4 head[PF];
 #pragma unroll
@@ -192,8 +193,9 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
 #pragma unroll
         for (int k = 0; k < PF; ++k)
             if (want[k]) {
-                s0[k] = p.offsets[bk[k]];
-                s1[k] = min(p.offsets[bk[k] + 1], n_slots);
+                const uint2 r = bucket_range(p.offsets, bk[k]);
+                s0[k] = r.x;
+                s1[k] = min(r.y, n_slots);
             }
         constexpr int UNROLL = 4;
 #pragma unroll
@@ -431,8 +433,9 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
                         if (!__syncthreads_or(need)) continue;
                         const uint32_t b = sg.bucket_base + (bucket_hash(cx, cy, cz) & sg.mask);
                         if (use_occ && !bucket_occupied(p, b)) continue;  // same b for every lane
-                        const uint32_t s0 = p.offsets[b];
-                        const uint32_t s1 = min(p.offsets[b + 1], g.n_slots);
+                        const uint2 r = bucket_range(p.offsets, b);
+                        const uint32_t s0 = r.x;
+                        const uint32_t s1 = min(r.y, g.n_slots);
                         for (uint32_t t0 = s0; t0 < s1; t0 += BLOCK) {
                             const uint32_t t = t0 + threadIdx.x;
                             if (t < s1) {

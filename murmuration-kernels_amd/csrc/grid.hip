@@ -263,7 +263,8 @@ __global__ __launch_bounds__(256) void k_canonical(Geometry g, Pools p, int rela
         return;
     }
     const uint32_t k = p.key[bird];
-    const uint32_t o0 = p.offsets[k], o1 = min(p.offsets[k + 1], g.n_slots);
+    const uint2 r = bucket_range(p.offsets, k);
+    const uint32_t o0 = r.x, o1 = min(r.y, g.n_slots);
     if (!(o0 <= s && s < o1)) {  // inconsistent grid: keep the slot
         atomicOr(p.err, ERR_SLOT);
         write_sorted(g, p, s, bird, relative);
@@ -294,7 +295,8 @@ __global__ __launch_bounds__(BIG_BLOCK) void k_canonical_big(Geometry g, Pools p
     const uint32_t nbig = min(*p.big_count, p.big_cap);
     for (uint32_t i = blockIdx.x; i < nbig; i += gridDim.x) {
         const uint32_t k = p.big[i];
-        const uint32_t o0 = p.offsets[k], o1 = min(p.offsets[k + 1], g.n_slots);
+        const uint2 r = bucket_range(p.offsets, k);
+        const uint32_t o0 = r.x, o1 = min(r.y, g.n_slots);
         const uint32_t m = o1 > o0 ? o1 - o0 : 0u;
         if (m > CANON_BIG_CAP) {
             if (threadIdx.x == 0) atomicOr(p.err, ERR_CANON);

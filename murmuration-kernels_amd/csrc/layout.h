@@ -74,6 +74,16 @@ struct Pools {
     uint32_t big_cap;      // capacity of `big`: big_list_cap(bird capacity)
 };
 
+// [offsets[b], offsets[b + 1]) with one 8-byte load: the two words share a
+// cache line (except when b is the last word of one), so one request fetches
+// both instead of two requests for the same line. gfx950 loads dword-aligned
+// 8-byte values natively.
+__device__ __forceinline__ uint2 bucket_range(const uint32_t *offsets, uint32_t b) {
+    uint2 r;
+    __builtin_memcpy(&r, offsets + b, sizeof r);
+    return r;
+}
+
 // Launch geometry of one build/step over the pools.
 struct Geometry {
     const Seg *segs;           // device segment table
