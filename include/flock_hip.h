@@ -86,6 +86,14 @@ typedef struct fh_params {
  * empty and the offsets table is far larger than the cache); results are
  * bit-identical. */
 #define FH_FLAG_OCC 32u
+/* fh_step keeps the flock's state in bucket order between steps: only the
+ * birds that change bucket are counted with atomics, and one near-sequential
+ * pass moves every record to its new slot (instead of hashing every bird and
+ * scattering sorted copies each step). Keys, counts and offsets stay the
+ * reference's; within a bucket, birds that stay keep their order and birds
+ * that arrive follow them. Single flocks only; not with FH_FLAG_CANONICAL.
+ * Any other call on the flock first puts the state back in bird order. */
+#define FH_FLAG_RESIDENT 64u
 
 /* Per-pass GPU time of fh_step, mean ms per timed step (hipEvents on the
  * handle's stream), in the rows of the reference report

@@ -257,6 +257,8 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
     fh_params prm;
     fh_status st = resolve_params(params, &prm, "fh_batch_create");
     if (st) return st;
+    if (prm.flags & FH_FLAG_RESIDENT)
+        return set_error(FH_ERR_INVALID, "fh_batch_create: FH_FLAG_RESIDENT is for single flocks (fh_step)");
     st = require_device();
     if (st) return st;
 

@@ -1,7 +1,10 @@
 // decide for gfx950: each workgroup takes a run of sorted slots, compacts the
 // birds that are due into LDS, and gives every lane one due bird. Birds are
 // visited in slot order, so lanes of a wave that share a cell walk the same
-// buckets.
+// buckets. The candidates come either from the sorted copies the scatter
+// writes (32-byte records: position, then velocity) or, in the resident
+// layout, from the state itself kept in slot order (positions and velocities
+// in their own arrays); a candidate is known by its slot.
 //
 // Mirrors crates/flock-ref/src/model.rs:53-145 over the grid of
 // cuda/flock.cu:190-290. The reference walks twice, once over the box of
@@ -21,6 +24,12 @@ struct Decision {
     float radius;
     uint32_t n_all;
     float centrality;
+};
+
+// Where a walk reads candidate t: position pos[STRIDE t], velocity
+// vel[STRIDE t] (STRIDE 2: interleaved sorted records; 1: resident arrays).
+struct Cand {
+    const float4 *pos, *vel;
 };
 
 // Per-bird walk. `visit(slot)` sees each candidate slot that passes the
@@ -70,9 +79,9 @@ __device__ __forceinline__ bool bucket_occupied(const Pools &p, uint32_t b) {
 // bits of the PF buckets are read first (a small, cache-resident table) and
 // empty buckets never touch the offsets table. Within a lane the visit order
 // is the walk order.
-template <int G, int PF, class Visit>
-__device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const Pools &p, uint32_t n_slots,
-                                           uint32_t me, int32_t first, bool use_occ, Visit &&visit) {
+template <int G, int PF, int STRIDE, class Visit>
+__device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const Pools &p, Cand cand, uint32_t n_slots,
+                                           uint32_t self, int32_t first, bool use_occ, Visit &&visit) {
     if (!w.ok) return;
     const int32_t nx = w.hi[0] - w.lo[0] + 1, ny = w.hi[1] - w.lo[1] + 1, nz = w.hi[2] - w.lo[2] + 1;
     const int32_t ncell = nx * ny * nz;
@@ -129,13 +138,13 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             s1[k] = min(s1[k], n_slots);
-            head[k] = s0[k] < s1[k] ? p.sbird[2u * s0[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            head[k] = s0[k] < s1[k] ? cand.pos[(size_t)STRIDE * s0[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             for (uint32_t t = s0[k]; t < s1[k]; ++t) {
-                const float4 c = t == s0[k] ? head[k] : p.sbird[2u * t];
-                if (__float_as_uint(c.w) == me) continue;
+                const float4 c = t == s0[k] ? head[k] : cand.pos[(size_t)STRIDE * t];
+                if (t == self) continue;
                 if (cell_coord(c.x, sg.inv_cell) != cx[k] || cell_coord(c.y, sg.inv_cell) != cy[k] ||
                     cell_coord(c.z, sg.inv_cell) != cz[k])
                     continue;
@@ -155,9 +164,10 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
 // lane of the group loads the same words); a lane then loads UNROLL of its
 // candidates (position and velocity) before it tests the first, so one
 // memory latency covers UNROLL candidates. `visit(d, d2, velocity)`.
-template <int G, int PF, class Visit>
-__device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, const Pools &p, uint32_t n_slots,
-                                                 uint32_t me, uint32_t lane, bool use_occ, Visit &&visit) {
+template <int G, int PF, int STRIDE, class Visit>
+__device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, const Pools &p, Cand cand,
+                                                 uint32_t n_slots, uint32_t self, uint32_t lane, bool use_occ,
+                                                 Visit &&visit) {
     if (!w.ok) return;
     const int32_t nx = w.hi[0] - w.lo[0] + 1, ny = w.hi[1] - w.lo[1] + 1, nz = w.hi[2] - w.lo[2] + 1;
     const int32_t ncell = nx * ny * nz;
@@ -205,16 +215,17 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) {
                     const uint32_t t = t0 + (uint32_t)u * G;
-                    c[u] = make_float4(0.f, 0.f, 0.f, __uint_as_float(me));
+                    c[u] = make_float4(0.f, 0.f, 0.f, 0.f);
                     v[u] = c[u];
                     if (t < s1[k]) {
-                        c[u] = p.sbird[2u * t];
-                        v[u] = p.sbird[2u * t + 1u];
+                        c[u] = cand.pos[(size_t)STRIDE * t];
+                        v[u] = cand.vel[(size_t)STRIDE * t];
                     }
                 }
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) {
-                    if (__float_as_uint(c[u].w) == me) continue;  // self, or past the bucket's end
+                    const uint32_t t = t0 + (uint32_t)u * G;
+                    if (t >= s1[k] || t == self) continue;  // past the bucket's end, or the bird itself
                     if (cell_coord(c[u].x, sg.inv_cell) != cx[k] || cell_coord(c[u].y, sg.inv_cell) != cy[k] ||
                         cell_coord(c[u].z, sg.inv_cell) != cz[k])
                         continue;
@@ -267,19 +278,20 @@ __device__ __forceinline__ void accumulate(Acc &a, f3 f_i, float r2, f3 d, float
     }
 }
 
-template <int G, int PF>
-__device__ __forceinline__ Acc walk_sums(const Pools &p, const Seg &sg, uint32_t n_slots, uint32_t me, const Walk &w,
-                                         f3 f_i, int32_t first, bool use_occ) {
+template <int G, int PF, int STRIDE>
+__device__ __forceinline__ Acc walk_sums(const Pools &p, Cand cand, const Seg &sg, uint32_t n_slots, uint32_t self,
+                                         const Walk &w, f3 f_i, int32_t first, bool use_occ) {
     Acc a;
     acc_zero(a);
     if constexpr (G <= 8)
-        walk_cells<G, PF>(w, sg, p, n_slots, me, first, use_occ, [&](uint32_t t, f3 d, float d2) {
-            accumulate(a, f_i, w.r2, d, d2, [&] { return p.sbird[2u * t + 1u]; });
+        walk_cells<G, PF, STRIDE>(w, sg, p, cand, n_slots, self, first, use_occ, [&](uint32_t t, f3 d, float d2) {
+            accumulate(a, f_i, w.r2, d, d2, [&] { return cand.vel[(size_t)STRIDE * t]; });
         });
     else
-        walk_cells_split<G, PF>(w, sg, p, n_slots, me, (uint32_t)first, use_occ, [&](f3 d, float d2, float4 vj) {
-            accumulate(a, f_i, w.r2, d, d2, [&] { return vj; });
-        });
+        walk_cells_split<G, PF, STRIDE>(w, sg, p, cand, n_slots, self, (uint32_t)first, use_occ,
+                                        [&](f3 d, float d2, float4 vj) {
+                                            accumulate(a, f_i, w.r2, d, d2, [&] { return vj; });
+                                        });
     return a;
 }
 
@@ -341,22 +353,29 @@ constexpr uint32_t NO_BIRD = 0xffffffffu;
 // coalesced loads and every lane that needs the cell reads them from LDS
 // (broadcast). A lane sees the same (cell, slot) sequence as in its own walk,
 // so its sums are bit-identical to the per-lane path.
-template <int BLOCK, int G>
-__global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float time, uint32_t chunk,
-                                                  int walk_flags) {
+template <int BLOCK, int G, bool RES>
+__global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, Records rec, float time, uint32_t chunk,
+                                                  int walk_flags, uint32_t *due_clear) {
     const bool allow_tiled = (walk_flags & WALK_TILED) != 0, use_occ = (walk_flags & WALK_OCC) != 0;
     constexpr int PF = G >= 16 ? 8 : (G >= 4 ? 4 : 8);
+    constexpr int STRIDE = RES ? 1 : 2;
+    const Cand cand = RES ? Cand{rec.p4, rec.v4} : Cand{p.sbird, p.sbird + 1};
     extern __shared__ uint32_t due_slots[];
     __shared__ uint32_t n_due;
     __shared__ int32_t u_lo[3], u_hi[3];
     __shared__ uint32_t u_seg_lo, u_seg_hi, u_vol, u_any;
     __shared__ float4 tile[BLOCK];
     if (threadIdx.x == 0) n_due = 0;
+    if (RES && due_clear && blockIdx.x == 0 && threadIdx.x == 0) *due_clear = 0;
     __syncthreads();
     const uint32_t start = blockIdx.x * chunk;
     const uint32_t end = min(start + chunk, g.n_slots);
+    // Sorted copies: time is relative (0) and snext holds next - time. Resident
+    // records: this step's clock is the segment clock + DT (committed by the
+    // integrate pass that follows) and p4.w holds next_decision.
+    const float tnow = RES ? g.segs[0].time + DT : time;
     for (uint32_t s = start + threadIdx.x; s < end; s += BLOCK)
-        if (time >= p.snext[s]) due_slots[atomicAdd(&n_due, 1u)] = s;
+        if (tnow >= (RES ? rec.p4[s].w : p.snext[s])) due_slots[atomicAdd(&n_due, 1u)] = s;
     __syncthreads();
     const uint32_t nd = n_due;
     if (threadIdx.x == 0 && nd) atomicAdd(p.due_count, nd);
@@ -365,7 +384,7 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
     for (uint32_t base = 0; base < nd; base += groups) {  // same trip count for every lane
         const uint32_t k = base + threadIdx.x / G;
         bool active = k < nd;
-        uint32_t me = 0, si = 0;
+        uint32_t me = 0, si = 0, slot = 0;
         f3 pos = mk3(0.f, 0.f, 0.f), vel = pos, f_i = pos;
         float r = 0.0f;
         Acc a;
@@ -373,19 +392,19 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
         Walk w;
         w.ok = false;
         if (active) {
-            const uint32_t s = due_slots[k];
-            const float4 q = p.sbird[2u * s];
-            me = __float_as_uint(q.w);
+            slot = due_slots[k];
+            const float4 q = cand.pos[(size_t)STRIDE * slot];
+            const float4 v = cand.vel[(size_t)STRIDE * slot];  // sorted copies: w is the bird's radius
+            me = RES ? rec.id[slot] : __float_as_uint(q.w);
             if (me >= g.n_birds) {
                 if (lane_g == 0) atomicOr(p.err, ERR_SLOT);
                 active = false;
             } else {
                 si = g.tile_seg ? g.tile_seg[me / TILE] : 0u;
-                const float4 v = p.sbird[2u * s + 1u];  // w: the bird's radius
                 pos = mk3(q.x, q.y, q.z);
                 vel = mk3(v.x, v.y, v.z);
                 f_i = forward_of(vel);
-                r = v.w;
+                r = RES ? rec.s4[slot].w : v.w;
                 w = make_walk(pos, r, g.segs[si]);
             }
         }
@@ -433,51 +452,57 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, float tim
                         if (!__syncthreads_or(need)) continue;
                         const uint32_t b = sg.bucket_base + (bucket_hash(cx, cy, cz) & sg.mask);
                         if (use_occ && !bucket_occupied(p, b)) continue;  // same b for every lane
-                        const uint2 r = bucket_range(p.offsets, b);
-                        const uint32_t s0 = r.x;
-                        const uint32_t s1 = min(r.y, g.n_slots);
+                        const uint2 rg = bucket_range(p.offsets, b);
+                        const uint32_t s0 = rg.x;
+                        const uint32_t s1 = min(rg.y, g.n_slots);
                         for (uint32_t t0 = s0; t0 < s1; t0 += BLOCK) {
                             const uint32_t t = t0 + threadIdx.x;
                             if (t < s1) {
-                                const float4 c = p.sbird[2u * t];
+                                const float4 c = cand.pos[(size_t)STRIDE * t];
                                 const bool in_cell = cell_coord(c.x, sg.inv_cell) == cx &&
                                                      cell_coord(c.y, sg.inv_cell) == cy &&
                                                      cell_coord(c.z, sg.inv_cell) == cz;
-                                tile[threadIdx.x] = make_float4(c.x, c.y, c.z, in_cell ? c.w : __uint_as_float(NO_BIRD));
+                                tile[threadIdx.x] = make_float4(c.x, c.y, c.z, __uint_as_float(in_cell ? t : NO_BIRD));
                             }
                             __syncthreads();
                             const uint32_t cnt = min((uint32_t)BLOCK, s1 - t0);
                             if (need) {
                                 for (uint32_t j = 0; j < cnt; ++j) {
                                     const float4 c = tile[j];
-                                    const uint32_t id = __float_as_uint(c.w);
-                                    if (id == NO_BIRD || id == me) continue;
+                                    const uint32_t ts = __float_as_uint(c.w);
+                                    if (ts == NO_BIRD || ts == slot) continue;
                                     const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
                                     const float d2 = dot3(d, d);
                                     if (d2 > w.c2) continue;
-                                    accumulate(a, f_i, w.r2, d, d2, [&] { return p.sbird[2u * (t0 + j) + 1u]; });
+                                    accumulate(a, f_i, w.r2, d, d2, [&] { return cand.vel[(size_t)STRIDE * ts]; });
                                 }
                             }
                             __syncthreads();
                         }
                     }
         } else if (active) {
-            a = walk_sums<G, PF>(p, g.segs[si], g.n_slots, me, w, f_i, (int32_t)lane_g, use_occ);
+            a = walk_sums<G, PF, STRIDE>(p, cand, g.segs[si], g.n_slots, slot, w, f_i, (int32_t)lane_g, use_occ);
         }
         if (G == 1) __syncthreads();  // the union box is reset by the next iteration
         if (G > 1) reduce_group<G>(a);
         if (active && lane_g == 0) {
             const Decision d = finish(a, pos, vel, f_i, r);
-            p.steer4[me] = make_float4(d.force.x, d.force.y, d.force.z, d.radius);
-            p.pos4[me].w = p.pos4[me].w + DU;
+            if (RES) {
+                rec.s4[slot] = make_float4(d.force.x, d.force.y, d.force.z, d.radius);
+                rec.p4[slot].w = rec.p4[slot].w + DU;
+            } else {
+                p.steer4[me] = make_float4(d.force.x, d.force.y, d.force.z, d.radius);
+                p.pos4[me].w = p.pos4[me].w + DU;
+            }
             p.nb[me] = d.n_all;
             p.cen[me] = d.centrality;
         }
     }
 }
 
-// Debug walk for one bird: the ids the grid walk finds within r (walk 1) or
-// 2r (walk 2), in walk order. One lane; used by the parity tests only.
+// Debug walk for one bird (sorted copies): the ids the grid walk finds within
+// r (walk 1) or 2r (walk 2), in walk order. One lane; used by the parity
+// tests only. The bird's own slot is looked up in its bucket first.
 __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint32_t *ids, uint32_t cap,
                              uint32_t *count) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -485,52 +510,71 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
     const float4 q = p.pos4[bird];
     const float r = p.steer4[bird].w;
     const Walk w = make_walk(mk3(q.x, q.y, q.z), r, sg);
+    const uint2 own = bucket_range(p.offsets, p.key[bird]);
+    uint32_t self = 0xffffffffu;
+    for (uint32_t t = own.x; t < min(own.y, g.n_slots); ++t)
+        if (__float_as_uint(p.sbird[2u * t].w) == bird) self = t;
     uint32_t c = 0;
-    walk_cells<1, 8>(w, sg, p, g.n_slots, bird, 0, true, [&](uint32_t t, f3, float d2) {
-        if (walk == 1 && d2 > w.r2) return;
-        if (c < cap) ids[c] = __float_as_uint(p.sbird[2u * t].w);
-        ++c;
-    });
+    walk_cells<1, 8, 2>(w, sg, p, Cand{p.sbird, p.sbird + 1}, g.n_slots, self, 0, true,
+                        [&](uint32_t t, f3, float d2) {
+                            if (walk == 1 && d2 > w.r2) return;
+                            if (c < cap) ids[c] = __float_as_uint(p.sbird[2u * t].w);
+                            ++c;
+                        });
     *count = c;
 }
 
-template <int BLOCK, int G>
-static void launch_decide_block(const Geometry &g, const Pools &p, float time, uint32_t chunk, int walk_flags,
-                                hipStream_t s) {
+template <int BLOCK, int G, bool RES>
+static void launch_decide_block(const Geometry &g, const Pools &p, const Records &rec, float time, uint32_t chunk,
+                                int walk_flags, uint32_t *due_clear, hipStream_t s) {
     const uint32_t blocks = (g.n_slots + chunk - 1) / chunk;
-    hipLaunchKernelGGL((k_decide<BLOCK, G>), dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s, g, p, time,
-                       chunk, walk_flags);
+    hipLaunchKernelGGL((k_decide<BLOCK, G, RES>), dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s, g, p, rec,
+                       time, chunk, walk_flags, due_clear);
 }
 
-template <int BLOCK>
-static hipError_t launch_decide_group(const Geometry &g, const Pools &p, float time, uint32_t chunk, uint32_t group,
-                                      int walk_flags, hipStream_t s) {
+template <int BLOCK, bool RES>
+static hipError_t launch_decide_group(const Geometry &g, const Pools &p, const Records &rec, float time,
+                                      uint32_t chunk, uint32_t group, int walk_flags, uint32_t *due_clear,
+                                      hipStream_t s) {
     switch (group) {
-        case 1: launch_decide_block<BLOCK, 1>(g, p, time, chunk, walk_flags, s); break;
-        case 2: launch_decide_block<BLOCK, 2>(g, p, time, chunk, walk_flags, s); break;
-        case 4: launch_decide_block<BLOCK, 4>(g, p, time, chunk, walk_flags, s); break;
-        case 8: launch_decide_block<BLOCK, 8>(g, p, time, chunk, walk_flags, s); break;
-        case 16: launch_decide_block<BLOCK, 16>(g, p, time, chunk, walk_flags, s); break;
-        case 64: launch_decide_block<BLOCK, 64>(g, p, time, chunk, walk_flags, s); break;
+        case 1: launch_decide_block<BLOCK, 1, RES>(g, p, rec, time, chunk, walk_flags, due_clear, s); break;
+        case 2: launch_decide_block<BLOCK, 2, RES>(g, p, rec, time, chunk, walk_flags, due_clear, s); break;
+        case 4: launch_decide_block<BLOCK, 4, RES>(g, p, rec, time, chunk, walk_flags, due_clear, s); break;
+        case 8: launch_decide_block<BLOCK, 8, RES>(g, p, rec, time, chunk, walk_flags, due_clear, s); break;
+        case 16: launch_decide_block<BLOCK, 16, RES>(g, p, rec, time, chunk, walk_flags, due_clear, s); break;
+        case 64: launch_decide_block<BLOCK, 64, RES>(g, p, rec, time, chunk, walk_flags, due_clear, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipSuccess;
 }
 
-hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
-                         uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s) {
+template <bool RES>
+static hipError_t launch_decide_any(const Geometry &g, const Pools &p, const Records &rec, float time,
+                                    uint32_t block, uint32_t birds_per_thread, uint32_t group, int walk_flags,
+                                    uint32_t *due_clear, hipStream_t s) {
     if (g.n_slots == 0) return hipSuccess;
     const uint32_t chunk = block * birds_per_thread;
     hipError_t e;
     switch (block) {
-        case 64: e = launch_decide_group<64>(g, p, time, chunk, group, walk_flags, s); break;
-        case 128: e = launch_decide_group<128>(g, p, time, chunk, group, walk_flags, s); break;
-        case 256: e = launch_decide_group<256>(g, p, time, chunk, group, walk_flags, s); break;
-        case 512: e = launch_decide_group<512>(g, p, time, chunk, group, walk_flags, s); break;
+        case 64: e = launch_decide_group<64, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, s); break;
+        case 128: e = launch_decide_group<128, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, s); break;
+        case 256: e = launch_decide_group<256, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, s); break;
+        case 512: e = launch_decide_group<512, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, s); break;
         default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
     return hipGetLastError();
+}
+
+hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
+                         uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s) {
+    return launch_decide_any<false>(g, p, Records{}, time, block, birds_per_thread, group, walk_flags, nullptr, s);
+}
+
+hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Records &rec, uint32_t block,
+                                  uint32_t birds_per_thread, uint32_t group, int walk_flags, uint32_t *due_clear,
+                                  hipStream_t s) {
+    return launch_decide_any<true>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear, s);
 }
 
 hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, int walk, uint32_t *ids,
@@ -541,7 +585,7 @@ hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, i
 
 hipError_t decide_kernel_attributes(const char *name, hipFuncAttributes *out) {
     if (!__builtin_strcmp(name, "decide"))
-        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_decide<256, 1>));
+        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_decide<256, 1, false>));
     return hipErrorInvalidValue;
 }
 

@@ -38,6 +38,16 @@ struct fh_flock {
     uint32_t *d_nbr = nullptr;  // neighbour read-back buffer
     uint32_t nbr_cap = 0;
     hipGraphExec_t graph = nullptr;
+    // Resident layout (FH_FLAG_RESIDENT): records R[q] and offsets off[q] of
+    // the current layout, q = parity; a step moves them to the other buffers.
+    void *res_arena = nullptr;
+    Records R[2]{};
+    ResWork W{};
+    uint32_t *off[2]{};
+    uint32_t *res_due = nullptr;  // due counter of each parity
+    bool resident = false;
+    uint32_t parity = 0;
+    hipGraphExec_t rgraph[2]{};
 };
 
 static fh_status check_handle(const fh_flock *h, const char *fn) {
@@ -149,6 +159,125 @@ static fh_status capture_graph(fh_flock *h) {
     return FH_OK;
 }
 
+// ---- resident layout ------------------------------------------------------
+
+static fh_status alloc_resident(fh_flock *h) {
+    if (h->res_arena) return FH_OK;
+    const size_t nn = h->n ? h->n : 1;
+    const size_t nb = h->padded_buckets;
+    Arena a;
+    size_t o_p4[2], o_v4[2], o_s4[2], o_spin[2], o_id[2], o_key[2];
+    for (int q = 0; q < 2; ++q) {
+        o_p4[q] = a.take<float4>(nn);
+        o_v4[q] = a.take<float4>(nn);
+        o_s4[q] = a.take<float4>(nn);
+        o_spin[q] = a.take<float>(nn);
+        o_id[q] = a.take<uint32_t>(nn);
+        o_key[q] = a.take<uint32_t>(nn);
+    }
+    const size_t o_knew = a.take<uint32_t>(nn), o_arank = a.take<uint32_t>(nn);
+    const size_t o_mbits = a.take<uint32_t>((nn + TILE - 1) / TILE * (TILE / 32));
+    const size_t o_arr = a.take<uint32_t>(nb), o_dep = a.take<uint32_t>(nb), o_abase = a.take<uint32_t>(nb);
+    const size_t o_off1 = a.take<uint32_t>(nb + 1), o_due = a.take<uint32_t>(2);
+    hipError_t e = hipMalloc(&h->res_arena, a.used);
+    if (e != hipSuccess) {
+        h->res_arena = nullptr;
+        return hip_fail(e, "hipMalloc(resident arena)");
+    }
+    char *base = static_cast<char *>(h->res_arena);
+    for (int q = 0; q < 2; ++q) {
+        h->R[q].p4 = reinterpret_cast<float4 *>(base + o_p4[q]);
+        h->R[q].v4 = reinterpret_cast<float4 *>(base + o_v4[q]);
+        h->R[q].s4 = reinterpret_cast<float4 *>(base + o_s4[q]);
+        h->R[q].spin = reinterpret_cast<float *>(base + o_spin[q]);
+        h->R[q].id = reinterpret_cast<uint32_t *>(base + o_id[q]);
+        h->R[q].key = reinterpret_cast<uint32_t *>(base + o_key[q]);
+    }
+    h->W.knew = reinterpret_cast<uint32_t *>(base + o_knew);
+    h->W.arank = reinterpret_cast<uint32_t *>(base + o_arank);
+    h->W.mbits = reinterpret_cast<uint32_t *>(base + o_mbits);
+    h->W.arr = reinterpret_cast<uint32_t *>(base + o_arr);
+    h->W.dep = reinterpret_cast<uint32_t *>(base + o_dep);
+    h->W.abase = reinterpret_cast<uint32_t *>(base + o_abase);
+    h->off[0] = h->P.offsets;
+    h->off[1] = reinterpret_cast<uint32_t *>(base + o_off1);
+    h->res_due = reinterpret_cast<uint32_t *>(base + o_due);
+    // arrivals and departures start at zero and every rebuild zeroes them again
+    FH_HIP(hipMemsetAsync(h->res_arena, 0, a.used, h->stream));
+    FH_HIP(hipStreamSynchronize(h->stream));
+    return FH_OK;
+}
+
+// Bird-order state -> resident records: the reference grid of the current
+// positions (hash, scan with the counts kept), then every bird to its slot.
+static fh_status enter_resident(fh_flock *h) {
+    fh_status st = alloc_resident(h);
+    if (st) return st;
+    if (h->counts_dirty) FH_HIP(launch_fill(h->P.counts, h->padded_buckets, 0u, h->stream));
+    FH_HIP(launch_hash_count(h->geo, h->P, (h->prm.flags & FH_FLAG_WAVE_AGG) != 0, h->stream));
+    Pools p = h->P;
+    p.offsets = h->off[0];
+    FH_HIP(launch_scan(h->geo, p, nullptr, 0u, 0.0f, h->stream, true));
+    FH_HIP(launch_resident_enter(h->geo, p, h->R[0], h->stream));
+    h->parity = 0;
+    h->resident = true;
+    h->counts_dirty = false;  // the counts now describe the current layout
+    h->stage = Stage::None;
+    return FH_OK;
+}
+
+// Resident records -> bird-order state (before any call that reads it).
+static fh_status leave_resident(fh_flock *h) {
+    if (!h->resident) return FH_OK;
+    FH_HIP(launch_resident_flush(h->geo, h->P, h->R[h->parity], h->stream));
+    h->resident = false;
+    h->counts_dirty = true;  // the counts hold the last layout, not a fresh histogram
+    h->stage = Stage::None;
+    return FH_OK;
+}
+
+// One resident step: decide over the current layout, banked flight (movers
+// counted), then the grid of the new positions and the records moved into it.
+// Events (per-pass timing): ev[0] | decide | ev[1] | integrate | ev[2] | scan
+// and move | ev[3] = ev[4] = ev[5].
+static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
+    const uint32_t q = h->parity;
+    Pools p = h->P;
+    p.offsets = h->off[q];
+    p.due_count = h->res_due + q;
+    if (ev) FH_HIP(hipEventRecord(ev[0], h->stream));
+    FH_HIP(launch_decide_resident(h->geo, p, h->R[q], h->prm.decide_block, h->prm.birds_per_thread,
+                                  h->prm.decide_group, walk_flags(h->prm.flags), h->res_due + (1u - q), h->stream));
+    if (ev) FH_HIP(hipEventRecord(ev[1], h->stream));
+    FH_HIP(launch_resident_integrate(h->geo, p, h->R[q], h->W, h->d_seg, h->stream));
+    if (ev) FH_HIP(hipEventRecord(ev[2], h->stream));
+    FH_HIP(launch_resident_rebuild(h->geo, p, h->R[q], h->R[1u - q], h->W, h->off[q], h->off[1u - q], h->stream));
+    if (ev)
+        for (int k = 3; k < 6; ++k) FH_HIP(hipEventRecord(ev[k], h->stream));
+    return FH_OK;
+}
+
+static fh_status capture_resident_graph(fh_flock *h) {
+    const uint32_t q = h->parity;
+    if (h->rgraph[q]) return FH_OK;
+    hipGraph_t g = nullptr;
+    FH_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    fh_status st = resident_step_launches(h, nullptr);
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&h->rgraph[q], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        h->rgraph[q] = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    return FH_OK;
+}
+
 extern "C" {
 
 fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
@@ -168,6 +297,10 @@ fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
     h->buckets = flock_buckets(n, prm);
     const uint32_t ntiles = (h->buckets + SCAN_TILE - 1) / SCAN_TILE;
     h->padded_buckets = ntiles * SCAN_TILE;
+    if ((prm.flags & FH_FLAG_RESIDENT) && ntiles > 2048u) {
+        delete h;
+        return set_error(FH_ERR_INVALID, "fh_create: FH_FLAG_RESIDENT supports at most 2048 * 4096 buckets");
+    }
 
     Arena a;
     const size_t nn = n ? n : 1;
@@ -248,6 +381,9 @@ fh_status fh_destroy(fh_flock *h) {
     if (!h) return FH_OK;
     (void)hipStreamSynchronize(h->stream);
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
+    for (auto &gx : h->rgraph)
+        if (gx) (void)hipGraphExecDestroy(gx);
+    if (h->res_arena) (void)hipFree(h->res_arena);
     if (h->d_nbr) (void)hipFree(h->d_nbr);
     (void)hipFree(h->arena);
     (void)hipStreamDestroy(h->stream);
@@ -294,6 +430,10 @@ fh_status fh_upload_words(fh_flock *h, const float *words, uint32_t n, float tim
     st = sync_clock(h);
     if (st) return st;
     FH_HIP(hipStreamSynchronize(h->stream));
+    if (h->resident) {  // the uploaded state replaces the resident records
+        h->resident = false;
+        h->counts_dirty = true;
+    }
     h->stage = Stage::None;
     return FH_OK;
 }
@@ -303,6 +443,7 @@ fh_status fh_download_words(fh_flock *h, float *words, float *time) {
     fh_status st = check_handle(h, "fh_download_words");
     if (st) return st;
     if (h->n && !words) return set_error(FH_ERR_INVALID, "fh_download_words: NULL words");
+    if ((st = leave_resident(h))) return st;
     const uint32_t n = h->n;
     std::vector<float4> pos(n), vel(n), steer(n);
     std::vector<float> spin(n), cen(n);
@@ -335,12 +476,14 @@ fh_status fh_download_words(fh_flock *h, float *words, float *time) {
 fh_status fh_hash_count(fh_flock *h) {
     clear_error();
     fh_status st = check_handle(h, "fh_hash_count");
+    if (!st) st = leave_resident(h);
     return st ? st : do_hash(h);
 }
 
 fh_status fh_scan(fh_flock *h) {
     clear_error();
     fh_status st = check_handle(h, "fh_scan");
+    if (!st) st = leave_resident(h);
     if (st) return st;
     if (h->stage != Stage::Hashed) return set_error(FH_ERR_STATE, "fh_scan: no histogram of the current positions");
     return do_scan(h);
@@ -349,6 +492,7 @@ fh_status fh_scan(fh_flock *h) {
 fh_status fh_scatter(fh_flock *h) {
     clear_error();
     fh_status st = check_handle(h, "fh_scatter");
+    if (!st) st = leave_resident(h);
     if (st) return st;
     if (h->stage != Stage::Scanned) return set_error(FH_ERR_STATE, "fh_scatter: scan the histogram first");
     return do_scatter(h, false);
@@ -357,6 +501,7 @@ fh_status fh_scatter(fh_flock *h) {
 fh_status fh_decide(fh_flock *h, float time) {
     clear_error();
     fh_status st = check_handle(h, "fh_decide");
+    if (!st) st = leave_resident(h);
     if (st) return st;
     if (h->stage != Stage::Scattered || !h->absolute_due)
         return set_error(FH_ERR_STATE, "fh_decide: build the grid (hash, scan, scatter) first");
@@ -368,12 +513,14 @@ fh_status fh_decide(fh_flock *h, float time) {
 fh_status fh_integrate(fh_flock *h) {
     clear_error();
     fh_status st = check_handle(h, "fh_integrate");
+    if (!st) st = leave_resident(h);
     return st ? st : do_integrate(h, false);
 }
 
 fh_status fh_integrate_hash(fh_flock *h) {
     clear_error();
     fh_status st = check_handle(h, "fh_integrate_hash");
+    if (!st) st = leave_resident(h);
     return st ? st : do_integrate(h, true);
 }
 
@@ -402,7 +549,9 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
     if (warmup > steps) warmup = steps;  // cuda/flock.cu:389
     const bool fused = !(h->prm.flags & FH_FLAG_UNFUSED);
     const bool graph = fused && (h->prm.flags & FH_FLAG_GRAPH);
+    const bool res = fused && (h->prm.flags & FH_FLAG_RESIDENT);
     if ((st = sync_clock(h))) return st;
+    if (res && !h->resident && (st = enter_resident(h))) return st;
     const uint32_t timed = steps - warmup;
     std::vector<hipEvent_t> ev;
     const bool per_pass = out && !graph;
@@ -421,7 +570,20 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
             if (out && graph) FH_HIP(hipEventRecord(ev[0], h->stream));
         }
         h->time += DT;  // host f32 clock (cuda/flock.cu:455); the device adds DT the same way
-        if (graph) {
+        if (res) {
+            if (graph) {
+                if ((st = capture_resident_graph(h))) break;
+                hipError_t e = hipGraphLaunch(h->rgraph[h->parity], h->stream);
+                if (e != hipSuccess) {
+                    st = hip_fail(e, "hipGraphLaunch");
+                    break;
+                }
+            } else {
+                hipEvent_t *e6 = (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 6] : nullptr;
+                if ((st = resident_step_launches(h, e6))) break;
+            }
+            h->parity ^= 1u;
+        } else if (graph) {
             if (h->stage != Stage::Hashed) {
                 if ((st = do_hash(h))) break;
             }
@@ -460,11 +622,19 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
                         FH_HIP(hipEventElapsedTime(&ms, ev[(size_t)s * 6 + k], ev[(size_t)s * 6 + k + 1]));
                         acc[k] += ms;
                     }
-                out->grid_histogram = acc[0] / timed;
-                out->grid_scan = acc[1] / timed;
-                out->grid_scatter = acc[2] / timed;
-                out->decide = acc[3] / timed;
-                out->integrate = acc[4] / timed;
+                if (res) {  // decide, integrate, then the rebuild (scan and move, reported as scatter)
+                    out->grid_histogram = 0.0;
+                    out->grid_scan = 0.0;
+                    out->grid_scatter = acc[2] / timed;
+                    out->decide = acc[0] / timed;
+                    out->integrate = acc[1] / timed;
+                } else {
+                    out->grid_histogram = acc[0] / timed;
+                    out->grid_scan = acc[1] / timed;
+                    out->grid_scatter = acc[2] / timed;
+                    out->decide = acc[3] / timed;
+                    out->integrate = acc[4] / timed;
+                }
                 out->step_total = (acc[0] + acc[1] + acc[2] + acc[3] + acc[4]) / timed;
             } else {
                 float ms = 0.0f;
@@ -476,7 +646,7 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
     destroy_events();
     if (out) {
         uint32_t due = 0;
-        FH_HIP(hipMemcpy(&due, h->P.due_count, 4, hipMemcpyDeviceToHost));
+        FH_HIP(hipMemcpy(&due, res ? h->res_due + (h->parity ^ 1u) : h->P.due_count, 4, hipMemcpyDeviceToHost));
         out->due_last = due;
     }
     return read_error_word(h);
@@ -486,9 +656,29 @@ fh_status fh_read_grid(fh_flock *h, uint32_t *keys, uint32_t *counts, uint32_t *
     clear_error();
     fh_status st = check_handle(h, "fh_read_grid");
     if (st) return st;
-    if (h->stage == Stage::None) return set_error(FH_ERR_STATE, "fh_read_grid: no grid built");
     const uint32_t n = h->n, B = h->buckets;
     FH_HIP(hipStreamSynchronize(h->stream));
+    if (h->resident) {
+        // The resident layout is the grid of the current positions: keys by
+        // bird, offsets, counts and the bird of each slot, read in place.
+        const Records &r = h->R[h->parity];
+        std::vector<uint32_t> id(n), key(n), off((size_t)B + 1);
+        if (n) {
+            FH_HIP(hipMemcpy(id.data(), r.id, (size_t)n * 4, hipMemcpyDeviceToHost));
+            FH_HIP(hipMemcpy(key.data(), r.key, (size_t)n * 4, hipMemcpyDeviceToHost));
+        }
+        FH_HIP(hipMemcpy(off.data(), h->off[h->parity], ((size_t)B + 1) * 4, hipMemcpyDeviceToHost));
+        for (uint32_t s2 = 0; s2 < n; ++s2) {
+            if (id[s2] >= n) return set_error(FH_ERR_HIP, "fh_read_grid: slot %u holds bird %u", s2, id[s2]);
+            if (keys) keys[id[s2]] = key[s2];
+            if (sorted) sorted[s2] = id[s2];
+        }
+        if (offsets) std::memcpy(offsets, off.data(), ((size_t)B + 1) * 4);
+        if (counts)
+            for (uint32_t b = 0; b < B; ++b) counts[b] = off[b + 1] - off[b];
+        return read_error_word(h);
+    }
+    if (h->stage == Stage::None) return set_error(FH_ERR_STATE, "fh_read_grid: no grid built");
     if (keys && n) FH_HIP(hipMemcpy(keys, h->P.key, (size_t)n * 4, hipMemcpyDeviceToHost));
     const bool have_offsets = h->stage != Stage::Hashed;
     std::vector<uint32_t> off;
@@ -518,6 +708,7 @@ fh_status fh_read_grid(fh_flock *h, uint32_t *keys, uint32_t *counts, uint32_t *
 fh_status fh_neighbours(fh_flock *h, uint32_t bird, int walk, uint32_t *ids, uint32_t cap, uint32_t *count) {
     clear_error();
     fh_status st = check_handle(h, "fh_neighbours");
+    if (!st) st = leave_resident(h);
     if (st) return st;
     if (bird >= h->n || (walk != 1 && walk != 2) || !count || (cap && !ids))
         return set_error(FH_ERR_INVALID, "fh_neighbours: bad bird/walk/buffer");

@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *__restrict__ count
                                                     const uint32_t *__restrict__ tile_in, uint32_t ntiles,
                                                     uint32_t *__restrict__ offsets,
                                                     uint32_t *__restrict__ occ, uint32_t *__restrict__ due_count,
-                                                    uint32_t *__restrict__ big_count) {
+                                                    uint32_t *__restrict__ big_count, int keep_counts) {
     uint32_t prefix;
     if (MERGED) {
         uint32_t part = 0;
@@ -207,9 +207,11 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *__restrict__ count
     uint4 *o4 = reinterpret_cast<uint4 *>(offsets + base);  // base is a multiple of 16
 #pragma unroll
     for (int k = 0; k < 4; ++k) o4[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
-    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    if (!keep_counts) {
+        const uint4 z = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) c4[k] = z;
+        for (int k = 0; k < 4; ++k) c4[k] = z;
+    }
 }
 
 // grid_scatter: slot = offsets[key] + rank. Writes the sorted copies directly,
@@ -372,7 +374,8 @@ hipError_t launch_hash_count(const Geometry &g, const Pools &p, bool wave_agg, h
 }
 
 hipError_t launch_scan(const Geometry &g, const Pools &p, Seg *tick_segs, uint32_t tick_nseg, float dt,
-                       hipStream_t s) {
+                       hipStream_t s, bool keep_counts) {
+    const int keep = keep_counts ? 1 : 0;
     const uint32_t ntiles = (g.n_buckets + SCAN_TILE - 1) / SCAN_TILE;
     if (ntiles == 0) {
         if (tick_nseg) return launch_set_time(tick_segs, tick_nseg, 0.0f, dt, s);
@@ -381,12 +384,12 @@ hipError_t launch_scan(const Geometry &g, const Pools &p, Seg *tick_segs, uint32
     hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_sums, tick_segs, tick_nseg, dt);
     if (ntiles <= SCAN_MERGE_TILES) {
         hipLaunchKernelGGL(k_scan_apply<true>, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_sums, ntiles,
-                           p.offsets, p.occ, p.due_count, p.big_count);
+                           p.offsets, p.occ, p.due_count, p.big_count, keep);
     } else {
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, p.tile_sums, p.tile_prefix, ntiles,
                            p.offsets + (size_t)ntiles * SCAN_TILE, p.due_count, p.big_count);
         hipLaunchKernelGGL(k_scan_apply<false>, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_prefix, ntiles,
-                           p.offsets, p.occ, p.due_count, p.big_count);
+                           p.offsets, p.occ, p.due_count, p.big_count, keep);
     }
     return hipGetLastError();
 }
@@ -403,6 +406,190 @@ hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, boo
         const uint32_t blocks = min(big_list_cap(g.n_slots), 256u);
         hipLaunchKernelGGL(k_canonical_big, dim3(blocks), dim3(BIG_BLOCK), 0, s, g, p, relative_time ? 1 : 0);
     }
+    return hipGetLastError();
+}
+
+// ---- resident layout (one flock) ------------------------------------------
+
+// Records from the bird-order state: slot = offsets[key] + rank, the same slot
+// the scatter gives the bird.
+__global__ __launch_bounds__(TILE) void k_r_enter(Geometry g, Pools p, Records rec) {
+    const uint32_t b = blockIdx.x * TILE + threadIdx.x;
+    if (b >= g.n_birds) return;
+    const uint32_t k = p.key[b];
+    const uint32_t s = p.offsets[k] + p.rank[b];
+    if (s >= g.n_slots) {
+        atomicOr(p.err, ERR_SLOT);
+        return;
+    }
+    rec.p4[s] = p.pos4[b];
+    rec.v4[s] = p.vel4[b];
+    rec.s4[s] = p.steer4[b];
+    rec.spin[s] = p.spin[b];
+    rec.id[s] = b;
+    rec.key[s] = k;
+}
+
+// Records back to the bird-order state.
+__global__ __launch_bounds__(TILE) void k_r_flush(Geometry g, Pools p, Records rec) {
+    const uint32_t s = blockIdx.x * TILE + threadIdx.x;
+    if (s >= g.n_slots) return;
+    const uint32_t b = rec.id[s];
+    if (b >= g.n_birds) {
+        atomicOr(p.err, ERR_SLOT);
+        return;
+    }
+    p.pos4[b] = rec.p4[s];
+    p.vel4[b] = rec.v4[s];
+    p.steer4[b] = rec.s4[s];
+    p.spin[b] = rec.spin[s];
+    p.key[b] = rec.key[s];
+}
+
+// Rebuild, level 1: counts += arrivals - departures (kept), tile sums.
+__global__ __launch_bounds__(256) void k_r_scan_reduce(uint32_t *__restrict__ counts,
+                                                       const uint32_t *__restrict__ arr,
+                                                       const uint32_t *__restrict__ dep,
+                                                       uint32_t *__restrict__ tile_sums) {
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE;
+    uint4 *c4 = reinterpret_cast<uint4 *>(counts + base);
+    const uint4 *a4 = reinterpret_cast<const uint4 *>(arr + base);
+    const uint4 *d4 = reinterpret_cast<const uint4 *>(dep + base);
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t i = (uint32_t)k * 256u + threadIdx.x;
+        uint4 c = c4[i];
+        const uint4 a = a4[i], d = d4[i];
+        c = make_uint4(c.x - d.x + a.x, c.y - d.y + a.y, c.z - d.z + a.z, c.w - d.w + a.w);
+        c4[i] = c;
+        sum += c.x + c.y + c.z + c.w;
+    }
+    uint32_t total;
+    block_excl_scan<4>(sum, &total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// Rebuild, level 2 (at most SCAN_MERGE_TILES tiles): the new offsets, the
+// first arrival slot of each bucket (its stayers come first), the occupancy
+// bits; arrivals and departures are zeroed for the next step.
+__global__ __launch_bounds__(256) void k_r_scan_apply(const uint32_t *__restrict__ counts,
+                                                      uint32_t *__restrict__ arr, uint32_t *__restrict__ dep,
+                                                      const uint32_t *__restrict__ tile_sums, uint32_t ntiles,
+                                                      uint32_t *__restrict__ noff, uint32_t *__restrict__ abase,
+                                                      uint32_t *__restrict__ occ) {
+    uint32_t part = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += 256u) part += tile_sums[t];
+    uint32_t prefix;
+    block_excl_scan<4>(part, &prefix);
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * 16u;
+    const uint4 *c4 = reinterpret_cast<const uint4 *>(counts + base);
+    uint4 *a4 = reinterpret_cast<uint4 *>(arr + base);
+    uint4 *d4 = reinterpret_cast<uint4 *>(dep + base);
+    uint32_t c[16], a[16];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint4 v = c4[k], w = a4[k];
+        c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w;
+        a[4 * k] = w.x; a[4 * k + 1] = w.y; a[4 * k + 2] = w.z; a[4 * k + 3] = w.w;
+    }
+    uint32_t sum = 0, bits = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        sum += c[k];
+        bits |= (c[k] != 0u ? 1u : 0u) << k;
+    }
+    const uint32_t hi = __shfl_xor(bits, 1);
+    if ((threadIdx.x & 1u) == 0u) occ[base / 32u] = bits | (hi << 16);
+    uint32_t total;
+    uint32_t run = prefix + block_excl_scan<4>(sum, &total);
+    if (blockIdx.x == ntiles - 1u && threadIdx.x == 0) noff[(size_t)ntiles * SCAN_TILE] = prefix + total;
+    uint32_t o[16], ab[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        o[k] = run;
+        ab[k] = run + c[k] - a[k];
+        run += c[k];
+    }
+    uint4 *o4 = reinterpret_cast<uint4 *>(noff + base);
+    uint4 *b4 = reinterpret_cast<uint4 *>(abase + base);
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        o4[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+        b4[k] = make_uint4(ab[4 * k], ab[4 * k + 1], ab[4 * k + 2], ab[4 * k + 3]);
+        a4[k] = z;
+        d4[k] = z;
+    }
+}
+
+// Movers among old slots [lo, hi) (lo <= hi), from the mover bits.
+__device__ __forceinline__ uint32_t movers_between(const uint32_t *mbits, uint32_t lo, uint32_t hi) {
+    if (lo >= hi) return 0u;
+    const uint32_t wl = lo >> 5, wh = hi >> 5;
+    const uint32_t low_mask = ~0u << (lo & 31u);        // bits >= lo in word wl
+    const uint32_t high_mask = (1u << (hi & 31u)) - 1u;  // bits < hi in word wh
+    if (wl == wh) return (uint32_t)__popc(mbits[wl] & low_mask & high_mask);
+    uint32_t n = (uint32_t)__popc(mbits[wl] & low_mask);
+    for (uint32_t x = wl + 1u; x < wh; ++x) n += (uint32_t)__popc(mbits[x]);
+    if (high_mask) n += (uint32_t)__popc(mbits[wh] & high_mask);
+    return n;
+}
+
+// Rebuild, level 3: every record moves to its slot of the new layout. A bird
+// that kept its bucket keeps its order among the bucket's stayers; a mover is
+// placed after the stayers of its new bucket, by arrival rank.
+__global__ __launch_bounds__(TILE) void k_r_move(Geometry g, Pools p, Records cur, Records nxt, ResWork w,
+                                                 const uint32_t *__restrict__ off_cur,
+                                                 const uint32_t *__restrict__ noff) {
+    const uint32_t s = blockIdx.x * TILE + threadIdx.x;
+    if (s >= g.n_slots) return;
+    const uint32_t ko = cur.key[s], kn = w.knew[s];
+    uint32_t ns;
+    if (kn == ko) {
+        const uint32_t o = off_cur[ko];
+        if (o > s) {
+            atomicOr(p.err, ERR_SLOT);
+            return;
+        }
+        ns = noff[ko] + (s - o) - movers_between(w.mbits, o, s);
+    } else {
+        ns = w.abase[kn] + w.arank[s];
+    }
+    if (ns >= g.n_slots) {
+        atomicOr(p.err, ERR_SLOT);
+        return;
+    }
+    nxt.p4[ns] = cur.p4[s];
+    nxt.v4[ns] = cur.v4[s];
+    nxt.s4[ns] = cur.s4[s];
+    nxt.spin[ns] = cur.spin[s];
+    nxt.id[ns] = cur.id[s];
+    nxt.key[ns] = kn;
+}
+
+hipError_t launch_resident_enter(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s) {
+    if (g.n_birds == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_r_enter, dim3((g.n_birds + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_resident_flush(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s) {
+    if (g.n_slots == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_r_flush, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
+                                   const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, hipStream_t s) {
+    const uint32_t ntiles = (g.n_buckets + SCAN_TILE - 1) / SCAN_TILE;
+    if (ntiles == 0 || ntiles > SCAN_MERGE_TILES) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_r_scan_reduce, dim3(ntiles), dim3(256), 0, s, p.counts, w.arr, w.dep, p.tile_sums);
+    hipLaunchKernelGGL(k_r_scan_apply, dim3(ntiles), dim3(256), 0, s, p.counts, w.arr, w.dep, p.tile_sums, ntiles,
+                       off_nxt, w.abase, p.occ);
+    if (g.n_slots)
+        hipLaunchKernelGGL(k_r_move, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, cur, nxt, w,
+                           off_cur, off_nxt);
     return hipGetLastError();
 }
 

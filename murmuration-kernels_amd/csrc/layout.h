@@ -84,6 +84,28 @@ __device__ __forceinline__ uint2 bucket_range(const uint32_t *offsets, uint32_t 
     return r;
 }
 
+// Resident layout (one flock, fh_step with FH_FLAG_RESIDENT): the state
+// itself is kept in slot order between steps, in two buffers the steps
+// alternate between. Slot s holds bird id[s], in bucket key[s]:
+//   p4 = (px, py, pz, next_decision), v4 = (vx, vy, vz, roll),
+//   s4 = (sx, sy, sz, radius), spin.
+struct Records {
+    float4 *p4, *v4, *s4;
+    float *spin;
+    uint32_t *id, *key;
+};
+
+// Per-step work arrays of the resident rebuild. A bird whose new bucket
+// differs from its old one ("mover") counts itself out of the old bucket
+// (dep) and into the new one (arr, returning its arrival rank).
+struct ResWork {
+    uint32_t *knew;   // per old slot: bucket of the new position
+    uint32_t *arank;  // per old slot (movers): arrival rank in the new bucket
+    uint32_t *mbits;  // per old slot: mover bit (32 slots per word)
+    uint32_t *arr, *dep;  // per bucket: arrivals, departures (zero between steps)
+    uint32_t *abase;      // per bucket: first slot of its arrivals in the new layout
+};
+
 // Launch geometry of one build/step over the pools.
 struct Geometry {
     const Seg *segs;           // device segment table
@@ -100,9 +122,10 @@ struct Geometry {
 hipError_t launch_hash_count(const Geometry &g, const Pools &p, bool wave_agg, hipStream_t s);
 // Exclusive scan of the counts into offsets (and the occupancy bitmap). With
 // tick_nseg > 0 it also advances the clocks of the live segments among
-// tick_segs[0, tick_nseg) by dt before any later launch reads them.
+// tick_segs[0, tick_nseg) by dt before any later launch reads them. The
+// counts are zeroed for the next histogram unless keep_counts.
 hipError_t launch_scan(const Geometry &g, const Pools &p, Seg *tick_segs, uint32_t tick_nseg, float dt,
-                       hipStream_t s);
+                       hipStream_t s, bool keep_counts = false);
 hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, bool relative_time,
                           hipStream_t s);
 // Walk switches of decide (bit set = enabled).
@@ -110,6 +133,22 @@ constexpr int WALK_TILED = 1;  // dense neighbourhoods through LDS
 constexpr int WALK_OCC = 2;    // skip empty buckets by the occupancy bitmap
 hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
                          uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s);
+// decide over resident records (p.offsets: the current offsets). Adds the
+// due birds to *p.due_count and zeroes *due_clear (the other step's counter).
+hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Records &rec, uint32_t block,
+                                  uint32_t birds_per_thread, uint32_t group, int walk_flags, uint32_t *due_clear,
+                                  hipStream_t s);
+// Resident passes (one flock): records from the bird-order state and a grid
+// built by hash + scan (counts kept); banked flight in slot order with the
+// movers counted (and the flock clock committed: += DT); the rebuild of the
+// grid and the move of every record to its slot in `nxt`; records back to the
+// bird-order state.
+hipError_t launch_resident_enter(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s);
+hipError_t launch_resident_integrate(const Geometry &g, const Pools &p, const Records &rec, const ResWork &w,
+                                     Seg *seg, hipStream_t s);
+hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
+                                   const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, hipStream_t s);
+hipError_t launch_resident_flush(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s);
 hipError_t launch_integrate(const Geometry &g, const Pools &p, bool hash, bool wave_agg,
                             hipStream_t s);
 hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, int walk,

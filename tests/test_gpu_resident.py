@@ -1,0 +1,119 @@
+"""Resident layout (FH_FLAG_RESIDENT): fh_step keeps the state in bucket order
+between steps and re-counts only the birds that change bucket. The grid it
+maintains must be the reference grid of the current positions (keys, counts
+and offsets bit-exact, per-bucket sets equal), and its steps must agree with
+the oracle like the standard path's."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+DT = np.float32(0.005)
+
+
+def cube(fh, n, seed=1):
+    return fh.gen_cube(n, fh.cube_side_for(n), seed)
+
+
+def rel_err(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)))) if a.size else 0.0
+
+
+def per_bucket_sets_equal(g_sorted, offsets, o_sorted):
+    for b in np.nonzero(np.diff(offsets.astype(np.int64)) > 1)[0]:
+        s, e = offsets[b], offsets[b + 1]
+        if set(g_sorted[s:e].tolist()) != set(o_sorted[s:e].tolist()):
+            return False
+    single = np.diff(offsets.astype(np.int64)) == 1
+    return np.array_equal(g_sorted[offsets[:-1][single]], o_sorted[offsets[:-1][single]])
+
+
+@pytest.mark.parametrize("n,kind,cell", [(200_000, "cube", 4.0), (50_000, "ref", 4.0), (300_000, "clumps", 2.0)])
+def test_resident_grid_is_the_reference_grid(fh, fr, n, kind, cell):
+    # After every run of resident steps the layout is the counting sort of the
+    # current positions; downloading leaves the layout and the next step
+    # enters it again.
+    if kind == "cube":
+        w0 = cube(fh, n, 3)
+    elif kind == "ref":
+        w0 = fh.gen_reference(n, 3)
+    else:
+        w0 = fh.gen_clumps(n, 4, 4.0, 200.0, 3)
+    with fh.Flock.from_words(w0, cell=cell, flags=fh.FLAG_RESIDENT | fh.FLAG_GRAPH) as f:
+        for k in range(3):
+            f.step(5 + 3 * k)
+            g = f.read_grid()
+            state, _ = f.download()
+            keys, counts, offsets, srt = fr.grid(state, f.buckets, cell)
+            print(kind, k, "movers-free check", int(counts.max()))
+            assert np.array_equal(g.keys, keys)
+            assert np.array_equal(g.counts, counts)
+            assert np.array_equal(g.offsets, offsets)
+            assert per_bucket_sets_equal(g.sorted, offsets, srt)
+
+
+@pytest.mark.parametrize("flags,group", [(64, 1), (64 | 4, 4), (64 | 4 | 32, 2), (64 | 4, 16), (64 | 4, 64)])
+def test_resident_step_matches_the_oracle(fh, fr, flags, group):
+    w0 = cube(fh, 200_000, 5)
+    with fh.Flock.from_words(w0, flags=flags, decide_group=group) as f:
+        f.step(1)
+        gpu, t = f.download()
+    cpu, tc = fr.step_grid(w0, 0.0, 1, threads=THREADS)
+    d = fr.diff(cpu, gpu)
+    print("resident 1-step", flags, group, d)
+    assert t == tc and d["nb_mismatch"] == 0
+    assert rel_err(gpu[:, 0:6], cpu[:, 0:6]) <= 1e-4
+
+
+def test_resident_c1_20_steps(fh, fr):
+    # 10,000 birds, 100 m cube: 20 consecutive resident steps against the
+    # oracle's 20 steps (the trajectories drift apart only by rounding).
+    w0 = cube(fh, 10_000)
+    with fh.Flock.from_words(w0, flags=fh.FLAG_RESIDENT | fh.FLAG_GRAPH) as f:
+        f.step(20)
+        gpu, t = f.download()
+    cpu, tc = fr.step(w0, 0.0, 20, threads=THREADS)
+    assert t == tc
+    d = fr.diff(cpu, gpu)
+    drift = rel_err(gpu[:, 0:6], cpu[:, 0:6])
+    print("resident C1 20-step drift", drift, d)
+    assert drift <= 1e-3
+
+
+def test_resident_and_standard_paths_agree(fh, fr):
+    # The same 40 steps on the standard path and on the resident one, with
+    # the pass API used in between (which leaves the resident layout).
+    w0 = cube(fh, 100_000, 7)
+    outs = {}
+    for name, flags in (("standard", fh.FLAG_GRAPH), ("resident", fh.FLAG_GRAPH | fh.FLAG_RESIDENT)):
+        with fh.Flock.from_words(w0, flags=flags) as f:
+            f.step(20)
+            f.hash_count()
+            f.scan()
+            f.scatter()
+            f.step(20)
+            outs[name] = f.download()
+    (a, ta), (b, tb) = outs["standard"], outs["resident"]
+    assert ta == tb
+    assert rel_err(b[:, 0:6], a[:, 0:6]) <= 1e-3
+    assert np.mean(b[:, 13].view(np.uint32) == a[:, 13].view(np.uint32)) > 0.999
+
+
+def test_resident_due_count_and_timing(fh):
+    w0 = cube(fh, 100_000, 9)
+    with fh.Flock.from_words(w0, flags=fh.FLAG_RESIDENT) as f:
+        ms = f.step(4, 1, timing=True)
+    assert ms["timed_steps"] == 3
+    assert ms["decide"] > 0 and ms["integrate"] > 0 and ms["grid_scatter"] > 0
+    assert 5_000 < ms["due_last"] < 15_000  # about a tenth of the birds decide each step
+
+
+def test_resident_rejects_canonical(fh):
+    with pytest.raises(fh.FlockHipError) as e:
+        fh.Flock(1000, flags=fh.FLAG_RESIDENT | fh.FLAG_CANONICAL)
+    assert e.value.status == fh.ERR_INVALID
