@@ -3,7 +3,8 @@
 Picks the grid cell edge, the bucket count (bucket_shift), the decide
 workgroup size, the birds each decide workgroup scans per lane
 (birds_per_thread), the lanes that share one bird's walk (decide_group) and
-whether decide tests the bucket occupancy bitmap, by timing short runs of
+whether decide tests the bucket occupancy bitmap and (single flocks) whether
+fh_step keeps the state in bucket order, by timing short runs of
 the real workload. Each candidate is timed over `steps` steps after `warmup`
 untimed ones. Results can be cached in a JSON file keyed by the workload's
 signature.
@@ -20,7 +21,7 @@ import time
 
 import numpy as np
 
-from . import FLAG_GRAPH, FLAG_OCC, Batch, Flock
+from . import FLAG_CANONICAL, FLAG_GRAPH, FLAG_OCC, FLAG_RESIDENT, Batch, Flock
 
 BPT = (1, 2, 4, 8, 16)
 BLOCKS = (64, 128, 256, 512)
@@ -78,7 +79,7 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
     """Returns {"params": {...}, "flags": f, "ms_per_step": x, "trials": [...]}
     for the batch of `starts` (kind "batch") or the single flock starts[0]
     ("plain"). `flags` is the caller's flags plus the switches the search
-    turned on (FLAG_OCC). With `thorough` the launch shape is searched
+    turned on (FLAG_OCC, FLAG_RESIDENT). With `thorough` the launch shape is searched
     exhaustively (group x birds_per_thread x block) before and after the
     cell and bucket searches; otherwise one coordinate sweep per tunable.
     `times` are the flocks' clocks (default 0), e.g. of states downloaded
@@ -93,13 +94,13 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
 
     def timer(prm):
         prm = dict(prm)
-        f = flags | (FLAG_OCC if prm.pop("occ") else 0)
+        f = flags | (FLAG_OCC if prm.pop("occ") else 0) | (FLAG_RESIDENT if prm.pop("resident") else 0)
         if kind == "batch":
             return _time_batch(starts, times, prm, f, warmup, steps)
         return _time_plain(starts[0], times[0], prm, f, warmup, steps)
 
     best = {"cell": float(cells[0]), "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0,
-            "occ": 0}
+            "occ": 0, "resident": 0}
     trials = []
     seen = {}
 
@@ -134,14 +135,17 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
     else:
         for nv in shape:
             sweep([nv])
-    for nv in (("cell", cells), ("bucket_shift", shifts), ("occ", (0, 1)), ("cell", cells)):
+    # the resident layout is a single-flock fh_step mode, not for canonical order
+    resident = (0, 1) if kind == "plain" and not (flags & FLAG_CANONICAL) else (0,)
+    for nv in (("resident", resident), ("cell", cells), ("bucket_shift", shifts), ("occ", (0, 1)), ("cell", cells)):
         sweep([nv])
     if thorough:
         sweep(shape[:2])
     else:
         sweep([shape[0]])
-    params = {k: v for k, v in best.items() if k != "occ"}
-    out = {"params": params, "flags": flags | (FLAG_OCC if best["occ"] else 0), "ms_per_step": best_t * 1e3,
+    params = {k: v for k, v in best.items() if k not in ("occ", "resident")}
+    out_flags = flags | (FLAG_OCC if best["occ"] else 0) | (FLAG_RESIDENT if best["resident"] else 0)
+    out = {"params": params, "flags": out_flags, "ms_per_step": best_t * 1e3,
            "trials": trials, "signature": key}
     if cache_path:
         cache[key] = out
