@@ -11,6 +11,7 @@ Prints one JSON line (the metric is bird-steps/s of the whole step).
 from __future__ import annotations
 
 import argparse
+import glob
 import json
 import os
 import sys
@@ -40,6 +41,24 @@ def step_bytes(n: int, buckets: int, due: int) -> dict:
         "decide": 28 * n + 4 * buckets + 32 * due,
         "integrate": 72 * n,
     }
+
+
+def profiled_traffic(params: dict, flags: int):
+    """HBM bytes per decide launch measured by the committed rocprofv3 counter
+    runs (FETCH_SIZE and WRITE_SIZE, each scaled by a copy-kernel
+    calibration): the 'traffic:' line of a summary under profiles/ taken at
+    these tunables and flags, or None when no profile matches them."""
+    want = {**{k: float(v) for k, v in params.items()}, "flags": float(flags)}
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as fp:
+            for line in fp:
+                if not line.startswith("traffic: kernel=decide "):
+                    continue
+                fields = dict(f.split("=", 1) for f in line.split()[1:])
+                got = {k: float(v) for k, v in (kv.split("=") for kv in fields["params"].split(","))}
+                if got == want:
+                    return {"bytes": float(fields["bytes"]), "profile": os.path.basename(path)}
+    return None
 
 
 def cpu_baseline(words: np.ndarray, n: int, sample: int) -> dict:
@@ -166,6 +185,8 @@ def main() -> int:
     achieved = nbytes["decide"] / (dec_ms * 1e-3) / 1e9
     step_gbs = sum(nbytes.values()) / (passes["step_total"] * 1e-3) / 1e9
     vgpr = fh.kernel_info("decide")["vgprs"]
+    full_prm = {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0, **prm}
+    traffic = profiled_traffic(full_prm, flags)
 
     out = {
         "metric": "bird-steps/s served+batched vs plain; decide ms @1M birds; p50/p99 latency",
@@ -186,7 +207,7 @@ def main() -> int:
             "birds": n,
             "buckets": buckets,
             "flags": flags,
-            "params": {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, **prm},
+            "params": full_prm,
             "autotuned": tuned is not None,
             "device": dev["name"],
         },
@@ -199,8 +220,9 @@ def main() -> int:
         "step_roofline": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_gbs / HBM_PEAK_GBS, "bytes": sum(nbytes.values())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "decide",
-                     "bytes_per_launch": nbytes["decide"]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
+                     "kernel": "decide", "bytes_per_launch": nbytes["decide"],
+                     "traffic_profile": traffic["profile"] if traffic else None},
     }
     if not args.no_cpu:
         log("cpu baseline")

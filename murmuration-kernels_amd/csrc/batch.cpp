@@ -115,10 +115,9 @@ static fh_status batch_step_launches(fh_batch *b, hipEvent_t *ev) {
     b->counts_dirty = false;
     if (ev) FH_HIP(hipEventRecord(ev[2], b->stream));
     FH_HIP(launch_scatter(b->geo, b->P, (b->prm.flags & FH_FLAG_CANONICAL) != 0, true, b->stream));
-    if (ev) FH_HIP(hipEventRecord(ev[3], b->stream));
+    // the decide launch records ev[3] and ev[4] at its kernel's start and end
     FH_HIP(launch_decide(b->geo, b->P, 0.0f, b->prm.decide_block, b->prm.birds_per_thread, b->prm.decide_group,
-                         walk_flags(b->prm.flags), b->stream));
-    if (ev) FH_HIP(hipEventRecord(ev[4], b->stream));
+                         walk_flags(b->prm.flags), b->stream, ev ? DecideTiming{ev[3], ev[4]} : DecideTiming{}));
     FH_HIP(launch_integrate(b->geo, b->P, true, (b->prm.flags & FH_FLAG_WAVE_AGG) != 0, b->stream));
     b->counts_dirty = true;
     b->hashed = true;

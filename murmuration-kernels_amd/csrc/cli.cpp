@@ -15,7 +15,8 @@
 static int usage() {
     std::fprintf(stderr,
                  "usage: flock_hip --in FILE [--steps K] [--warmup W] [--out FILE]\n"
-                 "       [--canonical] [--unfused] [--graph] [--cell M] [--block B] [--bpt K]\n"
+                 "       [--canonical] [--unfused] [--graph] [--resident] [--occ]\n"
+                 "       [--cell M] [--block B] [--bpt K] [--group G] [--shift S]\n"
                  "(the first W steps are not timed)\n");
     return 2;
 }
@@ -49,6 +50,10 @@ int main(int argc, char **argv) {
         else if (a == "--cell") prm.cell = (float)std::atof(val());
         else if (a == "--block") prm.decide_block = (uint32_t)std::atol(val());
         else if (a == "--bpt") prm.birds_per_thread = (uint32_t)std::atol(val());
+        else if (a == "--group") prm.decide_group = (uint32_t)std::atol(val());
+        else if (a == "--shift") prm.bucket_shift = (uint32_t)std::atol(val());
+        else if (a == "--occ") prm.flags |= FH_FLAG_OCC;
+        else if (a == "--resident") prm.flags |= FH_FLAG_RESIDENT;
         else return usage();
     }
     if (!in) {

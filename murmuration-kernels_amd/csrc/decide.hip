@@ -14,6 +14,8 @@
 // the r box is the r walk's own order, so each accumulator receives the same
 // candidates in the same order as in the reference; walk-1 membership is the
 // oracle's own test d^2 <= r^2.
+#include <hip/hip_ext.h>
+
 #include "layout.h"
 #include "model.h"
 
@@ -524,25 +526,31 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
     *count = c;
 }
 
+// With timing events, the launch itself records the kernel's start and end
+// (hipExtLaunchKernelGGL), so the measured interval is the kernel alone.
 template <int BLOCK, int G, bool RES>
 static void launch_decide_block(const Geometry &g, const Pools &p, const Records &rec, float time, uint32_t chunk,
-                                int walk_flags, uint32_t *due_clear, hipStream_t s) {
+                                int walk_flags, uint32_t *due_clear, const DecideTiming &tm, hipStream_t s) {
     const uint32_t blocks = (g.n_slots + chunk - 1) / chunk;
-    hipLaunchKernelGGL((k_decide<BLOCK, G, RES>), dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s, g, p, rec,
-                       time, chunk, walk_flags, due_clear);
+    if (tm.start)
+        hipExtLaunchKernelGGL((k_decide<BLOCK, G, RES>), dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s,
+                              tm.start, tm.stop, 0u, g, p, rec, time, chunk, walk_flags, due_clear);
+    else
+        hipLaunchKernelGGL((k_decide<BLOCK, G, RES>), dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s, g, p,
+                           rec, time, chunk, walk_flags, due_clear);
 }
 
 template <int BLOCK, bool RES>
 static hipError_t launch_decide_group(const Geometry &g, const Pools &p, const Records &rec, float time,
                                       uint32_t chunk, uint32_t group, int walk_flags, uint32_t *due_clear,
-                                      hipStream_t s) {
+                                      const DecideTiming &tm, hipStream_t s) {
     switch (group) {
-        case 1: launch_decide_block<BLOCK, 1, RES>(g, p, rec, time, chunk, walk_flags, due_clear, s); break;
-        case 2: launch_decide_block<BLOCK, 2, RES>(g, p, rec, time, chunk, walk_flags, due_clear, s); break;
-        case 4: launch_decide_block<BLOCK, 4, RES>(g, p, rec, time, chunk, walk_flags, due_clear, s); break;
-        case 8: launch_decide_block<BLOCK, 8, RES>(g, p, rec, time, chunk, walk_flags, due_clear, s); break;
-        case 16: launch_decide_block<BLOCK, 16, RES>(g, p, rec, time, chunk, walk_flags, due_clear, s); break;
-        case 64: launch_decide_block<BLOCK, 64, RES>(g, p, rec, time, chunk, walk_flags, due_clear, s); break;
+        case 1: launch_decide_block<BLOCK, 1, RES>(g, p, rec, time, chunk, walk_flags, due_clear, tm, s); break;
+        case 2: launch_decide_block<BLOCK, 2, RES>(g, p, rec, time, chunk, walk_flags, due_clear, tm, s); break;
+        case 4: launch_decide_block<BLOCK, 4, RES>(g, p, rec, time, chunk, walk_flags, due_clear, tm, s); break;
+        case 8: launch_decide_block<BLOCK, 8, RES>(g, p, rec, time, chunk, walk_flags, due_clear, tm, s); break;
+        case 16: launch_decide_block<BLOCK, 16, RES>(g, p, rec, time, chunk, walk_flags, due_clear, tm, s); break;
+        case 64: launch_decide_block<BLOCK, 64, RES>(g, p, rec, time, chunk, walk_flags, due_clear, tm, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipSuccess;
@@ -551,15 +559,15 @@ static hipError_t launch_decide_group(const Geometry &g, const Pools &p, const R
 template <bool RES>
 static hipError_t launch_decide_any(const Geometry &g, const Pools &p, const Records &rec, float time,
                                     uint32_t block, uint32_t birds_per_thread, uint32_t group, int walk_flags,
-                                    uint32_t *due_clear, hipStream_t s) {
+                                    uint32_t *due_clear, const DecideTiming &tm, hipStream_t s) {
     if (g.n_slots == 0) return hipSuccess;
     const uint32_t chunk = block * birds_per_thread;
     hipError_t e;
     switch (block) {
-        case 64: e = launch_decide_group<64, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, s); break;
-        case 128: e = launch_decide_group<128, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, s); break;
-        case 256: e = launch_decide_group<256, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, s); break;
-        case 512: e = launch_decide_group<512, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, s); break;
+        case 64: e = launch_decide_group<64, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, tm, s); break;
+        case 128: e = launch_decide_group<128, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, tm, s); break;
+        case 256: e = launch_decide_group<256, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, tm, s); break;
+        case 512: e = launch_decide_group<512, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, tm, s); break;
         default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
@@ -567,14 +575,15 @@ static hipError_t launch_decide_any(const Geometry &g, const Pools &p, const Rec
 }
 
 hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
-                         uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s) {
-    return launch_decide_any<false>(g, p, Records{}, time, block, birds_per_thread, group, walk_flags, nullptr, s);
+                         uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s,
+                         const DecideTiming &tm) {
+    return launch_decide_any<false>(g, p, Records{}, time, block, birds_per_thread, group, walk_flags, nullptr, tm, s);
 }
 
 hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Records &rec, uint32_t block,
                                   uint32_t birds_per_thread, uint32_t group, int walk_flags, uint32_t *due_clear,
-                                  hipStream_t s) {
-    return launch_decide_any<true>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear, s);
+                                  hipStream_t s, const DecideTiming &tm) {
+    return launch_decide_any<true>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear, tm, s);
 }
 
 hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, int walk, uint32_t *ids,

@@ -88,9 +88,9 @@ static fh_status do_scatter(fh_flock *h, bool relative) {
     return FH_OK;
 }
 
-static fh_status do_decide(fh_flock *h, float time_arg) {
+static fh_status do_decide(fh_flock *h, float time_arg, const DecideTiming &tm = DecideTiming{}) {
     FH_HIP(launch_decide(h->geo, h->P, time_arg, h->prm.decide_block, h->prm.birds_per_thread,
-                         h->prm.decide_group, walk_flags(h->prm.flags), h->stream));
+                         h->prm.decide_group, walk_flags(h->prm.flags), h->stream, tm));
     h->stage = Stage::Decided;
     return FH_OK;
 }
@@ -120,9 +120,8 @@ static fh_status step_launches(fh_flock *h, bool fused, hipEvent_t *ev) {
     if (st) return st;
     if (ev) FH_HIP(hipEventRecord(ev[2], h->stream));
     if ((st = do_scatter(h, true))) return st;
-    if (ev) FH_HIP(hipEventRecord(ev[3], h->stream));
-    if ((st = do_decide(h, 0.0f))) return st;
-    if (ev) FH_HIP(hipEventRecord(ev[4], h->stream));
+    // the decide launch records ev[3] and ev[4] at its kernel's start and end
+    if ((st = do_decide(h, 0.0f, ev ? DecideTiming{ev[3], ev[4]} : DecideTiming{}))) return st;
     if ((st = do_integrate(h, fused))) return st;
     if (ev) FH_HIP(hipEventRecord(ev[5], h->stream));
     return FH_OK;
@@ -245,10 +244,10 @@ static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
     Pools p = h->P;
     p.offsets = h->off[q];
     p.due_count = h->res_due + q;
-    if (ev) FH_HIP(hipEventRecord(ev[0], h->stream));
+    // the decide launch records ev[0] and ev[1] at its kernel's start and end
     FH_HIP(launch_decide_resident(h->geo, p, h->R[q], h->prm.decide_block, h->prm.birds_per_thread,
-                                  h->prm.decide_group, walk_flags(h->prm.flags), h->res_due + (1u - q), h->stream));
-    if (ev) FH_HIP(hipEventRecord(ev[1], h->stream));
+                                  h->prm.decide_group, walk_flags(h->prm.flags), h->res_due + (1u - q), h->stream,
+                                  ev ? DecideTiming{ev[0], ev[1]} : DecideTiming{}));
     FH_HIP(launch_resident_integrate(h->geo, p, h->R[q], h->W, h->d_seg, h->stream));
     if (ev) FH_HIP(hipEventRecord(ev[2], h->stream));
     FH_HIP(launch_resident_rebuild(h->geo, p, h->R[q], h->R[1u - q], h->W, h->off[q], h->off[1u - q], h->stream));

@@ -131,13 +131,18 @@ hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, boo
 // Walk switches of decide (bit set = enabled).
 constexpr int WALK_TILED = 1;  // dense neighbourhoods through LDS
 constexpr int WALK_OCC = 2;    // skip empty buckets by the occupancy bitmap
+// Events the decide launch records itself (kernel start and end), or none.
+struct DecideTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
 hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
-                         uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s);
+                         uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s,
+                         const DecideTiming &tm = DecideTiming{});
 // decide over resident records (p.offsets: the current offsets). Adds the
 // due birds to *p.due_count and zeroes *due_clear (the other step's counter).
 hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Records &rec, uint32_t block,
                                   uint32_t birds_per_thread, uint32_t group, int walk_flags, uint32_t *due_clear,
-                                  hipStream_t s);
+                                  hipStream_t s, const DecideTiming &tm = DecideTiming{});
 // Resident passes (one flock): records from the bird-order state and a grid
 // built by hash + scan (counts kept); banked flight in slot order with the
 // movers counted (and the flock clock committed: += DT); the rebuild of the
