@@ -261,7 +261,7 @@ def reference_leg(args) -> dict:
         w = fh.gen_reference(n, 1)
         # the reference grid (4 m cells, buckets_for(n) buckets); launch shape autotuned
         tuned = autotune.tune([w], "plain", flags=fh.FLAG_UNFUSED, warmup=20, steps=5, cells=(4.0,), shifts=(0,),
-                              thorough=False, progress=log)
+                              spatial=(0,), thorough=False, progress=log)
         for name, prm, flags in (("default", {}, fh.FLAG_UNFUSED), ("tuned", tuned["params"], tuned["flags"])):
             with fh.Flock.from_words(w, flags=flags, **prm) as f:
                 f.step(20)

@@ -94,6 +94,11 @@ typedef struct fh_params {
  * that arrive follow them. Single flocks only; not with FH_FLAG_CANONICAL.
  * Any other call on the flock first puts the state back in bird order. */
 #define FH_FLAG_RESIDENT 64u
+/* Bucket cells by a locality-preserving hash (the Morton order of the cell
+ * coordinates within 1024-cell blocks) instead of the reference's hash: nearby
+ * cells get nearby buckets and slots, so decide's walks share cache lines.
+ * Neighbour sets are unchanged; keys differ from the reference's. */
+#define FH_FLAG_SPATIAL 128u
 
 /* Per-pass GPU time of fh_step, mean ms per timed step (hipEvents on the
  * handle's stream), in the rows of the reference report

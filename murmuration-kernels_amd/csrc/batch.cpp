@@ -388,6 +388,7 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
     s.inv_cell = 1.0f / b->prm.cell;
     s.time = time;
     s.live = 1;
+    s.mode = (b->prm.flags & FH_FLAG_SPATIAL) ? SEG_SPATIAL : 0u;
     b->segs[f] = s;
     b->times[f] = time;
     for (uint32_t t = 0; t < ntile; ++t) b->tile_seg[t0 + t] = f;

@@ -111,7 +111,7 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
             s0[k] = 0u;
             s1[k] = 0u;
             want[k] = base + k * G < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2);
-            bk[k] = sg.bucket_base + (bucket_hash(x, y, z) & sg.mask);
+            bk[k] = bucket_of_cell(sg, x, y, z);
 #pragma unroll
             for (int s = 0; s < G; ++s)
                 if (++x > w.hi[0]) {
@@ -186,7 +186,7 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
             s0[k] = 0u;
             s1[k] = 0u;
             want[k] = base + k < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2);
-            bk[k] = sg.bucket_base + (bucket_hash(x, y, z) & sg.mask);
+            bk[k] = bucket_of_cell(sg, x, y, z);
             if (++x > w.hi[0]) {
                 x = w.lo[0];
                 if (++y > w.hi[1]) {
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, Records r
                                           cy <= w.hi[1] && cz >= w.lo[2] && cz <= w.hi[2] &&
                                           cell_meets_sphere(cx, cy, cz, sg.cell, w.p, w.c2);
                         if (!__syncthreads_or(need)) continue;
-                        const uint32_t b = sg.bucket_base + (bucket_hash(cx, cy, cz) & sg.mask);
+                        const uint32_t b = bucket_of_cell(sg, cx, cy, cz);
                         if (use_occ && !bucket_occupied(p, b)) continue;  // same b for every lane
                         const uint2 rg = bucket_range(p.offsets, b);
                         const uint32_t s0 = rg.x;

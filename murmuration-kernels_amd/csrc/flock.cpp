@@ -351,6 +351,7 @@ fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
     h->seg.inv_cell = 1.0f / prm.cell;
     h->seg.time = 0.0f;
     h->seg.live = 1;
+    h->seg.mode = (prm.flags & FH_FLAG_SPATIAL) ? SEG_SPATIAL : 0u;
     h->geo.segs = h->d_seg;
     h->geo.tile_seg = nullptr;
     h->geo.n_birds = n;

@@ -52,9 +52,7 @@ __device__ __forceinline__ const Seg &tile_segment(const Geometry &g, uint32_t t
 }
 
 __device__ __forceinline__ uint32_t key_of(float x, float y, float z, const Seg &sg) {
-    return sg.bucket_base +
-           (bucket_hash(cell_coord(x, sg.inv_cell), cell_coord(y, sg.inv_cell), cell_coord(z, sg.inv_cell)) &
-            sg.mask);
+    return bucket_of_cell(sg, cell_coord(x, sg.inv_cell), cell_coord(y, sg.inv_cell), cell_coord(z, sg.inv_cell));
 }
 
 // counts[key]++ returning the bird's arrival rank in its bucket. With

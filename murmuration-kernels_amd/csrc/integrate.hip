@@ -85,9 +85,8 @@ __global__ __launch_bounds__(TILE) void k_integrate(Geometry g, Pools p) {
         p.vel4[bird] = v4;
         p.spin[bird] = rate;
         if (HASH) {
-            k = sg.bucket_base + (bucket_hash(cell_coord(p4.x, sg.inv_cell), cell_coord(p4.y, sg.inv_cell),
-                                              cell_coord(p4.z, sg.inv_cell)) &
-                                  sg.mask);
+            k = bucket_of_cell(sg, cell_coord(p4.x, sg.inv_cell), cell_coord(p4.y, sg.inv_cell),
+                               cell_coord(p4.z, sg.inv_cell));
             p.key[bird] = k;
         }
     }
@@ -115,9 +114,8 @@ __global__ __launch_bounds__(TILE) void k_r_integrate(Geometry g, Records rec, R
         rec.p4[s] = p4;
         rec.v4[s] = v4;
         rec.spin[s] = rate;
-        const uint32_t kn = sg.bucket_base + (bucket_hash(cell_coord(p4.x, sg.inv_cell), cell_coord(p4.y, sg.inv_cell),
-                                                          cell_coord(p4.z, sg.inv_cell)) &
-                                              sg.mask);
+        const uint32_t kn = bucket_of_cell(sg, cell_coord(p4.x, sg.inv_cell), cell_coord(p4.y, sg.inv_cell),
+                                           cell_coord(p4.z, sg.inv_cell));
         const uint32_t ko = rec.key[s];
         w.knew[s] = kn;
         if (kn != ko) {

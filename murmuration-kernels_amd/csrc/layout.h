@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "model.h"
+
 namespace fh {
 
 constexpr uint32_t TILE = 256;        // birds per workgroup of the bird-order passes
@@ -55,7 +57,16 @@ struct Seg {
     float inv_cell;        // 1 / cell
     float time;            // clock of the step in flight
     uint32_t live;         // 0 for a freed segment
+    uint32_t mode;         // SEG_* bits
 };
+
+constexpr uint32_t SEG_SPATIAL = 1u;  // buckets by spatial_hash instead of the reference hash
+
+// Bucket of cell (cx, cy, cz) in segment sg.
+__device__ __forceinline__ uint32_t bucket_of_cell(const Seg &sg, int32_t cx, int32_t cy, int32_t cz) {
+    const uint32_t h = (sg.mode & SEG_SPATIAL) ? spatial_hash(cx, cy, cz) : bucket_hash(cx, cy, cz);
+    return sg.bucket_base + (h & sg.mask);
+}
 
 struct Pools {
     float4 *pos4, *vel4, *steer4;

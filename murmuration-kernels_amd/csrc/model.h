@@ -109,6 +109,25 @@ FH_HD int32_t cell_coord(float p, float inv_cell) {
 FH_HD uint32_t bucket_hash(int32_t cx, int32_t cy, int32_t cz) {
     return (uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u ^ (uint32_t)cz * 83492791u;
 }
+// Locality-preserving bucket hash (FH_FLAG_SPATIAL; not the reference's): the
+// Morton interleave of the low 10 bits of each cell coordinate, plus the
+// reference hash of the 1024-cell block the cell lies in. Nearby cells get
+// nearby buckets, so their ranges share lines of the offsets table and their
+// birds neighbouring slots; after the bucket mask, cells 2^(log2(B)/3) apart
+// along an axis share buckets as in any hash.
+FH_HD uint32_t spread10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+FH_HD uint32_t spatial_hash(int32_t cx, int32_t cy, int32_t cz) {
+    const uint32_t m = spread10((uint32_t)cx) | (spread10((uint32_t)cy) << 1) | (spread10((uint32_t)cz) << 2);
+    return m + bucket_hash((int32_t)((uint32_t)cx >> 10), (int32_t)((uint32_t)cy >> 10),
+                           (int32_t)((uint32_t)cz >> 10));
+}
 // cell_intersects_sphere (cuda/flock.cu:85-91), +1e-4 slack on r^2.
 FH_HD bool cell_meets_sphere(int32_t cx, int32_t cy, int32_t cz, float cell, f3 p, float r2) {
     const float lo_x = (float)cx * cell, lo_y = (float)cy * cell, lo_z = (float)cz * cell;

@@ -27,6 +27,7 @@ FLAG_WAVE_AGG = 8
 FLAG_NO_TILE = 16
 FLAG_OCC = 32
 FLAG_RESIDENT = 64
+FLAG_SPATIAL = 128
 
 OK = 0
 ERR_INVALID, ERR_FLK, ERR_NOMEM, ERR_HIP, ERR_BUDGET, ERR_STATE, ERR_IO, ERR_NODEVICE = range(1, 9)
@@ -549,6 +550,32 @@ def server_replay(birds, steps, seeds, arrival_s, **kw) -> tuple[list[dict], dic
     _check(lib().fh_server_replay(C.addressof(cfg), _ptr(b), _ptr(s), _ptr(sd), _ptr(at), n, C.addressof(res),
                                   C.addressof(st)))
     return [res[i].as_dict() for i in range(n)], st.as_dict()
+
+
+def _spread10(v: np.ndarray) -> np.ndarray:
+    v = v.astype(np.uint64) & 0x3FF
+    v = (v | (v << 16)) & 0x030000FF
+    v = (v | (v << 8)) & 0x0300F00F
+    v = (v | (v << 4)) & 0x030C30C3
+    v = (v | (v << 2)) & 0x09249249
+    return v
+
+
+def spatial_keys(words: np.ndarray, buckets: int, cell: float = 4.0) -> np.ndarray:
+    """Bucket of each bird under FLAG_SPATIAL (the library's spatial hash:
+    Morton order of the low 10 bits of each cell coordinate plus the
+    reference hash of the 1024-cell block), for checking the grid."""
+    w = np.ascontiguousarray(words, np.float32).reshape(-1, WORDS)
+    inv = np.float32(1.0) / np.float32(cell)
+    c = []
+    for a in range(3):
+        f = np.floor(w[:, a] * inv)
+        f = np.where(np.isnan(f), 0.0, np.clip(f, -2147483648.0, 2147483647.0))
+        c.append(f.astype(np.int64).astype(np.uint32).astype(np.uint64))
+    m = _spread10(c[0]) | (_spread10(c[1]) << 1) | (_spread10(c[2]) << 2)
+    hx, hy, hz = (ci >> 10 for ci in c)
+    blk = ((hx * 73856093) ^ (hy * 19349663) ^ (hz * 83492791)) & 0xFFFFFFFF
+    return (((m + blk) & 0xFFFFFFFF) & (buckets - 1)).astype(np.uint32)
 
 
 def state_checksum(words: np.ndarray) -> int:

@@ -3,8 +3,9 @@
 Picks the grid cell edge, the bucket count (bucket_shift), the decide
 workgroup size, the birds each decide workgroup scans per lane
 (birds_per_thread), the lanes that share one bird's walk (decide_group) and
-whether decide tests the bucket occupancy bitmap and (single flocks) whether
-fh_step keeps the state in bucket order, by timing short runs of
+whether decide tests the bucket occupancy bitmap, whether cells are bucketed
+by the spatial hash and (single flocks) whether fh_step keeps the state in
+bucket order, by timing short runs of
 the real workload. Each candidate is timed over `steps` steps after `warmup`
 untimed ones. Results can be cached in a JSON file keyed by the workload's
 signature.
@@ -21,7 +22,7 @@ import time
 
 import numpy as np
 
-from . import FLAG_CANONICAL, FLAG_GRAPH, FLAG_OCC, FLAG_RESIDENT, Batch, Flock
+from . import FLAG_CANONICAL, FLAG_GRAPH, FLAG_OCC, FLAG_RESIDENT, FLAG_SPATIAL, Batch, Flock
 
 BPT = (1, 2, 4, 8, 16)
 BLOCKS = (64, 128, 256, 512)
@@ -75,11 +76,12 @@ def _buckets(n: int) -> int:
 
 def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, steps: int = 8,
          cache_path: str | None = None, cells=CELLS, progress=None, thorough: bool = True, times=None,
-         shifts=SHIFTS) -> dict:
+         shifts=SHIFTS, spatial=(0, 1)) -> dict:
     """Returns {"params": {...}, "flags": f, "ms_per_step": x, "trials": [...]}
     for the batch of `starts` (kind "batch") or the single flock starts[0]
     ("plain"). `flags` is the caller's flags plus the switches the search
-    turned on (FLAG_OCC, FLAG_RESIDENT). With `thorough` the launch shape is searched
+    turned on (FLAG_OCC, FLAG_RESIDENT, FLAG_SPATIAL); `spatial=(0,)` keeps
+    the reference hash. With `thorough` the launch shape is searched
     exhaustively (group x birds_per_thread x block) before and after the
     cell and bucket searches; otherwise one coordinate sweep per tunable.
     `times` are the flocks' clocks (default 0), e.g. of states downloaded
@@ -95,12 +97,13 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
     def timer(prm):
         prm = dict(prm)
         f = flags | (FLAG_OCC if prm.pop("occ") else 0) | (FLAG_RESIDENT if prm.pop("resident") else 0)
+        f |= FLAG_SPATIAL if prm.pop("spatial") else 0
         if kind == "batch":
             return _time_batch(starts, times, prm, f, warmup, steps)
         return _time_plain(starts[0], times[0], prm, f, warmup, steps)
 
     best = {"cell": float(cells[0]), "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0,
-            "occ": 0, "resident": 0}
+            "occ": 0, "resident": 0, "spatial": 0}
     trials = []
     seen = {}
 
@@ -137,14 +140,16 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
             sweep([nv])
     # the resident layout is a single-flock fh_step mode, not for canonical order
     resident = (0, 1) if kind == "plain" and not (flags & FLAG_CANONICAL) else (0,)
-    for nv in (("resident", resident), ("cell", cells), ("bucket_shift", shifts), ("occ", (0, 1)), ("cell", cells)):
+    for nv in (("resident", resident), ("spatial", spatial), ("cell", cells), ("bucket_shift", shifts),
+               ("occ", (0, 1)), ("cell", cells)):
         sweep([nv])
     if thorough:
         sweep(shape[:2])
     else:
         sweep([shape[0]])
-    params = {k: v for k, v in best.items() if k not in ("occ", "resident")}
+    params = {k: v for k, v in best.items() if k not in ("occ", "resident", "spatial")}
     out_flags = flags | (FLAG_OCC if best["occ"] else 0) | (FLAG_RESIDENT if best["resident"] else 0)
+    out_flags |= FLAG_SPATIAL if best["spatial"] else 0
     out = {"params": params, "flags": out_flags, "ms_per_step": best_t * 1e3,
            "trials": trials, "signature": key}
     if cache_path:

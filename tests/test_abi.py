@@ -139,3 +139,15 @@ def test_bad_params_are_rejected_before_the_device(fh, kw):
     with pytest.raises(fh.FlockHipError) as e:
         fh.Flock(100, **kw)
     assert e.value.status == fh.ERR_INVALID, kw
+
+
+def test_spatial_keys_known_answers(fh):
+    # Cells (0,0,0), (1,0,0), (0,1,0), (0,0,1), (1,1,1), (2,0,0), (1023,0,0) at
+    # cell 4 m: Morton codes 0, 1, 2, 4, 7, 8, 0x9249249 (block hash 0); cell
+    # (1024,0,0) lies in block (1,0,0): Morton 0 + 73856093.
+    cells = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1], [2, 0, 0], [1023, 0, 0],
+                      [1024, 0, 0]], np.float32)
+    w = np.zeros((len(cells), fh.WORDS), np.float32)
+    w[:, 0:3] = cells * 4.0 + 0.5
+    keys = fh.spatial_keys(w, 1 << 30)
+    assert keys.tolist() == [0, 1, 2, 4, 7, 8, 0x9249249, 73856093]

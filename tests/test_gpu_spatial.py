@@ -1,0 +1,106 @@
+"""Spatial bucket hash (FH_FLAG_SPATIAL): the grid is still a counting sort of
+the cells (keys follow the library's spatial hash, restated in
+flock_hip.spatial_keys), neighbour sets are the brute-force ones, and the
+step agrees with the oracle like the reference hash's."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+SPATIAL = 128
+
+
+def cube(fh, n, seed=1):
+    return fh.gen_cube(n, fh.cube_side_for(n), seed)
+
+
+def rel_err(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)))) if a.size else 0.0
+
+
+def check_grid(fh, g, w, buckets, cell):
+    keys = fh.spatial_keys(w, buckets, cell)
+    assert np.array_equal(g.keys, keys)
+    counts = np.bincount(keys, minlength=buckets).astype(np.uint32)
+    assert np.array_equal(g.counts, counts)
+    off = np.zeros(buckets + 1, np.uint32)
+    off[1:] = np.cumsum(counts)
+    assert np.array_equal(g.offsets, off)
+    # every slot holds a bird of its bucket, each bird once
+    slot_bucket = np.repeat(np.arange(buckets, dtype=np.uint32), counts)
+    assert np.array_equal(keys[g.sorted], slot_bucket)
+    assert np.array_equal(np.sort(g.sorted), np.arange(len(keys), dtype=np.uint32))
+
+
+@pytest.mark.parametrize("n,cell,extra", [(200_000, 4.0, 0), (100_000, 6.0, 1), (5000, 4.0, 0)])
+def test_spatial_grid_is_a_counting_sort(fh, n, cell, extra):
+    w = cube(fh, n, 2)
+    if extra:  # coordinates past the 1024-cell blocks and the i32 range
+        w[:50, 0] = np.float32(1e6)
+        w[50:60, 1] = np.float32(-3e10)
+        w[60:70, 2] = np.float32(np.nan)
+    with fh.Flock.from_words(w, cell=cell, flags=SPATIAL) as f:
+        f.hash_count()
+        f.scan()
+        f.scatter()
+        check_grid(fh, f.read_grid(), w, f.buckets, cell)
+
+
+def test_spatial_neighbour_sets_match_brute_force(fh, fr):
+    w = cube(fh, 20_000, 4)
+    w[:, 8] = np.float32(2.5)
+    with fh.Flock.from_words(w, flags=SPATIAL) as f:
+        f.hash_count()
+        f.scan()
+        f.scatter()
+        for i in np.linspace(0, 19_999, 40).astype(int):
+            for walk in (1, 2):
+                got = f.neighbours(int(i), walk)
+                want = fr.neighbours_brute(w, int(i), walk)
+                r = float(w[i, 8]) * walk
+                for j in set(got.tolist()) ^ set(want.tolist()):  # only ties within 1e-5 of the radius
+                    dj = float(np.linalg.norm(w[j, :3].astype(np.float64) - w[i, :3].astype(np.float64)))
+                    assert abs(dj - r) < 1e-5, (i, walk, j)
+
+
+@pytest.mark.parametrize("flags,group", [(SPATIAL, 1), (SPATIAL | 4 | 64, 4), (SPATIAL | 4, 64)])
+def test_spatial_step_matches_the_oracle(fh, fr, flags, group):
+    w0 = cube(fh, 200_000, 5)
+    with fh.Flock.from_words(w0, flags=flags, decide_group=group) as f:
+        f.step(1)
+        gpu, t = f.download()
+    cpu, tc = fr.step_grid(w0, 0.0, 1, threads=THREADS)
+    d = fr.diff(cpu, gpu)
+    print("spatial 1-step", flags, group, d)
+    assert t == tc and d["nb_mismatch"] == 0
+    assert rel_err(gpu[:, 0:6], cpu[:, 0:6]) <= 1e-4
+
+
+def test_spatial_resident_grid_after_steps(fh):
+    w0 = cube(fh, 200_000, 6)
+    with fh.Flock.from_words(w0, cell=6.0, flags=SPATIAL | 64 | 4) as f:
+        f.step(17)
+        g = f.read_grid()
+        state, _ = f.download()
+        check_grid(fh, g, state, f.buckets, 6.0)
+
+
+def test_spatial_batch_equals_alone(fh):
+    starts = [cube(fh, n, 10 + n) for n in (3000, 40_000, 17_000)]
+    flags = SPATIAL | 1  # canonical
+    alone = []
+    for w in starts:
+        with fh.Flock.from_words(w, flags=flags) as f:
+            f.step(12)
+            alone.append(f.download()[0])
+    with fh.Batch(100_000, 1 << 21, 4, flags=flags) as b:
+        ids = [b.add(w) for w in starts]
+        b.step(12)
+        for i, a in zip(ids, alone):
+            got, _ = b.download(i)
+            assert np.array_equal(got.view(np.uint32), a.view(np.uint32))
