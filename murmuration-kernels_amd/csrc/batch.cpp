@@ -115,9 +115,11 @@ static fh_status batch_step_launches(fh_batch *b, hipEvent_t *ev) {
     b->counts_dirty = false;
     if (ev) FH_HIP(hipEventRecord(ev[2], b->stream));
     FH_HIP(launch_scatter(b->geo, b->P, (b->prm.flags & FH_FLAG_CANONICAL) != 0, true, b->stream));
-    // the decide launch records ev[3] and ev[4] at its kernel's start and end
+    if (ev) FH_HIP(hipEventRecord(ev[3], b->stream));
+    // the decide launch records ev[6] and ev[7] at its kernel's start and end
     FH_HIP(launch_decide(b->geo, b->P, 0.0f, b->prm.decide_block, b->prm.birds_per_thread, b->prm.decide_group,
-                         walk_flags(b->prm.flags), b->stream, ev ? DecideTiming{ev[3], ev[4]} : DecideTiming{}));
+                         walk_flags(b->prm.flags), b->stream, ev ? DecideTiming{ev[6], ev[7]} : DecideTiming{}));
+    if (ev) FH_HIP(hipEventRecord(ev[4], b->stream));
     FH_HIP(launch_integrate(b->geo, b->P, true, (b->prm.flags & FH_FLAG_WAVE_AGG) != 0, b->stream));
     b->counts_dirty = true;
     b->hashed = true;
@@ -162,7 +164,7 @@ fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *o
     const bool per_pass = out && !graph;
     std::vector<hipEvent_t> ev;
     if (out) {
-        ev.resize(per_pass ? (size_t)timed * 6 : 2);
+        ev.resize(per_pass ? (size_t)timed * 8 : 2);
         for (auto &x : ev) FH_HIP(hipEventCreate(&x));
     }
     auto destroy = [&] {
@@ -186,7 +188,7 @@ fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *o
             b->hashed = true;
             b->counts_dirty = true;
         } else {
-            st = batch_step_launches(b, (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 6] : nullptr);
+            st = batch_step_launches(b, (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 8] : nullptr);
         }
     }
     if (st) {
@@ -203,19 +205,26 @@ fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *o
         if (timed) {
             out->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
             if (per_pass) {
-                double acc[5] = {0, 0, 0, 0, 0};
-                for (uint32_t s = 0; s < timed; ++s)
+                // events per step: six between the passes, then decide's kernel start and end
+                double acc[5] = {0, 0, 0, 0, 0}, dec = 0.0, total = 0.0;
+                for (uint32_t s = 0; s < timed; ++s) {
+                    hipEvent_t *e = &ev[(size_t)s * 8];
+                    float ms = 0.0f;
                     for (int k = 0; k < 5; ++k) {
-                        float ms = 0.0f;
-                        FH_HIP(hipEventElapsedTime(&ms, ev[(size_t)s * 6 + k], ev[(size_t)s * 6 + k + 1]));
+                        FH_HIP(hipEventElapsedTime(&ms, e[k], e[k + 1]));
                         acc[k] += ms;
                     }
+                    FH_HIP(hipEventElapsedTime(&ms, e[6], e[7]));
+                    dec += ms;
+                    FH_HIP(hipEventElapsedTime(&ms, e[0], e[5]));
+                    total += ms;
+                }
                 out->grid_histogram = acc[0] / timed;
                 out->grid_scan = acc[1] / timed;
                 out->grid_scatter = acc[2] / timed;
-                out->decide = acc[3] / timed;
+                out->decide = dec / timed;
                 out->integrate = acc[4] / timed;
-                out->step_total = (acc[0] + acc[1] + acc[2] + acc[3] + acc[4]) / timed;
+                out->step_total = total / timed;
             } else {
                 float ms = 0.0f;
                 FH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));

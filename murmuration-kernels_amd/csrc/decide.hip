@@ -255,17 +255,19 @@ __device__ __forceinline__ void acc_zero(Acc &a) {
     a.n_all = a.n_star = a.n_align = a.n_g = 0;
 }
 
-// One candidate with offset d and d^2 <= 4r^2; `velocity()` gives its
-// velocity (read only for aligned neighbours within r).
-template <class Velocity>
-__device__ __forceinline__ void accumulate(Acc &a, f3 f_i, float r2, f3 d, float d2, Velocity &&velocity) {
-    const float dist = sqrtf(d2);
-    const f3 dhat = d / fmaxf(dist, 1e-6f);
-    // walk 2 (model.rs:104-115): csum += d / max(sqrt(d2), 1e-6)
+// walk 2 (model.rs:104-115) for a candidate with d^2 <= 4r^2: csum += d /
+// max(sqrt(d2), 1e-6); returns the unit offset and the distance.
+__device__ __forceinline__ void accumulate_2r(Acc &a, f3 d, float d2, f3 &dhat, float &dist) {
+    dist = sqrtf(d2);
+    dhat = d / fmaxf(dist, 1e-6f);
     a.csum = a.csum + dhat;
     ++a.n_g;
-    if (d2 > r2) return;
-    // walk 1 (model.rs:67-93)
+}
+
+// walk 1 (model.rs:67-93) for a candidate within r; `velocity()` gives its
+// velocity (read only for an aligned neighbour).
+template <class Velocity>
+__device__ __forceinline__ void accumulate_r(Acc &a, f3 f_i, f3 dhat, float dist, Velocity &&velocity) {
     ++a.n_all;
     const float g = dist <= R_H ? 1.0f : expf(-((dist - R_H) * (dist - R_H)) / SEP_DENOM);
     a.sep = a.sep - dhat * g;
@@ -278,6 +280,15 @@ __device__ __forceinline__ void accumulate(Acc &a, f3 f_i, float r2, f3 d, float
         a.align = a.align + forward_of(mk3(vj.x, vj.y, vj.z));
         ++a.n_align;
     }
+}
+
+// One candidate with offset d and d^2 <= 4r^2.
+template <class Velocity>
+__device__ __forceinline__ void accumulate(Acc &a, f3 f_i, float r2, f3 d, float d2, Velocity &&velocity) {
+    f3 dhat;
+    float dist;
+    accumulate_2r(a, d, d2, dhat, dist);
+    if (d2 <= r2) accumulate_r(a, f_i, dhat, dist, velocity);
 }
 
 template <int G, int PF, int STRIDE>
@@ -359,7 +370,10 @@ template <int BLOCK, int G, bool RES>
 __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, Records rec, float time, uint32_t chunk,
                                                   int walk_flags, uint32_t *due_clear) {
     const bool allow_tiled = (walk_flags & WALK_TILED) != 0, use_occ = (walk_flags & WALK_OCC) != 0;
-    constexpr int PF = G >= 16 ? 8 : (G >= 4 ? 4 : 8);
+    // Cells whose bucket ranges a lane loads at once. The split walks (G >= 16)
+    // take one cell at a time: their buckets are long, and the smaller code
+    // (the candidate loop is unrolled per cell) runs faster.
+    constexpr int PF = G >= 16 ? 1 : (G >= 4 ? 4 : 8);
     constexpr int STRIDE = RES ? 1 : 2;
     const Cand cand = RES ? Cand{rec.p4, rec.v4} : Cand{p.sbird, p.sbird + 1};
     extern __shared__ uint32_t due_slots[];

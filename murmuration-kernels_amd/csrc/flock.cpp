@@ -50,6 +50,10 @@ struct fh_flock {
     hipGraphExec_t rgraph[2]{};
 };
 
+// Events per timed step with per-pass timing: six on the stream between the
+// passes, two recorded by the decide launch at its kernel's start and end.
+constexpr size_t EV_PER_STEP = 8;
+
 static fh_status check_handle(const fh_flock *h, const char *fn) {
     if (!h) return set_error(FH_ERR_INVALID, "%s: NULL handle", fn);
     return FH_OK;
@@ -120,8 +124,10 @@ static fh_status step_launches(fh_flock *h, bool fused, hipEvent_t *ev) {
     if (st) return st;
     if (ev) FH_HIP(hipEventRecord(ev[2], h->stream));
     if ((st = do_scatter(h, true))) return st;
-    // the decide launch records ev[3] and ev[4] at its kernel's start and end
-    if ((st = do_decide(h, 0.0f, ev ? DecideTiming{ev[3], ev[4]} : DecideTiming{}))) return st;
+    if (ev) FH_HIP(hipEventRecord(ev[3], h->stream));
+    // the decide launch records ev[6] and ev[7] at its kernel's start and end
+    if ((st = do_decide(h, 0.0f, ev ? DecideTiming{ev[6], ev[7]} : DecideTiming{}))) return st;
+    if (ev) FH_HIP(hipEventRecord(ev[4], h->stream));
     if ((st = do_integrate(h, fused))) return st;
     if (ev) FH_HIP(hipEventRecord(ev[5], h->stream));
     return FH_OK;
@@ -237,17 +243,18 @@ static fh_status leave_resident(fh_flock *h) {
 
 // One resident step: decide over the current layout, banked flight (movers
 // counted), then the grid of the new positions and the records moved into it.
-// Events (per-pass timing): ev[0] | decide | ev[1] | integrate | ev[2] | scan
-// and move | ev[3] = ev[4] = ev[5].
+// Events (per-pass timing): ev[0] | decide (its kernel: ev[6] to ev[7]) |
+// ev[1] | integrate | ev[2] | scan and move | ev[3] = ev[4] = ev[5].
 static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
     const uint32_t q = h->parity;
     Pools p = h->P;
     p.offsets = h->off[q];
     p.due_count = h->res_due + q;
-    // the decide launch records ev[0] and ev[1] at its kernel's start and end
+    if (ev) FH_HIP(hipEventRecord(ev[0], h->stream));
     FH_HIP(launch_decide_resident(h->geo, p, h->R[q], h->prm.decide_block, h->prm.birds_per_thread,
                                   h->prm.decide_group, walk_flags(h->prm.flags), h->res_due + (1u - q), h->stream,
-                                  ev ? DecideTiming{ev[0], ev[1]} : DecideTiming{}));
+                                  ev ? DecideTiming{ev[6], ev[7]} : DecideTiming{}));
+    if (ev) FH_HIP(hipEventRecord(ev[1], h->stream));
     FH_HIP(launch_resident_integrate(h->geo, p, h->R[q], h->W, h->d_seg, h->stream));
     if (ev) FH_HIP(hipEventRecord(ev[2], h->stream));
     FH_HIP(launch_resident_rebuild(h->geo, p, h->R[q], h->R[1u - q], h->W, h->off[q], h->off[1u - q], h->stream));
@@ -556,7 +563,7 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
     std::vector<hipEvent_t> ev;
     const bool per_pass = out && !graph;
     if (out) {
-        ev.resize(per_pass ? (size_t)timed * 6 : 2);
+        ev.resize(per_pass ? (size_t)timed * EV_PER_STEP : 2);
         for (auto &x : ev) FH_HIP(hipEventCreate(&x));
     }
     auto destroy_events = [&] {
@@ -579,7 +586,7 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
                     break;
                 }
             } else {
-                hipEvent_t *e6 = (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 6] : nullptr;
+                hipEvent_t *e6 = (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * EV_PER_STEP] : nullptr;
                 if ((st = resident_step_launches(h, e6))) break;
             }
             h->parity ^= 1u;
@@ -597,7 +604,7 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
             h->counts_dirty = true;
             h->absolute_due = false;
         } else {
-            hipEvent_t *e6 = (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 6] : nullptr;
+            hipEvent_t *e6 = (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * EV_PER_STEP] : nullptr;
             if ((st = step_launches(h, fused, e6))) break;
         }
     }
@@ -615,27 +622,33 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
         if (timed) {
             out->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
             if (per_pass) {
-                double acc[5] = {0, 0, 0, 0, 0};
-                for (uint32_t s = 0; s < timed; ++s)
+                // iv[k]: ev[k] -> ev[k+1] for k < 5; iv[5]: decide's kernel; iv[6]: the step
+                double iv[7] = {0, 0, 0, 0, 0, 0, 0};
+                for (uint32_t s = 0; s < timed; ++s) {
+                    hipEvent_t *e = &ev[(size_t)s * EV_PER_STEP];
+                    float ms = 0.0f;
                     for (int k = 0; k < 5; ++k) {
-                        float ms = 0.0f;
-                        FH_HIP(hipEventElapsedTime(&ms, ev[(size_t)s * 6 + k], ev[(size_t)s * 6 + k + 1]));
-                        acc[k] += ms;
+                        FH_HIP(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+                        iv[k] += ms;
                     }
+                    FH_HIP(hipEventElapsedTime(&ms, e[6], e[7]));
+                    iv[5] += ms;
+                    FH_HIP(hipEventElapsedTime(&ms, e[0], e[5]));
+                    iv[6] += ms;
+                }
+                out->decide = iv[5] / timed;
                 if (res) {  // decide, integrate, then the rebuild (scan and move, reported as scatter)
                     out->grid_histogram = 0.0;
                     out->grid_scan = 0.0;
-                    out->grid_scatter = acc[2] / timed;
-                    out->decide = acc[0] / timed;
-                    out->integrate = acc[1] / timed;
+                    out->grid_scatter = iv[2] / timed;
+                    out->integrate = iv[1] / timed;
                 } else {
-                    out->grid_histogram = acc[0] / timed;
-                    out->grid_scan = acc[1] / timed;
-                    out->grid_scatter = acc[2] / timed;
-                    out->decide = acc[3] / timed;
-                    out->integrate = acc[4] / timed;
+                    out->grid_histogram = iv[0] / timed;
+                    out->grid_scan = iv[1] / timed;
+                    out->grid_scatter = iv[2] / timed;
+                    out->integrate = iv[4] / timed;
                 }
-                out->step_total = (acc[0] + acc[1] + acc[2] + acc[3] + acc[4]) / timed;
+                out->step_total = iv[6] / timed;
             } else {
                 float ms = 0.0f;
                 FH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));

@@ -117,3 +117,47 @@ def test_resident_rejects_canonical(fh):
     with pytest.raises(fh.FlockHipError) as e:
         fh.Flock(1000, flags=fh.FLAG_RESIDENT | fh.FLAG_CANONICAL)
     assert e.value.status == fh.ERR_INVALID
+
+
+@pytest.mark.parametrize("n", [1, 2, 257])
+def test_resident_tiny_flocks(fh, fr, n):
+    w0 = fh.gen_reference(n, 4)
+    with fh.Flock.from_words(w0, flags=fh.FLAG_RESIDENT | fh.FLAG_GRAPH) as f:
+        f.step(30)
+        gpu, t = f.download()
+    cpu, tc = fr.step(w0, 0.0, 30)
+    assert t == tc
+    assert rel_err(gpu[:, 0:6], cpu[:, 0:6]) <= 1e-3
+
+
+def test_resident_dense_single_cell(fh, fr):
+    # 5000 birds inside one 4 m cell (one bucket of 5000) plus a spread: every
+    # stayer of the dense bucket counts the movers before it.
+    rng = np.random.default_rng(8)
+    w0 = fh.gen_reference(8000, 5)
+    w0[:5000, 0:3] = (rng.random((5000, 3)) * 3.9 + np.array([0.05, 0.05, 100.05])).astype(np.float32)
+    w0[:, 8] = np.float32(0.5)
+    with fh.Flock.from_words(w0, flags=fh.FLAG_RESIDENT) as f:
+        for k in range(4):
+            f.step(5)
+            g = f.read_grid()
+            state, _ = f.download()
+            keys, counts, offsets, srt = fr.grid(state, f.buckets)
+            assert np.array_equal(g.keys, keys) and np.array_equal(g.offsets, offsets)
+            assert per_bucket_sets_equal(g.sorted, offsets, srt)
+
+
+def test_resident_upload_replaces_the_records(fh):
+    # An upload while resident drops the resident records; the next steps
+    # start from the uploaded state.
+    a = cube(fh, 20_000, 11)
+    b = cube(fh, 20_000, 12)
+    with fh.Flock.from_words(a, flags=fh.FLAG_RESIDENT) as f:
+        f.step(5)
+        f.upload(b, 0.0)
+        f.step(5)
+        got, _ = f.download()
+    with fh.Flock.from_words(b, flags=fh.FLAG_RESIDENT) as f:
+        f.step(5)
+        want, _ = f.download()
+    assert rel_err(got[:, 0:6], want[:, 0:6]) <= 1e-5
