@@ -99,6 +99,11 @@ typedef struct fh_params {
  * cells get nearby buckets and slots, so decide's walks share cache lines.
  * Neighbour sets are unchanged; keys differ from the reference's. */
 #define FH_FLAG_SPATIAL 128u
+/* decide runs as a persistent kernel: a grid of as many workgroups as the
+ * GPU holds at once, each taking chunks of slots from a device counter until
+ * none are left (instead of one workgroup per chunk). Results are
+ * bit-identical; the autotuner decides whether it pays. */
+#define FH_FLAG_PERSISTENT 256u
 
 /* Per-pass GPU time of fh_step, mean ms per timed step (hipEvents on the
  * handle's stream), in the rows of the reference report

@@ -317,6 +317,7 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
     b->P.tile_prefix = reinterpret_cast<uint32_t *>(base + o_tpre);
     b->P.err = reinterpret_cast<uint32_t *>(base + o_err);
     b->P.due_count = reinterpret_cast<uint32_t *>(base + o_due);
+    b->P.work = b->P.due_count + 1;
     b->P.big_count = reinterpret_cast<uint32_t *>(base + o_bigc);
     b->P.big = reinterpret_cast<uint32_t *>(base + o_big);
     b->P.big_cap = big_list_cap((uint32_t)nn);

@@ -15,7 +15,7 @@
 static int usage() {
     std::fprintf(stderr,
                  "usage: flock_hip --in FILE [--steps K] [--warmup W] [--out FILE]\n"
-                 "       [--canonical] [--unfused] [--graph] [--resident] [--occ]\n"
+                 "       [--canonical] [--unfused] [--graph] [--resident] [--occ] [--spatial] [--persistent]\n"
                  "       [--cell M] [--block B] [--bpt K] [--group G] [--shift S]\n"
                  "(the first W steps are not timed)\n");
     return 2;
@@ -54,6 +54,8 @@ int main(int argc, char **argv) {
         else if (a == "--shift") prm.bucket_shift = (uint32_t)std::atol(val());
         else if (a == "--occ") prm.flags |= FH_FLAG_OCC;
         else if (a == "--resident") prm.flags |= FH_FLAG_RESIDENT;
+        else if (a == "--spatial") prm.flags |= FH_FLAG_SPATIAL;
+        else if (a == "--persistent") prm.flags |= FH_FLAG_PERSISTENT;
         else return usage();
     }
     if (!in) {

@@ -366,10 +366,16 @@ constexpr uint32_t NO_BIRD = 0xffffffffu;
 // coalesced loads and every lane that needs the cell reads them from LDS
 // (broadcast). A lane sees the same (cell, slot) sequence as in its own walk,
 // so its sums are bit-identical to the per-lane path.
+//
+// Persistent form (WALK_PERSIST): a grid sized to the machine rather than to
+// the flock; each workgroup takes chunks from *p.work until they run out, so
+// a workgroup that drew a sparse chunk moves on at once and the tail of the
+// launch is the last chunk's walk, not the slowest workgroup's share.
 template <int BLOCK, int G, bool RES>
 __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, Records rec, float time, uint32_t chunk,
-                                                  int walk_flags, uint32_t *due_clear) {
+                                                  int walk_flags, uint32_t *due_clear, uint32_t *work_clear) {
     const bool allow_tiled = (walk_flags & WALK_TILED) != 0, use_occ = (walk_flags & WALK_OCC) != 0;
+    const bool persist = (walk_flags & WALK_PERSIST) != 0;
     // Cells whose bucket ranges a lane loads at once. The split walks (G >= 16)
     // take one cell at a time: their buckets are long, and the smaller code
     // (the candidate loop is unrolled per cell) runs faster.
@@ -381,10 +387,21 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, Records r
     __shared__ int32_t u_lo[3], u_hi[3];
     __shared__ uint32_t u_seg_lo, u_seg_hi, u_vol, u_any;
     __shared__ float4 tile[BLOCK];
-    if (threadIdx.x == 0) n_due = 0;
-    if (RES && due_clear && blockIdx.x == 0 && threadIdx.x == 0) *due_clear = 0;
+    __shared__ uint32_t cur_chunk;
+    if (RES && blockIdx.x == 0 && threadIdx.x == 0) {
+        if (due_clear) *due_clear = 0;
+        if (work_clear) *work_clear = 0;
+    }
+    const uint32_t n_chunks = (g.n_slots + chunk - 1) / chunk;
+    for (uint32_t round = 0;; ++round) {  // one chunk per trip; every lane leaves together
+    if (threadIdx.x == 0) {
+        n_due = 0;
+        cur_chunk = persist ? atomicAdd(p.work, 1u) : (round ? n_chunks : blockIdx.x);
+    }
     __syncthreads();
-    const uint32_t start = blockIdx.x * chunk;
+    const uint32_t c_idx = cur_chunk;
+    if (c_idx >= n_chunks) break;
+    const uint32_t start = c_idx * chunk;
     const uint32_t end = min(start + chunk, g.n_slots);
     // Sorted copies: time is relative (0) and snext holds next - time. Resident
     // records: this step's clock is the segment clock + DT (committed by the
@@ -514,6 +531,8 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, Records r
             p.cen[me] = d.centrality;
         }
     }
+    __syncthreads();  // due_slots and n_due are reused by the next chunk
+    }
 }
 
 // Debug walk for one bird (sorted copies): the ids the grid walk finds within
@@ -540,48 +559,89 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
     *count = c;
 }
 
+// Workgroups of one persistent decide launch: as many as the chip holds at
+// once (occupancy of this instance at this LDS size, times the CUs), and no
+// more than there are chunks. The occupancy is a property of the compiled
+// kernel; it is looked up once per instance and chunk size.
+template <int BLOCK, int G, bool RES>
+static uint32_t persistent_blocks(uint32_t chunk, uint32_t n_chunks) {
+    static int per_cu[5] = {0, 0, 0, 0, 0};  // by log2(chunk / BLOCK): 1..16 birds per thread
+    const uint32_t bpt = chunk / BLOCK;
+    const int idx = bpt <= 1 ? 0 : bpt <= 2 ? 1 : bpt <= 4 ? 2 : bpt <= 8 ? 3 : 4;
+    int occ = __atomic_load_n(&per_cu[idx], __ATOMIC_RELAXED);
+    if (occ <= 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_decide<BLOCK, G, RES>, BLOCK,
+                                                         chunk * sizeof(uint32_t)) != hipSuccess ||
+            occ <= 0)
+            occ = 1;
+        __atomic_store_n(&per_cu[idx], occ, __ATOMIC_RELAXED);
+    }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    return min(n_chunks, (uint32_t)occ * (uint32_t)cus);
+}
+
 // With timing events, the launch itself records the kernel's start and end
 // (hipExtLaunchKernelGGL), so the measured interval is the kernel alone.
 template <int BLOCK, int G, bool RES>
 static void launch_decide_block(const Geometry &g, const Pools &p, const Records &rec, float time, uint32_t chunk,
-                                int walk_flags, uint32_t *due_clear, const DecideTiming &tm, hipStream_t s) {
-    const uint32_t blocks = (g.n_slots + chunk - 1) / chunk;
+                                int walk_flags, uint32_t *due_clear, uint32_t *work_clear, const DecideTiming &tm,
+                                hipStream_t s) {
+    const uint32_t n_chunks = (g.n_slots + chunk - 1) / chunk;
+    const uint32_t blocks =
+        (walk_flags & WALK_PERSIST) ? persistent_blocks<BLOCK, G, RES>(chunk, n_chunks) : n_chunks;
     if (tm.start)
         hipExtLaunchKernelGGL((k_decide<BLOCK, G, RES>), dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s,
-                              tm.start, tm.stop, 0u, g, p, rec, time, chunk, walk_flags, due_clear);
+                              tm.start, tm.stop, 0u, g, p, rec, time, chunk, walk_flags, due_clear, work_clear);
     else
         hipLaunchKernelGGL((k_decide<BLOCK, G, RES>), dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s, g, p,
-                           rec, time, chunk, walk_flags, due_clear);
+                           rec, time, chunk, walk_flags, due_clear, work_clear);
 }
 
 template <int BLOCK, bool RES>
 static hipError_t launch_decide_group(const Geometry &g, const Pools &p, const Records &rec, float time,
                                       uint32_t chunk, uint32_t group, int walk_flags, uint32_t *due_clear,
-                                      const DecideTiming &tm, hipStream_t s) {
+                                      uint32_t *work_clear, const DecideTiming &tm, hipStream_t s) {
+#define FH_DECIDE_G(GV)                                                                                    \
+    case GV:                                                                                               \
+        launch_decide_block<BLOCK, GV, RES>(g, p, rec, time, chunk, walk_flags, due_clear, work_clear, tm, s); \
+        break;
     switch (group) {
-        case 1: launch_decide_block<BLOCK, 1, RES>(g, p, rec, time, chunk, walk_flags, due_clear, tm, s); break;
-        case 2: launch_decide_block<BLOCK, 2, RES>(g, p, rec, time, chunk, walk_flags, due_clear, tm, s); break;
-        case 4: launch_decide_block<BLOCK, 4, RES>(g, p, rec, time, chunk, walk_flags, due_clear, tm, s); break;
-        case 8: launch_decide_block<BLOCK, 8, RES>(g, p, rec, time, chunk, walk_flags, due_clear, tm, s); break;
-        case 16: launch_decide_block<BLOCK, 16, RES>(g, p, rec, time, chunk, walk_flags, due_clear, tm, s); break;
-        case 64: launch_decide_block<BLOCK, 64, RES>(g, p, rec, time, chunk, walk_flags, due_clear, tm, s); break;
+        FH_DECIDE_G(1)
+        FH_DECIDE_G(2)
+        FH_DECIDE_G(4)
+        FH_DECIDE_G(8)
+        FH_DECIDE_G(16)
+        FH_DECIDE_G(64)
         default: return hipErrorInvalidValue;
     }
+#undef FH_DECIDE_G
     return hipSuccess;
 }
 
 template <bool RES>
 static hipError_t launch_decide_any(const Geometry &g, const Pools &p, const Records &rec, float time,
                                     uint32_t block, uint32_t birds_per_thread, uint32_t group, int walk_flags,
-                                    uint32_t *due_clear, const DecideTiming &tm, hipStream_t s) {
+                                    uint32_t *due_clear, uint32_t *work_clear, const DecideTiming &tm,
+                                    hipStream_t s) {
     if (g.n_slots == 0) return hipSuccess;
     const uint32_t chunk = block * birds_per_thread;
     hipError_t e;
     switch (block) {
-        case 64: e = launch_decide_group<64, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, tm, s); break;
-        case 128: e = launch_decide_group<128, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, tm, s); break;
-        case 256: e = launch_decide_group<256, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, tm, s); break;
-        case 512: e = launch_decide_group<512, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, tm, s); break;
+        case 64:
+            e = launch_decide_group<64, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, work_clear, tm, s);
+            break;
+        case 128:
+            e = launch_decide_group<128, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, work_clear, tm, s);
+            break;
+        case 256:
+            e = launch_decide_group<256, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, work_clear, tm, s);
+            break;
+        case 512:
+            e = launch_decide_group<512, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, work_clear, tm, s);
+            break;
         default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
@@ -591,13 +651,15 @@ static hipError_t launch_decide_any(const Geometry &g, const Pools &p, const Rec
 hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
                          uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s,
                          const DecideTiming &tm) {
-    return launch_decide_any<false>(g, p, Records{}, time, block, birds_per_thread, group, walk_flags, nullptr, tm, s);
+    return launch_decide_any<false>(g, p, Records{}, time, block, birds_per_thread, group, walk_flags, nullptr,
+                                    nullptr, tm, s);
 }
 
 hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Records &rec, uint32_t block,
                                   uint32_t birds_per_thread, uint32_t group, int walk_flags, uint32_t *due_clear,
-                                  hipStream_t s, const DecideTiming &tm) {
-    return launch_decide_any<true>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear, tm, s);
+                                  uint32_t *work_clear, hipStream_t s, const DecideTiming &tm) {
+    return launch_decide_any<true>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear,
+                                   work_clear, tm, s);
 }
 
 hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, int walk, uint32_t *ids,

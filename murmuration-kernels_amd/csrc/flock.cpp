@@ -44,7 +44,7 @@ struct fh_flock {
     Records R[2]{};
     ResWork W{};
     uint32_t *off[2]{};
-    uint32_t *res_due = nullptr;  // due counter of each parity
+    uint32_t *res_due = nullptr;  // due counter of each parity, then the persistent decide's chunk counters
     bool resident = false;
     uint32_t parity = 0;
     hipGraphExec_t rgraph[2]{};
@@ -183,7 +183,7 @@ static fh_status alloc_resident(fh_flock *h) {
     const size_t o_knew = a.take<uint32_t>(nn), o_arank = a.take<uint32_t>(nn);
     const size_t o_mbits = a.take<uint32_t>((nn + TILE - 1) / TILE * (TILE / 32));
     const size_t o_arr = a.take<uint32_t>(nb), o_dep = a.take<uint32_t>(nb), o_abase = a.take<uint32_t>(nb);
-    const size_t o_off1 = a.take<uint32_t>(nb + 1), o_due = a.take<uint32_t>(2);
+    const size_t o_off1 = a.take<uint32_t>(nb + 1), o_due = a.take<uint32_t>(4);
     hipError_t e = hipMalloc(&h->res_arena, a.used);
     if (e != hipSuccess) {
         h->res_arena = nullptr;
@@ -224,6 +224,8 @@ static fh_status enter_resident(fh_flock *h) {
     p.offsets = h->off[0];
     FH_HIP(launch_scan(h->geo, p, nullptr, 0u, 0.0f, h->stream, true));
     FH_HIP(launch_resident_enter(h->geo, p, h->R[0], h->stream));
+    // parity 0's counters may hold a step taken before the last leave
+    FH_HIP(launch_fill(h->res_due, 4, 0u, h->stream));
     h->parity = 0;
     h->resident = true;
     h->counts_dirty = false;  // the counts now describe the current layout
@@ -250,9 +252,11 @@ static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
     Pools p = h->P;
     p.offsets = h->off[q];
     p.due_count = h->res_due + q;
+    p.work = h->res_due + 2 + q;
     if (ev) FH_HIP(hipEventRecord(ev[0], h->stream));
     FH_HIP(launch_decide_resident(h->geo, p, h->R[q], h->prm.decide_block, h->prm.birds_per_thread,
-                                  h->prm.decide_group, walk_flags(h->prm.flags), h->res_due + (1u - q), h->stream,
+                                  h->prm.decide_group, walk_flags(h->prm.flags), h->res_due + (1u - q),
+                                  h->res_due + 2 + (1u - q), h->stream,
                                   ev ? DecideTiming{ev[6], ev[7]} : DecideTiming{}));
     if (ev) FH_HIP(hipEventRecord(ev[1], h->stream));
     FH_HIP(launch_resident_integrate(h->geo, p, h->R[q], h->W, h->d_seg, h->stream));
@@ -344,6 +348,7 @@ fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
     h->P.tile_prefix = reinterpret_cast<uint32_t *>(base + o_tpre);
     h->P.err = reinterpret_cast<uint32_t *>(base + o_err);
     h->P.due_count = reinterpret_cast<uint32_t *>(base + o_due);
+    h->P.work = h->P.due_count + 1;
     h->P.big_count = reinterpret_cast<uint32_t *>(base + o_bigc);
     h->P.big = reinterpret_cast<uint32_t *>(base + o_big);
     h->P.big_cap = big_list_cap((uint32_t)nn);

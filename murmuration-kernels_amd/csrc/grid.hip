@@ -123,10 +123,11 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *__restrict_
                                                      uint32_t *__restrict__ tile_prefix, uint32_t ntiles,
                                                      uint32_t *__restrict__ offsets_end,
                                                      uint32_t *__restrict__ due_count,
-                                                     uint32_t *__restrict__ big_count) {
+                                                     uint32_t *__restrict__ big_count, uint32_t *__restrict__ work) {
     if (threadIdx.x == 0) {  // a new grid starts a new step
         *due_count = 0;
         *big_count = 0;
+        *work = 0;
     }
     uint32_t carry = 0;
     for (uint32_t base = 0; base < ntiles; base += 4096u) {
@@ -163,7 +164,8 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *__restrict__ count
                                                     const uint32_t *__restrict__ tile_in, uint32_t ntiles,
                                                     uint32_t *__restrict__ offsets,
                                                     uint32_t *__restrict__ occ, uint32_t *__restrict__ due_count,
-                                                    uint32_t *__restrict__ big_count, int keep_counts) {
+                                                    uint32_t *__restrict__ big_count, uint32_t *__restrict__ work,
+                                                    int keep_counts) {
     uint32_t prefix;
     if (MERGED) {
         uint32_t part = 0;
@@ -172,6 +174,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *__restrict__ count
         if (blockIdx.x == 0 && threadIdx.x == 0) {  // a new grid starts a new step
             *due_count = 0;
             *big_count = 0;
+            *work = 0;
         }
     } else {
         prefix = tile_in[blockIdx.x];
@@ -382,12 +385,12 @@ hipError_t launch_scan(const Geometry &g, const Pools &p, Seg *tick_segs, uint32
     hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_sums, tick_segs, tick_nseg, dt);
     if (ntiles <= SCAN_MERGE_TILES) {
         hipLaunchKernelGGL(k_scan_apply<true>, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_sums, ntiles,
-                           p.offsets, p.occ, p.due_count, p.big_count, keep);
+                           p.offsets, p.occ, p.due_count, p.big_count, p.work, keep);
     } else {
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, p.tile_sums, p.tile_prefix, ntiles,
-                           p.offsets + (size_t)ntiles * SCAN_TILE, p.due_count, p.big_count);
+                           p.offsets + (size_t)ntiles * SCAN_TILE, p.due_count, p.big_count, p.work);
         hipLaunchKernelGGL(k_scan_apply<false>, dim3(ntiles), dim3(256), 0, s, p.counts, p.tile_prefix, ntiles,
-                           p.offsets, p.occ, p.due_count, p.big_count, keep);
+                           p.offsets, p.occ, p.due_count, p.big_count, p.work, keep);
     }
     return hipGetLastError();
 }

@@ -80,6 +80,7 @@ struct Pools {
     uint32_t *sorted_tmp;  // canonical mode: slot -> bird before reordering
     uint32_t *err;         // device error word (ERR_* bits)
     uint32_t *due_count;   // birds that decided in the current step (zeroed by the scan)
+    uint32_t *work;        // persistent decide: next chunk to take (zeroed by the scan)
     uint32_t *big;         // canonical mode: buckets with more than CANON_CAP birds
     uint32_t *big_count;   // entries of `big` this step (zeroed by the scan)
     uint32_t big_cap;      // capacity of `big`: big_list_cap(bird capacity)
@@ -142,6 +143,7 @@ hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, boo
 // Walk switches of decide (bit set = enabled).
 constexpr int WALK_TILED = 1;  // dense neighbourhoods through LDS
 constexpr int WALK_OCC = 2;    // skip empty buckets by the occupancy bitmap
+constexpr int WALK_PERSIST = 4;  // a fixed grid of workgroups takes chunks from a counter
 // Events the decide launch records itself (kernel start and end), or none.
 struct DecideTiming {
     hipEvent_t start = nullptr, stop = nullptr;
@@ -150,10 +152,11 @@ hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t
                          uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s,
                          const DecideTiming &tm = DecideTiming{});
 // decide over resident records (p.offsets: the current offsets). Adds the
-// due birds to *p.due_count and zeroes *due_clear (the other step's counter).
+// due birds to *p.due_count, takes chunks from *p.work (persistent), and
+// zeroes *due_clear and *work_clear (the other step's counters).
 hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Records &rec, uint32_t block,
                                   uint32_t birds_per_thread, uint32_t group, int walk_flags, uint32_t *due_clear,
-                                  hipStream_t s, const DecideTiming &tm = DecideTiming{});
+                                  uint32_t *work_clear, hipStream_t s, const DecideTiming &tm = DecideTiming{});
 // Resident passes (one flock): records from the bird-order state and a grid
 // built by hash + scan (counts kept); banked flight in slot order with the
 // movers counted (and the flock clock committed: += DT); the rebuild of the

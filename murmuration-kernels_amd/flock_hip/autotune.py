@@ -22,13 +22,15 @@ import time
 
 import numpy as np
 
-from . import FLAG_CANONICAL, FLAG_GRAPH, FLAG_OCC, FLAG_RESIDENT, FLAG_SPATIAL, Batch, Flock
+from . import FLAG_CANONICAL, FLAG_GRAPH, FLAG_OCC, FLAG_PERSISTENT, FLAG_RESIDENT, FLAG_SPATIAL, Batch, Flock
 
 BPT = (1, 2, 4, 8, 16)
 BLOCKS = (64, 128, 256, 512)
 GROUPS = (1, 2, 4, 8, 16, 64)
 CELLS = (4.0, 3.0, 5.0, 6.0, 8.0, 10.0, 12.0)
 SHIFTS = (0, 1, 2, 3, 4)
+# on/off tunables and the flag each one sets
+SWITCHES = {"occ": FLAG_OCC, "resident": FLAG_RESIDENT, "spatial": FLAG_SPATIAL, "persist": FLAG_PERSISTENT}
 
 
 def _signature(kind: str, sizes, steps: int, flags: int) -> str:
@@ -80,7 +82,7 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
     """Returns {"params": {...}, "flags": f, "ms_per_step": x, "trials": [...]}
     for the batch of `starts` (kind "batch") or the single flock starts[0]
     ("plain"). `flags` is the caller's flags plus the switches the search
-    turned on (FLAG_OCC, FLAG_RESIDENT, FLAG_SPATIAL); `spatial=(0,)` keeps
+    turned on (FLAG_OCC, FLAG_RESIDENT, FLAG_SPATIAL, FLAG_PERSISTENT); `spatial=(0,)` keeps
     the reference hash. With `thorough` the launch shape is searched
     exhaustively (group x birds_per_thread x block) before and after the
     cell and bucket searches; otherwise one coordinate sweep per tunable.
@@ -96,14 +98,15 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
 
     def timer(prm):
         prm = dict(prm)
-        f = flags | (FLAG_OCC if prm.pop("occ") else 0) | (FLAG_RESIDENT if prm.pop("resident") else 0)
-        f |= FLAG_SPATIAL if prm.pop("spatial") else 0
+        f = flags
+        for name, bit in SWITCHES.items():
+            f |= bit if prm.pop(name) else 0
         if kind == "batch":
             return _time_batch(starts, times, prm, f, warmup, steps)
         return _time_plain(starts[0], times[0], prm, f, warmup, steps)
 
     best = {"cell": float(cells[0]), "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0,
-            "occ": 0, "resident": 0, "spatial": 0}
+            "occ": 0, "resident": 0, "spatial": 0, "persist": 0}
     trials = []
     seen = {}
 
@@ -141,15 +144,16 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
     # the resident layout is a single-flock fh_step mode, not for canonical order
     resident = (0, 1) if kind == "plain" and not (flags & FLAG_CANONICAL) else (0,)
     for nv in (("resident", resident), ("spatial", spatial), ("cell", cells), ("bucket_shift", shifts),
-               ("occ", (0, 1)), ("cell", cells)):
+               ("occ", (0, 1)), ("persist", (0, 1)), ("cell", cells)):
         sweep([nv])
     if thorough:
         sweep(shape[:2])
     else:
         sweep([shape[0]])
-    params = {k: v for k, v in best.items() if k not in ("occ", "resident", "spatial")}
-    out_flags = flags | (FLAG_OCC if best["occ"] else 0) | (FLAG_RESIDENT if best["resident"] else 0)
-    out_flags |= FLAG_SPATIAL if best["spatial"] else 0
+    params = {k: v for k, v in best.items() if k not in SWITCHES}
+    out_flags = flags
+    for name, bit in SWITCHES.items():
+        out_flags |= bit if best[name] else 0
     out = {"params": params, "flags": out_flags, "ms_per_step": best_t * 1e3,
            "trials": trials, "signature": key}
     if cache_path:

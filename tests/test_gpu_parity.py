@@ -160,7 +160,7 @@ def test_neighbour_sets_match_brute_force(fh, fr, n, radius):
                 assert ties_ok(w, int(i), walk, got, want), (i, walk)
 
 
-@pytest.mark.parametrize("flags", [0, 1, 32])  # arrival order, canonical, occupancy skip
+@pytest.mark.parametrize("flags", [0, 1, 32, 256])  # arrival order, canonical, occupancy skip, persistent
 def test_decide_matches_brute_force(fh, fr, flags):
     w = fh.gen_reference(4000, 6)
     t = np.float32(0.0) + DT
@@ -322,6 +322,33 @@ def test_occupancy_skip_is_bit_identical(fh, n, cell, group, shift):
             f.step(12)
             outs.append(f.download()[0])
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("n,group,block,bpt,graph", [(200_000, 1, 256, 8, 0), (1_000_000, 4, 512, 1, 4),
+                                                    (100_000, 16, 128, 2, 4), (3000, 64, 64, 1, 0)])
+def test_persistent_decide_is_bit_identical(fh, n, group, block, bpt, graph):
+    # The persistent grid takes chunks from a counter the scan zeroes each
+    # step; every chunk must be decided exactly once, in every step.
+    w0 = cube(fh, n, 8)
+    outs = []
+    for flags in (fh.FLAG_CANONICAL | fh.FLAG_PERSISTENT | graph, fh.FLAG_CANONICAL):
+        with fh.Flock.from_words(w0, flags=flags, decide_group=group, decide_block=block,
+                                 birds_per_thread=bpt) as f:
+            f.step(9)
+            outs.append(f.download()[0])
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+def test_persistent_decide_in_a_batch(fh):
+    starts = [cube(fh, n, 20 + n) for n in (5000, 60_000, 1)]
+    res = []
+    for flags in (fh.FLAG_CANONICAL | fh.FLAG_PERSISTENT, fh.FLAG_CANONICAL):
+        with fh.Batch(100_000, 1 << 21, 4, flags=flags, birds_per_thread=2) as b:
+            ids = [b.add(w) for w in starts]
+            b.step(7)
+            res.append([b.download(i)[0] for i in ids])
+    for a, b in zip(*res):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 def test_graph_and_unfused_replays_match_eager(fh):

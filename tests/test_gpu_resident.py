@@ -57,7 +57,8 @@ def test_resident_grid_is_the_reference_grid(fh, fr, n, kind, cell):
             assert per_bucket_sets_equal(g.sorted, offsets, srt)
 
 
-@pytest.mark.parametrize("flags,group", [(64, 1), (64 | 4, 4), (64 | 4 | 32, 2), (64 | 4, 16), (64 | 4, 64)])
+@pytest.mark.parametrize("flags,group", [(64, 1), (64 | 4, 4), (64 | 4 | 32, 2), (64 | 4, 16), (64 | 4, 64),
+                                         (64 | 4 | 256, 4), (64 | 256, 1)])
 def test_resident_step_matches_the_oracle(fh, fr, flags, group):
     w0 = cube(fh, 200_000, 5)
     with fh.Flock.from_words(w0, flags=flags, decide_group=group) as f:
@@ -108,9 +109,13 @@ def test_resident_due_count_and_timing(fh):
     w0 = cube(fh, 100_000, 9)
     with fh.Flock.from_words(w0, flags=fh.FLAG_RESIDENT) as f:
         ms = f.step(4, 1, timing=True)
-    assert ms["timed_steps"] == 3
-    assert ms["decide"] > 0 and ms["integrate"] > 0 and ms["grid_scatter"] > 0
-    assert 5_000 < ms["due_last"] < 15_000  # about a tenth of the birds decide each step
+        assert ms["timed_steps"] == 3
+        assert ms["decide"] > 0 and ms["integrate"] > 0 and ms["grid_scatter"] > 0
+        assert 5_000 < ms["due_last"] < 15_000  # about a tenth of the birds decide each step
+        for _ in range(3):  # leaving and entering again restarts the counters
+            f.download()
+            ms = f.step(1, 0, timing=True)
+            assert 5_000 < ms["due_last"] < 15_000
 
 
 def test_resident_rejects_canonical(fh):
@@ -145,6 +150,24 @@ def test_resident_dense_single_cell(fh, fr):
             keys, counts, offsets, srt = fr.grid(state, f.buckets)
             assert np.array_equal(g.keys, keys) and np.array_equal(g.offsets, offsets)
             assert per_bucket_sets_equal(g.sorted, offsets, srt)
+
+
+def test_resident_persistent_after_leaving_at_odd_parity(fh):
+    # Leaving the resident layout after an odd number of steps and entering
+    # it again restarts at parity 0: its chunk counter must be zero again, or
+    # the persistent decide would find every chunk taken.
+    w0 = cube(fh, 100_000, 13)
+    outs = []
+    for flags in (fh.FLAG_RESIDENT | fh.FLAG_PERSISTENT | fh.FLAG_GRAPH, fh.FLAG_RESIDENT | fh.FLAG_GRAPH):
+        with fh.Flock.from_words(w0, flags=flags) as f:
+            f.step(3)
+            f.download()
+            f.step(4)
+            outs.append(f.download()[0])
+    # arrivals take their slots in atomic order, so sums may round apart; the
+    # decision clocks (advanced only by decide) must agree exactly
+    assert np.array_equal(outs[0][:, 12], outs[1][:, 12])
+    assert rel_err(outs[0][:, 0:6], outs[1][:, 0:6]) <= 1e-5
 
 
 def test_resident_upload_replaces_the_records(fh):
