@@ -32,12 +32,18 @@ def log(msg: str) -> None:
     print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
-def step_bytes(n: int, buckets: int, due: int) -> dict:
-    """Algorithmic HBM bytes per pass (FP32 SoA; n birds, B buckets, d due)."""
+def step_bytes(n: int, buckets: int, due: int, resident: bool) -> dict:
+    """Algorithmic HBM bytes per pass (n birds, B buckets, d due), keyed like
+    the per-pass times: the standard step (histogram, scan, scatter of the
+    sorted copies, decide, integrate) or the resident one (decide, integrate
+    counting the movers, then scan and move as one 'grid_scatter' pass)."""
+    if resident:
+        return {"decide": 28 * n + 4 * buckets + 32 * due, "integrate": 72 * n,
+                "grid_scatter": 16 * buckets + 120 * n}
     return {
-        "hash_count": 16 * n + 8 * buckets,
-        "scan": 8 * buckets,
-        "scatter": 56 * n + 4 * buckets,
+        "grid_histogram": 16 * n + 8 * buckets,
+        "grid_scan": 8 * buckets,
+        "grid_scatter": 56 * n + 4 * buckets,
         "decide": 28 * n + 4 * buckets + 32 * due,
         "integrate": 72 * n,
     }
@@ -59,6 +65,11 @@ def profiled_traffic(params: dict, flags: int):
                 if got == want:
                     return {"bytes": float(fields["bytes"]), "profile": os.path.basename(path)}
     return None
+
+
+def tune_cache(args, label: str) -> dict:
+    """Cache arguments of autotune.tune for one leg's workload."""
+    return {"cache_path": args.tune_cache or None, "label": label, "refresh": args.retune}
 
 
 def cpu_baseline(words: np.ndarray, n: int, sample: int) -> dict:
@@ -103,6 +114,10 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-reference", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="plain leg at the default tunables")
+    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "murmuration-kernels_amd", "flock_hip",
+                                                         "tuned_mi355x.json"),
+                    help="autotuner results by device and workload; a workload found there is not searched again")
+    ap.add_argument("--retune", action="store_true", help="search every leg's tunables again (updates the cache)")
     ap.add_argument("--params", default="",
                     help="plain leg at these tunables instead of autotuning, e.g. "
                          "'cell=4,decide_block=512,birds_per_thread=4,decide_group=4,bucket_shift=0,flags=4'")
@@ -156,7 +171,8 @@ def main() -> int:
 
         # trials time the same window as the timed run (radii grow each decision)
         log(f"plain leg: autotune at {n} birds")
-        tuned = autotune.tune([words], "plain", flags=args.flags, warmup=args.warmup, steps=10, progress=log)
+        tuned = autotune.tune([words], "plain", flags=args.flags, warmup=args.warmup, steps=args.steps, progress=log,
+                              **tune_cache(args, f"plain-cube-{n}-seed{args.seed}"))
         prm, flags = tuned["params"], tuned["flags"]
 
     def timed_run(params: dict, flags: int) -> tuple[float, int]:
@@ -180,10 +196,11 @@ def main() -> int:
         passes = f.step(args.steps, 0, timing=True)
     due_last = int(passes["due_last"])  # birds that decided in the last timed step
 
-    nbytes = step_bytes(n, buckets, due_last)
+    nbytes = step_bytes(n, buckets, due_last, bool(flags & fh.FLAG_RESIDENT))
     dec_ms = passes["decide"]
     achieved = nbytes["decide"] / (dec_ms * 1e-3) / 1e9
     step_gbs = sum(nbytes.values()) / (passes["step_total"] * 1e-3) / 1e9
+    pass_gbs = {k: b / (passes[k] * 1e-3) / 1e9 for k, b in nbytes.items() if passes[k] > 0}
     vgpr = fh.kernel_info("decide")["vgprs"]
     full_prm = {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0, **prm}
     traffic = profiled_traffic(full_prm, flags)
@@ -209,12 +226,15 @@ def main() -> int:
             "flags": flags,
             "params": full_prm,
             "autotuned": tuned is not None,
+            "tunables_from_cache": bool(tuned and tuned.get("from_cache")),
             "device": dev["name"],
         },
         "ms_per_step_default_params": default_ms,
         "decide_ms": dec_ms,
         "passes_ms": {k: passes[k] for k in ("grid_histogram", "grid_scan", "grid_scatter", "decide",
                                             "integrate", "step_total")},
+        "passes_bytes": nbytes,
+        "passes_gbs": pass_gbs,
         "decide_vgprs": vgpr,
         "due_birds": due_last,
         "step_roofline": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -261,7 +281,8 @@ def reference_leg(args) -> dict:
         w = fh.gen_reference(n, 1)
         # the reference grid (4 m cells, buckets_for(n) buckets); launch shape autotuned
         tuned = autotune.tune([w], "plain", flags=fh.FLAG_UNFUSED, warmup=20, steps=5, cells=(4.0,), shifts=(0,),
-                              spatial=(0,), thorough=False, progress=log)
+                              spatial=(0,), thorough=False, progress=log,
+                              **tune_cache(args, f"reference-protocol-{n}-seed1"))
         for name, prm, flags in (("default", {}, fh.FLAG_UNFUSED), ("tuned", tuned["params"], tuned["flags"])):
             with fh.Flock.from_words(w, flags=flags, **prm) as f:
                 f.step(20)
@@ -270,6 +291,7 @@ def reference_leg(args) -> dict:
                                       "step_total")}
             row["bird_steps_per_s"] = n / (ms["step_total"] * 1e-3)
             row["params"] = prm
+            row["flags"] = flags
             rows[f"{n}_{name}"] = row
     rows["published_rtx4000ada_decide_ms"] = {"100000": 1.24, "1000000": 9.46}
     rows["published_rtx4000ada_step_total_ms"] = {"100000": 1.276, "1000000": 9.979}
@@ -288,7 +310,8 @@ def clumps_leg(args) -> dict:
     n = args.clump_birds
     w = workloads.clumps(n, 8, 5.0, 1000.0, seed=args.seed)
     tuned = autotune.tune([w], "plain", flags=fh.FLAG_GRAPH, warmup=1, steps=2,
-                          cells=(4.0, 1.0, 2.0, 3.0), progress=log, thorough=False)
+                          cells=(4.0, 1.0, 2.0, 3.0), progress=log, thorough=False,
+                          **tune_cache(args, f"clumps-8-sigma5-{n}-seed{args.seed}"))
     prm, flags = tuned["params"], tuned["flags"]
     with fh.Flock.from_words(w, flags=flags, **prm) as f:
         f.step(args.clump_warmup)
@@ -310,6 +333,7 @@ def clumps_leg(args) -> dict:
         "autotuned": prm,
         "flags": flags,
         "autotune_trials": len(tuned["trials"]),
+        "tunables_from_cache": bool(tuned.get("from_cache")),
         "leg_seconds": time.perf_counter() - t0,
     }
 
@@ -326,7 +350,7 @@ def batched_leg(args) -> dict:
     sizes = [int(w.shape[0]) for w in starts]
     total = sum(sizes)
     tuned = autotune.tune(starts, "batch", flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, warmup=2, steps=5,
-                          progress=log, thorough=False)
+                          progress=log, thorough=False, **tune_cache(args, f"mix-{args.flocks}-seed{args.seed}"))
     prm, flags = tuned["params"], tuned["flags"]
     buckets = sum(autotune._buckets(s) for s in sizes)
     with fh.Batch(total + 256 * len(starts), buckets + 4096 * len(starts), len(starts), flags=flags, **prm) as b:
@@ -359,6 +383,7 @@ def batched_leg(args) -> dict:
         "autotuned": prm,
         "flags": flags,
         "autotune_trials": len(tuned["trials"]),
+        "tunables_from_cache": bool(tuned.get("from_cache")),
         "leg_seconds": time.perf_counter() - t0,
     }
 
@@ -422,7 +447,8 @@ def served_leg(args, batched: dict) -> dict:
     log("served leg: autotune on 32 warmed request flocks")
     tuned = autotune.tune([w for w, _ in warm], "batch", flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, warmup=1, steps=4,
                           times=[t for _, t in warm],
-                          cells=(4.0, 6.0, 8.0, 10.0, 12.0), progress=log)
+                          cells=(4.0, 6.0, 8.0, 10.0, 12.0), progress=log,
+                          **tune_cache(args, f"served-warm250-seed{args.seed + 7}"))
     prm, flags = tuned["params"], tuned["flags"] & ~fh.FLAG_GRAPH
     del warm
     sat = workloads.stream(500, 5000.0, seed=args.seed + 1)
@@ -446,6 +472,7 @@ def served_leg(args, batched: dict) -> dict:
         "capacity_vs_plain": capacity["bird_steps_per_s"] / plain["bird_steps_per_s"],
         "autotuned": prm,
         "flags": flags,
+        "tunables_from_cache": bool(tuned.get("from_cache")),
         "leg_seconds": time.perf_counter() - t0,
     }
 

@@ -104,6 +104,11 @@ typedef struct fh_params {
  * none are left (instead of one workgroup per chunk). Results are
  * bit-identical; the autotuner decides whether it pays. */
 #define FH_FLAG_PERSISTENT 256u
+/* decide maps workgroups to chunks of slots XCD by XCD: the workgroups the
+ * GPU dispatches to one XCD (one L2) take one contiguous run of chunks, so
+ * neighbouring chunks share their halo cells in that L2 (no effect with
+ * FH_FLAG_PERSISTENT). Results are bit-identical. */
+#define FH_FLAG_XCD 512u
 
 /* Per-pass GPU time of fh_step, mean ms per timed step (hipEvents on the
  * handle's stream), in the rows of the reference report

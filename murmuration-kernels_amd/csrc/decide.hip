@@ -396,7 +396,18 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, Records r
     for (uint32_t round = 0;; ++round) {  // one chunk per trip; every lane leaves together
     if (threadIdx.x == 0) {
         n_due = 0;
-        cur_chunk = persist ? atomicAdd(p.work, 1u) : (round ? n_chunks : blockIdx.x);
+        uint32_t c = round ? n_chunks : blockIdx.x;
+        if (persist) {
+            c = atomicAdd(p.work, 1u);
+        } else if (!round && (walk_flags & WALK_XCD)) {
+            // The workgroups of XCD x (blockIdx % XCDS == x) take the x-th
+            // contiguous run of chunks in order, so the halo cells two
+            // neighbouring chunks share are read through one L2.
+            const uint32_t per = n_chunks / XCDS, rem = n_chunks % XCDS;
+            const uint32_t x = blockIdx.x % XCDS, i = blockIdx.x / XCDS;
+            c = x * per + min(x, rem) + i;
+        }
+        cur_chunk = c;
     }
     __syncthreads();
     const uint32_t c_idx = cur_chunk;

@@ -144,6 +144,10 @@ hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, boo
 constexpr int WALK_TILED = 1;  // dense neighbourhoods through LDS
 constexpr int WALK_OCC = 2;    // skip empty buckets by the occupancy bitmap
 constexpr int WALK_PERSIST = 4;  // a fixed grid of workgroups takes chunks from a counter
+constexpr int WALK_XCD = 8;      // each XCD's workgroups take one contiguous run of chunks
+// XCDs of an MI355X: workgroup i of a launch is dispatched to XCD i % 8, and
+// each XCD has its own L2.
+constexpr uint32_t XCDS = 8;
 // Events the decide launch records itself (kernel start and end), or none.
 struct DecideTiming {
     hipEvent_t start = nullptr, stop = nullptr;

@@ -3,12 +3,15 @@
 Picks the grid cell edge, the bucket count (bucket_shift), the decide
 workgroup size, the birds each decide workgroup scans per lane
 (birds_per_thread), the lanes that share one bird's walk (decide_group) and
-whether decide tests the bucket occupancy bitmap, whether cells are bucketed
-by the spatial hash and (single flocks) whether fh_step keeps the state in
-bucket order, by timing short runs of
-the real workload. Each candidate is timed over `steps` steps after `warmup`
-untimed ones. Results can be cached in a JSON file keyed by the workload's
-signature.
+whether decide tests the bucket occupancy bitmap, runs as a persistent grid
+or maps chunks XCD by XCD, whether cells are bucketed by the spatial hash and
+(single flocks) whether fh_step keeps the state in bucket order, by timing
+short runs of the real workload. Each candidate is timed over `steps` steps
+after `warmup` untimed ones; the fastest few are timed again and the lowest
+median kept. Results can be cached in a JSON file keyed by
+the search version, the device name, the caller's workload label and the
+workload's signature (flock sizes, steps, flags, search mode); a cached entry
+is returned without timing anything.
 
 Changing the cell edge changes the hash keys (the keys match the reference's
 only at 4 m) but not the neighbour sets: the walk always covers the 2r box.
@@ -22,7 +25,8 @@ import time
 
 import numpy as np
 
-from . import FLAG_CANONICAL, FLAG_GRAPH, FLAG_OCC, FLAG_PERSISTENT, FLAG_RESIDENT, FLAG_SPATIAL, Batch, Flock
+from . import (FLAG_CANONICAL, FLAG_GRAPH, FLAG_OCC, FLAG_PERSISTENT, FLAG_RESIDENT, FLAG_SPATIAL, FLAG_XCD, Batch,
+               Flock, device_info)
 
 BPT = (1, 2, 4, 8, 16)
 BLOCKS = (64, 128, 256, 512)
@@ -30,7 +34,13 @@ GROUPS = (1, 2, 4, 8, 16, 64)
 CELLS = (4.0, 3.0, 5.0, 6.0, 8.0, 10.0, 12.0)
 SHIFTS = (0, 1, 2, 3, 4)
 # on/off tunables and the flag each one sets
-SWITCHES = {"occ": FLAG_OCC, "resident": FLAG_RESIDENT, "spatial": FLAG_SPATIAL, "persist": FLAG_PERSISTENT}
+SWITCHES = {"occ": FLAG_OCC, "resident": FLAG_RESIDENT, "spatial": FLAG_SPATIAL, "persist": FLAG_PERSISTENT,
+            "xcd": FLAG_XCD}
+# Part of every cache key: raised whenever the searched space changes, so an
+# entry found in a smaller space is not reused.
+SEARCH_VERSION = 3
+FINALISTS = 6
+CONFIRM = 3
 
 
 def _signature(kind: str, sizes, steps: int, flags: int) -> str:
@@ -78,23 +88,27 @@ def _buckets(n: int) -> int:
 
 def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, steps: int = 8,
          cache_path: str | None = None, cells=CELLS, progress=None, thorough: bool = True, times=None,
-         shifts=SHIFTS, spatial=(0, 1)) -> dict:
+         shifts=SHIFTS, spatial=(0, 1), label: str = "", refresh: bool = False) -> dict:
     """Returns {"params": {...}, "flags": f, "ms_per_step": x, "trials": [...]}
     for the batch of `starts` (kind "batch") or the single flock starts[0]
     ("plain"). `flags` is the caller's flags plus the switches the search
-    turned on (FLAG_OCC, FLAG_RESIDENT, FLAG_SPATIAL, FLAG_PERSISTENT); `spatial=(0,)` keeps
+    turned on (FLAG_OCC, FLAG_RESIDENT, FLAG_SPATIAL, FLAG_PERSISTENT, FLAG_XCD); `spatial=(0,)` keeps
     the reference hash. With `thorough` the launch shape is searched
     exhaustively (group x birds_per_thread x block) before and after the
     cell and bucket searches; otherwise one coordinate sweep per tunable.
     `times` are the flocks' clocks (default 0), e.g. of states downloaded
     after some steps. `progress`, if given, is called with one line of text
-    per trial."""
+    per trial. `label` names the workload in the cache key (flocks of the
+    same sizes but another distribution want their own entry); `refresh`
+    searches again and replaces a cached entry. A result read from the cache
+    carries "from_cache": True."""
     times = [0.0] * len(starts) if times is None else [float(t) for t in times]
     sizes = [int(w.shape[0]) for w in starts]
-    key = _signature(kind, sizes, steps, flags) + (":t" if thorough else ":q")
+    key = (f"v{SEARCH_VERSION}|{device_info()['name']}|{label}|" + _signature(kind, sizes, steps, flags) +
+           (":t" if thorough else ":q"))
     cache = _load(cache_path) if cache_path else {}
-    if key in cache:
-        return cache[key]
+    if key in cache and not refresh:
+        return {**cache[key], "from_cache": True}
 
     def timer(prm):
         prm = dict(prm)
@@ -106,7 +120,7 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
         return _time_plain(starts[0], times[0], prm, f, warmup, steps)
 
     best = {"cell": float(cells[0]), "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0,
-            "occ": 0, "resident": 0, "spatial": 0, "persist": 0}
+            "occ": 0, "resident": 0, "spatial": 0, "persist": 0, "xcd": 0}
     trials = []
     seen = {}
 
@@ -144,12 +158,26 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
     # the resident layout is a single-flock fh_step mode, not for canonical order
     resident = (0, 1) if kind == "plain" and not (flags & FLAG_CANONICAL) else (0,)
     for nv in (("resident", resident), ("spatial", spatial), ("cell", cells), ("bucket_shift", shifts),
-               ("occ", (0, 1)), ("persist", (0, 1)), ("cell", cells)):
+               ("occ", (0, 1)), ("persist", (0, 1)), ("xcd", (0, 1)), ("cell", cells)):
         sweep([nv])
     if thorough:
         sweep(shape[:2])
     else:
         sweep([shape[0]])
+    # Short trials are noisy, so a search can settle a few percent away from
+    # the best point: the fastest FINALISTS distinct points are timed
+    # CONFIRM times more on fresh flocks and the lowest median wins.
+    finalists = sorted(seen.items(), key=lambda kv: kv[1])[:FINALISTS]
+    confirmed = []
+    for k, t in finalists:
+        prm = dict(k)
+        ts = sorted([t] + [timer(prm) for _ in range(CONFIRM)])
+        med = ts[len(ts) // 2]
+        confirmed.append((med, prm))
+        trials.append({**prm, "ms": med * 1e3, "confirm": [x * 1e3 for x in ts]})
+        if progress:
+            progress(f"{kind} confirm: {prm} median {med * 1e3:.4f} ms/step")
+    best_t, best = min(confirmed, key=lambda c: c[0])
     params = {k: v for k, v in best.items() if k not in SWITCHES}
     out_flags = flags
     for name, bit in SWITCHES.items():

@@ -324,14 +324,17 @@ def test_occupancy_skip_is_bit_identical(fh, n, cell, group, shift):
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
-@pytest.mark.parametrize("n,group,block,bpt,graph", [(200_000, 1, 256, 8, 0), (1_000_000, 4, 512, 1, 4),
-                                                    (100_000, 16, 128, 2, 4), (3000, 64, 64, 1, 0)])
-def test_persistent_decide_is_bit_identical(fh, n, group, block, bpt, graph):
-    # The persistent grid takes chunks from a counter the scan zeroes each
-    # step; every chunk must be decided exactly once, in every step.
+@pytest.mark.parametrize("n,group,block,bpt,graph,mode", [(200_000, 1, 256, 8, 0, 256), (1_000_000, 4, 512, 1, 4, 256),
+                                                         (100_000, 16, 128, 2, 4, 256), (3000, 64, 64, 1, 0, 256),
+                                                         (1_000_000, 4, 512, 4, 4, 512), (200_003, 1, 64, 2, 0, 512),
+                                                         (5000, 16, 128, 1, 0, 512)])
+def test_chunk_mapping_is_bit_identical(fh, n, group, block, bpt, graph, mode):
+    # Persistent grid (256): chunks are taken from a counter the scan zeroes
+    # each step. XCD mapping (512): a permutation of workgroups to chunks.
+    # Either way every chunk must be decided exactly once, in every step.
     w0 = cube(fh, n, 8)
     outs = []
-    for flags in (fh.FLAG_CANONICAL | fh.FLAG_PERSISTENT | graph, fh.FLAG_CANONICAL):
+    for flags in (fh.FLAG_CANONICAL | mode | graph, fh.FLAG_CANONICAL):
         with fh.Flock.from_words(w0, flags=flags, decide_group=group, decide_block=block,
                                  birds_per_thread=bpt) as f:
             f.step(9)
