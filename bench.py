@@ -67,6 +67,11 @@ def profiled_traffic(params: dict, flags: int):
     return None
 
 
+def full_prm_block(prm: dict) -> int:
+    """decide workgroup size of a tunables dict (the library default: 256)."""
+    return int(prm.get("decide_block", 256) or 256)
+
+
 def tune_cache(args, label: str) -> dict:
     """Cache arguments of autotune.tune for one leg's workload."""
     return {"cache_path": args.tune_cache or None, "label": label, "refresh": args.retune}
@@ -201,7 +206,12 @@ def main() -> int:
     achieved = nbytes["decide"] / (dec_ms * 1e-3) / 1e9
     step_gbs = sum(nbytes.values()) / (passes["step_total"] * 1e-3) / 1e9
     pass_gbs = {k: b / (passes[k] * 1e-3) / 1e9 for k, b in nbytes.items() if passes[k] > 0}
-    vgpr = fh.kernel_info("decide")["vgprs"]
+    dec_inst = (f"decide:{full_prm_block(prm)}:{prm.get('decide_group', 1)}:"
+                f"{'r' if flags & fh.FLAG_RESIDENT else 's'}")
+    kinfo = fh.kernel_info(dec_inst)
+    vgpr = kinfo["vgprs"]
+    # waves per SIMD the register file allows (512 VGPRs per lane, granule 8)
+    waves = min(8, 512 // (((vgpr + 7) // 8) * 8)) if vgpr else None
     full_prm = {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0, **prm}
     traffic = profiled_traffic(full_prm, flags)
 
@@ -235,7 +245,8 @@ def main() -> int:
                                             "integrate", "step_total")},
         "passes_bytes": nbytes,
         "passes_gbs": pass_gbs,
-        "decide_vgprs": vgpr,
+        "decide_kernel": {"instance": dec_inst, "vgprs": vgpr, "scratch_bytes": kinfo["scratch_bytes"],
+                          "waves_per_simd_by_vgprs": waves},
         "due_birds": due_last,
         "step_roofline": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_gbs / HBM_PEAK_GBS, "bytes": sum(nbytes.values())},

@@ -109,6 +109,11 @@ typedef struct fh_params {
  * neighbouring chunks share their halo cells in that L2 (no effect with
  * FH_FLAG_PERSISTENT). Results are bit-identical. */
 #define FH_FLAG_XCD 512u
+/* decide stages the positions of each chunk of slots, with a halo of half a
+ * chunk on each side, in LDS with coalesced loads, and reads candidates that
+ * fall in that window from LDS. Pays off with FH_FLAG_SPATIAL, whose bucket
+ * order keeps most candidates near the chunk. Results are bit-identical. */
+#define FH_FLAG_WINDOW 1024u
 
 /* Per-pass GPU time of fh_step, mean ms per timed step (hipEvents on the
  * handle's stream), in the rows of the reference report
@@ -140,7 +145,9 @@ fh_status fh_params_default(fh_params *p);
 /* Device name, compute units and HBM bytes of device 0. */
 fh_status fh_device_info(char *name, int name_cap, int *compute_units, size_t *hbm_bytes);
 /* Resources of the kernel named `kernel` ("decide", "integrate", "scatter",
- * "hash_count", "scan"): VGPRs per lane, static LDS bytes, scratch bytes. */
+ * "hash_count", "scan"; "decide:BLOCK:G:r" or "decide:BLOCK:G:s" for the
+ * decide instance of that workgroup size and lane group over resident
+ * records or sorted copies): VGPRs per lane, static LDS bytes, scratch bytes. */
 fh_status fh_kernel_info(const char *kernel, int *vgprs, int *lds_bytes, int *scratch_bytes);
 
 /* ---- .flk files and generators (host only; state.rs:58-170) ------------- */

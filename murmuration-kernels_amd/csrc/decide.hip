@@ -16,6 +16,8 @@
 // oracle's own test d^2 <= r^2.
 #include <hip/hip_ext.h>
 
+#include <cstdio>
+
 #include "layout.h"
 #include "model.h"
 
@@ -30,9 +32,25 @@ struct Decision {
 
 // Where a walk reads candidate t: position pos[STRIDE t], velocity
 // vel[STRIDE t] (STRIDE 2: interleaved sorted records; 1: resident arrays).
+// With a window, the positions and velocities of slots [wlo, wlo + wn) are
+// also held in LDS (winp, winv) and read from there.
 struct Cand {
     const float4 *pos, *vel;
+    const float4 *winp = nullptr, *winv = nullptr;
+    uint32_t wlo = 0, wn = 0;
 };
+
+template <int STRIDE>
+__device__ __forceinline__ float4 cand_pos(const Cand &c, uint32_t t) {
+    const uint32_t o = t - c.wlo;  // wraps for t < wlo
+    return o < c.wn ? c.winp[o] : c.pos[(size_t)STRIDE * t];
+}
+
+template <int STRIDE>
+__device__ __forceinline__ float4 cand_vel(const Cand &c, uint32_t t) {
+    const uint32_t o = t - c.wlo;
+    return o < c.wn ? c.winv[o] : c.vel[(size_t)STRIDE * t];
+}
 
 // Per-bird walk. `visit(slot)` sees each candidate slot that passes the
 // self and exact-cell checks and d^2 <= 4r^2, in walk order.
@@ -140,12 +158,12 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             s1[k] = min(s1[k], n_slots);
-            head[k] = s0[k] < s1[k] ? cand.pos[(size_t)STRIDE * s0[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            head[k] = s0[k] < s1[k] ? cand_pos<STRIDE>(cand, s0[k]) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             for (uint32_t t = s0[k]; t < s1[k]; ++t) {
-                const float4 c = t == s0[k] ? head[k] : cand.pos[(size_t)STRIDE * t];
+                const float4 c = t == s0[k] ? head[k] : cand_pos<STRIDE>(cand, t);
                 if (t == self) continue;
                 if (cell_coord(c.x, sg.inv_cell) != cx[k] || cell_coord(c.y, sg.inv_cell) != cy[k] ||
                     cell_coord(c.z, sg.inv_cell) != cz[k])
@@ -220,8 +238,8 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
                     c[u] = make_float4(0.f, 0.f, 0.f, 0.f);
                     v[u] = c[u];
                     if (t < s1[k]) {
-                        c[u] = cand.pos[(size_t)STRIDE * t];
-                        v[u] = cand.vel[(size_t)STRIDE * t];
+                        c[u] = cand_pos<STRIDE>(cand, t);
+                        v[u] = cand_vel<STRIDE>(cand, t);
                     }
                 }
 #pragma unroll
@@ -298,7 +316,7 @@ __device__ __forceinline__ Acc walk_sums(const Pools &p, Cand cand, const Seg &s
     acc_zero(a);
     if constexpr (G <= 8)
         walk_cells<G, PF, STRIDE>(w, sg, p, cand, n_slots, self, first, use_occ, [&](uint32_t t, f3 d, float d2) {
-            accumulate(a, f_i, w.r2, d, d2, [&] { return cand.vel[(size_t)STRIDE * t]; });
+            accumulate(a, f_i, w.r2, d, d2, [&] { return cand_vel<STRIDE>(cand, t); });
         });
     else
         walk_cells_split<G, PF, STRIDE>(w, sg, p, cand, n_slots, self, (uint32_t)first, use_occ,
@@ -371,22 +389,35 @@ constexpr uint32_t NO_BIRD = 0xffffffffu;
 // the flock; each workgroup takes chunks from *p.work until they run out, so
 // a workgroup that drew a sparse chunk moves on at once and the tail of the
 // launch is the last chunk's walk, not the slowest workgroup's share.
+// Waves per SIMD the compiler keeps room for (1: no floor). With G >= 4 the
+// walk fits 95 VGPRs without spilling, so 5 waves per SIMD instead of the 4
+// that its natural 105-115 VGPRs allow (measured: -2% to -9% decide time);
+// G = 1-2 would spill.
+constexpr int decide_min_waves(int g) { return g >= 4 ? 5 : 1; }
+
 template <int BLOCK, int G, bool RES>
-__global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, Records rec, float time, uint32_t chunk,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_min_waves(G), 8))) void k_decide(Geometry g, Pools p, Records rec, float time, uint32_t chunk,
                                                   int walk_flags, uint32_t *due_clear, uint32_t *work_clear) {
     const bool allow_tiled = (walk_flags & WALK_TILED) != 0, use_occ = (walk_flags & WALK_OCC) != 0;
     const bool persist = (walk_flags & WALK_PERSIST) != 0;
     // Cells whose bucket ranges a lane loads at once. The split walks (G >= 16)
     // take one cell at a time: their buckets are long, and the smaller code
-    // (the candidate loop is unrolled per cell) runs faster.
-    constexpr int PF = G >= 16 ? 1 : (G >= 4 ? 4 : 8);
+    // (the candidate loop is unrolled per cell) runs faster. A lane of G = 4-8
+    // walks one or two cells of a typical box, so 2 suffice; fewer registers
+    // for the prefetch mean more waves (G = 2: -20% decide time at 1M birds).
+    constexpr int PF = G >= 16 ? 1 : (G >= 4 ? 2 : 4);
     constexpr int STRIDE = RES ? 1 : 2;
-    const Cand cand = RES ? Cand{rec.p4, rec.v4} : Cand{p.sbird, p.sbird + 1};
+    Cand cand = RES ? Cand{rec.p4, rec.v4} : Cand{p.sbird, p.sbird + 1};
+    const bool use_win = (walk_flags & WALK_WINDOW) != 0;
+    // [chunk] due slots, then with a window float4 positions and velocities
+    // [window_slots(chunk)] each
     extern __shared__ uint32_t due_slots[];
+    float4 *const winp = reinterpret_cast<float4 *>(due_slots + chunk);
+    float4 *const winv = winp + window_slots(chunk);
     __shared__ uint32_t n_due;
     __shared__ int32_t u_lo[3], u_hi[3];
     __shared__ uint32_t u_seg_lo, u_seg_hi, u_vol, u_any;
-    __shared__ float4 tile[BLOCK];
+    __shared__ float4 tile[G == 1 ? BLOCK : 1];  // the tiled path (G = 1 only)
     __shared__ uint32_t cur_chunk;
     if (RES && blockIdx.x == 0 && threadIdx.x == 0) {
         if (due_clear) *due_clear = 0;
@@ -414,12 +445,29 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, Records r
     if (c_idx >= n_chunks) break;
     const uint32_t start = c_idx * chunk;
     const uint32_t end = min(start + chunk, g.n_slots);
+    if (use_win) {
+        // Stage the positions and velocities of the chunk and a halo on each
+        // side with coalesced loads; with the spatial hash most candidates
+        // of the chunk's due birds lie in this window of slots.
+        const uint32_t halo = window_halo(chunk);
+        const uint32_t wlo = start > halo ? start - halo : 0u;
+        const uint32_t whi = min(min(end + halo, g.n_slots), wlo + window_slots(chunk));
+        for (uint32_t t = wlo + threadIdx.x; t < whi; t += BLOCK) {
+            winp[t - wlo] = cand.pos[(size_t)STRIDE * t];
+            winv[t - wlo] = cand.vel[(size_t)STRIDE * t];
+        }
+        cand.winp = winp;
+        cand.winv = winv;
+        cand.wlo = wlo;
+        cand.wn = whi - wlo;
+        __syncthreads();
+    }
     // Sorted copies: time is relative (0) and snext holds next - time. Resident
     // records: this step's clock is the segment clock + DT (committed by the
     // integrate pass that follows) and p4.w holds next_decision.
     const float tnow = RES ? g.segs[0].time + DT : time;
     for (uint32_t s = start + threadIdx.x; s < end; s += BLOCK)
-        if (tnow >= (RES ? rec.p4[s].w : p.snext[s])) due_slots[atomicAdd(&n_due, 1u)] = s;
+        if (tnow >= (RES ? cand_pos<STRIDE>(cand, s).w : p.snext[s])) due_slots[atomicAdd(&n_due, 1u)] = s;
     __syncthreads();
     const uint32_t nd = n_due;
     if (threadIdx.x == 0 && nd) atomicAdd(p.due_count, nd);
@@ -437,8 +485,8 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, Records r
         w.ok = false;
         if (active) {
             slot = due_slots[k];
-            const float4 q = cand.pos[(size_t)STRIDE * slot];
-            const float4 v = cand.vel[(size_t)STRIDE * slot];  // sorted copies: w is the bird's radius
+            const float4 q = cand_pos<STRIDE>(cand, slot);
+            const float4 v = cand_vel<STRIDE>(cand, slot);  // sorted copies: w is the bird's radius
             me = RES ? rec.id[slot] : __float_as_uint(q.w);
             if (me >= g.n_birds) {
                 if (lane_g == 0) atomicOr(p.err, ERR_SLOT);
@@ -502,7 +550,7 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, Records r
                         for (uint32_t t0 = s0; t0 < s1; t0 += BLOCK) {
                             const uint32_t t = t0 + threadIdx.x;
                             if (t < s1) {
-                                const float4 c = cand.pos[(size_t)STRIDE * t];
+                                const float4 c = cand_pos<STRIDE>(cand, t);
                                 const bool in_cell = cell_coord(c.x, sg.inv_cell) == cx &&
                                                      cell_coord(c.y, sg.inv_cell) == cy &&
                                                      cell_coord(c.z, sg.inv_cell) == cz;
@@ -518,7 +566,7 @@ __global__ __launch_bounds__(BLOCK) void k_decide(Geometry g, Pools p, Records r
                                     const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
                                     const float d2 = dot3(d, d);
                                     if (d2 > w.c2) continue;
-                                    accumulate(a, f_i, w.r2, d, d2, [&] { return cand.vel[(size_t)STRIDE * ts]; });
+                                    accumulate(a, f_i, w.r2, d, d2, [&] { return cand_vel<STRIDE>(cand, ts); });
                                 }
                             }
                             __syncthreads();
@@ -575,17 +623,18 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
 // more than there are chunks. The occupancy is a property of the compiled
 // kernel; it is looked up once per instance and chunk size.
 template <int BLOCK, int G, bool RES>
-static uint32_t persistent_blocks(uint32_t chunk, uint32_t n_chunks) {
-    static int per_cu[5] = {0, 0, 0, 0, 0};  // by log2(chunk / BLOCK): 1..16 birds per thread
+static uint32_t persistent_blocks(uint32_t chunk, uint32_t n_chunks, size_t lds) {
+    // by log2(chunk / BLOCK) (1..16 birds per thread), without and with a window
+    static int per_cu[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
     const uint32_t bpt = chunk / BLOCK;
     const int idx = bpt <= 1 ? 0 : bpt <= 2 ? 1 : bpt <= 4 ? 2 : bpt <= 8 ? 3 : 4;
-    int occ = __atomic_load_n(&per_cu[idx], __ATOMIC_RELAXED);
+    int *slot = &per_cu[lds > chunk * sizeof(uint32_t) ? 1 : 0][idx];
+    int occ = __atomic_load_n(slot, __ATOMIC_RELAXED);
     if (occ <= 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_decide<BLOCK, G, RES>, BLOCK,
-                                                         chunk * sizeof(uint32_t)) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_decide<BLOCK, G, RES>, BLOCK, lds) != hipSuccess ||
             occ <= 0)
             occ = 1;
-        __atomic_store_n(&per_cu[idx], occ, __ATOMIC_RELAXED);
+        __atomic_store_n(slot, occ, __ATOMIC_RELAXED);
     }
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -601,14 +650,15 @@ static void launch_decide_block(const Geometry &g, const Pools &p, const Records
                                 int walk_flags, uint32_t *due_clear, uint32_t *work_clear, const DecideTiming &tm,
                                 hipStream_t s) {
     const uint32_t n_chunks = (g.n_slots + chunk - 1) / chunk;
+    const size_t lds = decide_lds_bytes(chunk, walk_flags);
     const uint32_t blocks =
-        (walk_flags & WALK_PERSIST) ? persistent_blocks<BLOCK, G, RES>(chunk, n_chunks) : n_chunks;
+        (walk_flags & WALK_PERSIST) ? persistent_blocks<BLOCK, G, RES>(chunk, n_chunks, lds) : n_chunks;
     if (tm.start)
-        hipExtLaunchKernelGGL((k_decide<BLOCK, G, RES>), dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s,
-                              tm.start, tm.stop, 0u, g, p, rec, time, chunk, walk_flags, due_clear, work_clear);
+        hipExtLaunchKernelGGL((k_decide<BLOCK, G, RES>), dim3(blocks), dim3(BLOCK), lds, s, tm.start, tm.stop, 0u, g,
+                              p, rec, time, chunk, walk_flags, due_clear, work_clear);
     else
-        hipLaunchKernelGGL((k_decide<BLOCK, G, RES>), dim3(blocks), dim3(BLOCK), chunk * sizeof(uint32_t), s, g, p,
-                           rec, time, chunk, walk_flags, due_clear, work_clear);
+        hipLaunchKernelGGL((k_decide<BLOCK, G, RES>), dim3(blocks), dim3(BLOCK), lds, s, g, p, rec, time, chunk,
+                           walk_flags, due_clear, work_clear);
 }
 
 template <int BLOCK, bool RES>
@@ -679,9 +729,42 @@ hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, i
     return hipGetLastError();
 }
 
+template <int BLOCK, bool RES>
+static const void *decide_fn_group(uint32_t group) {
+    switch (group) {
+        case 1: return reinterpret_cast<const void *>(k_decide<BLOCK, 1, RES>);
+        case 2: return reinterpret_cast<const void *>(k_decide<BLOCK, 2, RES>);
+        case 4: return reinterpret_cast<const void *>(k_decide<BLOCK, 4, RES>);
+        case 8: return reinterpret_cast<const void *>(k_decide<BLOCK, 8, RES>);
+        case 16: return reinterpret_cast<const void *>(k_decide<BLOCK, 16, RES>);
+        case 64: return reinterpret_cast<const void *>(k_decide<BLOCK, 64, RES>);
+        default: return nullptr;
+    }
+}
+
+template <bool RES>
+static const void *decide_fn(uint32_t block, uint32_t group) {
+    switch (block) {
+        case 64: return decide_fn_group<64, RES>(group);
+        case 128: return decide_fn_group<128, RES>(group);
+        case 256: return decide_fn_group<256, RES>(group);
+        case 512: return decide_fn_group<512, RES>(group);
+        default: return nullptr;
+    }
+}
+
+// "decide" is the default instance (block 256, G 1, sorted copies);
+// "decide:BLOCK:G:r" or "decide:BLOCK:G:s" names the instance for that
+// workgroup size and lane group over resident records or sorted copies.
 hipError_t decide_kernel_attributes(const char *name, hipFuncAttributes *out) {
     if (!__builtin_strcmp(name, "decide"))
         return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_decide<256, 1, false>));
+    unsigned block = 0, group = 0;
+    char kind = 0;
+    if (std::sscanf(name, "decide:%u:%u:%c", &block, &group, &kind) == 3 && (kind == 'r' || kind == 's')) {
+        const void *f = kind == 'r' ? decide_fn<true>(block, group) : decide_fn<false>(block, group);
+        if (f) return hipFuncGetAttributes(out, f);
+    }
     return hipErrorInvalidValue;
 }
 

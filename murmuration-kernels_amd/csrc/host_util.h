@@ -45,7 +45,8 @@ inline uint32_t flock_buckets(uint32_t n, const fh_params &p) {
 // decide's walk switches (WALK_* of layout.h) for the FH_FLAG_* flags.
 inline int walk_flags(uint32_t flags) {
     return ((flags & FH_FLAG_NO_TILE) ? 0 : 1) | ((flags & FH_FLAG_OCC) ? 2 : 0) |
-           ((flags & FH_FLAG_PERSISTENT) ? 4 : 0) | ((flags & FH_FLAG_XCD) ? 8 : 0);
+           ((flags & FH_FLAG_PERSISTENT) ? 4 : 0) | ((flags & FH_FLAG_XCD) ? 8 : 0) |
+           ((flags & FH_FLAG_WINDOW) ? 16 : 0);
 }
 
 // .flk record words (crates/flock-ref/src/state.rs:4-7).

@@ -145,6 +145,28 @@ constexpr int WALK_TILED = 1;  // dense neighbourhoods through LDS
 constexpr int WALK_OCC = 2;    // skip empty buckets by the occupancy bitmap
 constexpr int WALK_PERSIST = 4;  // a fixed grid of workgroups takes chunks from a counter
 constexpr int WALK_XCD = 8;      // each XCD's workgroups take one contiguous run of chunks
+constexpr int WALK_WINDOW = 16;  // stage the positions of the chunk and a halo in LDS
+// decide's LDS window: the chunk and a halo of half a chunk on each side, at
+// most WINDOW_MAX slots and within DECIDE_LDS_MAX bytes of dynamic LDS together
+// with the chunk's due-slot list (32 bytes per slot: position and velocity).
+constexpr uint32_t WINDOW_MAX = 4096;
+constexpr uint32_t DECIDE_LDS_MAX = 52u << 10;  // + at most 8.3 KB static: under 64 KB
+FH_HD uint32_t window_cap(uint32_t chunk) {
+    const uint32_t room = chunk * 4u < DECIDE_LDS_MAX ? (DECIDE_LDS_MAX - chunk * 4u) / 32u : 0u;
+    return room < WINDOW_MAX ? room : WINDOW_MAX;
+}
+FH_HD uint32_t window_halo(uint32_t chunk) {
+    const uint32_t cap = window_cap(chunk);
+    return chunk >= cap ? 0u : (chunk / 2 < (cap - chunk) / 2 ? chunk / 2 : (cap - chunk) / 2);
+}
+FH_HD uint32_t window_slots(uint32_t chunk) {
+    const uint32_t w = chunk + 2u * window_halo(chunk), cap = window_cap(chunk);
+    return w < cap ? w : cap;
+}
+// Dynamic LDS of one decide workgroup: the due-slot list, then the window.
+inline size_t decide_lds_bytes(uint32_t chunk, int walk_flags) {
+    return chunk * sizeof(uint32_t) + ((walk_flags & WALK_WINDOW) ? 2u * window_slots(chunk) * sizeof(float4) : 0u);
+}
 // XCDs of an MI355X: workgroup i of a launch is dispatched to XCD i % 8, and
 // each XCD has its own L2.
 constexpr uint32_t XCDS = 8;

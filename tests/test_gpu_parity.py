@@ -327,14 +327,23 @@ def test_occupancy_skip_is_bit_identical(fh, n, cell, group, shift):
 @pytest.mark.parametrize("n,group,block,bpt,graph,mode", [(200_000, 1, 256, 8, 0, 256), (1_000_000, 4, 512, 1, 4, 256),
                                                          (100_000, 16, 128, 2, 4, 256), (3000, 64, 64, 1, 0, 256),
                                                          (1_000_000, 4, 512, 4, 4, 512), (200_003, 1, 64, 2, 0, 512),
-                                                         (5000, 16, 128, 1, 0, 512)])
+                                                         (5000, 16, 128, 1, 0, 512),
+                                                         (1_000_000, 4, 512, 2, 4, 1024 | 512 | 128),
+                                                         (200_000, 1, 256, 8, 0, 1024),
+                                                         (100_000, 16, 128, 4, 0, 1024 | 256),
+                                                         (300_000, 64, 512, 16, 4, 1024 | 128),
+                                                         (2000, 2, 64, 1, 0, 1024)])
 def test_chunk_mapping_is_bit_identical(fh, n, group, block, bpt, graph, mode):
     # Persistent grid (256): chunks are taken from a counter the scan zeroes
     # each step. XCD mapping (512): a permutation of workgroups to chunks.
+    # LDS window (1024): candidate positions near the chunk read from LDS.
     # Either way every chunk must be decided exactly once, in every step.
+    # Both runs use the same hash: with G >= 16 the lanes split each bucket's
+    # candidates by their place in the bucket, so another hash (other
+    # collisions) groups the partial sums differently.
     w0 = cube(fh, n, 8)
     outs = []
-    for flags in (fh.FLAG_CANONICAL | mode | graph, fh.FLAG_CANONICAL):
+    for flags in (fh.FLAG_CANONICAL | mode | graph, fh.FLAG_CANONICAL | (mode & fh.FLAG_SPATIAL)):
         with fh.Flock.from_words(w0, flags=flags, decide_group=group, decide_block=block,
                                  birds_per_thread=bpt) as f:
             f.step(9)
@@ -384,6 +393,18 @@ def test_empty_flock_steps(fh):
         f.step(3)
         w, t = f.download()
         assert w.shape == (0, 15) and abs(t - 0.015) < 1e-6
+
+
+def test_decide_instances_fit_their_register_budget(fh):
+    # Lane groups of 4 and more are compiled for at least 5 waves per SIMD
+    # (at most 96 VGPRs of the 512 per lane) without spilling to scratch.
+    for block in (64, 256, 512):
+        for group in (4, 8, 16, 64):
+            for kind in "rs":
+                k = fh.kernel_info(f"decide:{block}:{group}:{kind}")
+                assert 0 < k["vgprs"] <= 96 and k["scratch_bytes"] == 0, (block, group, kind, k)
+    with pytest.raises(fh.FlockHipError):
+        fh.kernel_info("decide:512:3:r")
 
 
 def test_timing_report(fh):

@@ -58,7 +58,8 @@ def test_resident_grid_is_the_reference_grid(fh, fr, n, kind, cell):
 
 
 @pytest.mark.parametrize("flags,group", [(64, 1), (64 | 4, 4), (64 | 4 | 32, 2), (64 | 4, 16), (64 | 4, 64),
-                                         (64 | 4 | 256, 4), (64 | 256, 1), (64 | 4 | 128 | 512, 4)])
+                                         (64 | 4 | 256, 4), (64 | 256, 1), (64 | 4 | 128 | 512, 4),
+                                         (64 | 4 | 128 | 512 | 1024, 4), (64 | 1024, 1), (64 | 4 | 1024 | 256, 16)])
 def test_resident_step_matches_the_oracle(fh, fr, flags, group):
     w0 = cube(fh, 200_000, 5)
     with fh.Flock.from_words(w0, flags=flags, decide_group=group) as f:
