@@ -95,10 +95,11 @@ __device__ __forceinline__ bool bucket_occupied(const Pools &p, uint32_t b) {
 // `first` (G = 1: every cell). Its cells are taken PF at a time: the bucket
 // ranges of all PF cells are loaded together, then the first candidate of
 // each non-empty bucket, so a lane has up to 2 * PF independent loads in
-// flight instead of a chain of dependent ones. With `use_occ` the occupancy
-// bits of the PF buckets are read first (a small, cache-resident table) and
-// empty buckets never touch the offsets table. Within a lane the visit order
-// is the walk order.
+// flight instead of a chain of dependent ones; within a bucket, candidates
+// are loaded AHEAD at a time. With `use_occ` the occupancy bits of the PF
+// buckets are read first (a small, cache-resident table) and empty buckets
+// never touch the offsets table. Within a lane the visit order is the walk
+// order.
 template <int G, int PF, int STRIDE, class Visit>
 __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const Pools &p, Cand cand, uint32_t n_slots,
                                            uint32_t self, int32_t first, bool use_occ, Visit &&visit) {
@@ -160,18 +161,32 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
             s1[k] = min(s1[k], n_slots);
             head[k] = s0[k] < s1[k] ? cand_pos<STRIDE>(cand, s0[k]) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
+        // AHEAD candidates of a bucket are loaded before the first is tested, so
+        // one memory latency covers them (a walk is a chain of dependent
+        // loads; this is what bounds decide). Same candidates, same order.
+        constexpr int AHEAD = 4;
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
-            for (uint32_t t = s0[k]; t < s1[k]; ++t) {
-                const float4 c = t == s0[k] ? head[k] : cand_pos<STRIDE>(cand, t);
-                if (t == self) continue;
-                if (cell_coord(c.x, sg.inv_cell) != cx[k] || cell_coord(c.y, sg.inv_cell) != cy[k] ||
-                    cell_coord(c.z, sg.inv_cell) != cz[k])
-                    continue;
-                const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
-                const float d2 = dot3(d, d);
-                if (d2 > w.c2) continue;
-                visit(t, d, d2);
+            for (uint32_t t0 = s0[k]; t0 < s1[k]; t0 += AHEAD) {
+                float4 c[AHEAD];
+#pragma unroll
+                for (int u = 0; u < AHEAD; ++u) {
+                    const uint32_t t = t0 + (uint32_t)u;
+                    c[u] = u == 0 && t == s0[k] ? head[k]
+                           : (t < s1[k] ? cand_pos<STRIDE>(cand, t) : make_float4(0.f, 0.f, 0.f, 0.f));
+                }
+#pragma unroll
+                for (int u = 0; u < AHEAD; ++u) {
+                    const uint32_t t = t0 + (uint32_t)u;
+                    if (t >= s1[k] || t == self) continue;
+                    if (cell_coord(c[u].x, sg.inv_cell) != cx[k] || cell_coord(c[u].y, sg.inv_cell) != cy[k] ||
+                        cell_coord(c[u].z, sg.inv_cell) != cz[k])
+                        continue;
+                    const f3 d = mk3(c[u].x - w.p.x, c[u].y - w.p.y, c[u].z - w.p.z);
+                    const float d2 = dot3(d, d);
+                    if (d2 > w.c2) continue;
+                    visit(t, d, d2);
+                }
             }
         }
     }
@@ -403,9 +418,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
     // Cells whose bucket ranges a lane loads at once. The split walks (G >= 16)
     // take one cell at a time: their buckets are long, and the smaller code
     // (the candidate loop is unrolled per cell) runs faster. A lane of G = 4-8
-    // walks one or two cells of a typical box, so 2 suffice; fewer registers
-    // for the prefetch mean more waves (G = 2: -20% decide time at 1M birds).
-    constexpr int PF = G >= 16 ? 1 : (G >= 4 ? 2 : 4);
+    // walks one or two cells of a typical box and loads its candidates 4
+    // ahead, so one cell at a time keeps it within 95 VGPRs (5 waves per
+    // SIMD) without spilling. Fewer registers for the prefetch mean more
+    // waves (G = 2 with 4 cells instead of 8: -20% decide time at 1M birds).
+    constexpr int PF = G >= 4 ? 1 : 4;
     constexpr int STRIDE = RES ? 1 : 2;
     Cand cand = RES ? Cand{rec.p4, rec.v4} : Cand{p.sbird, p.sbird + 1};
     const bool use_win = (walk_flags & WALK_WINDOW) != 0;

@@ -204,6 +204,7 @@ static fh_status alloc_resident(fh_flock *h) {
     h->W.arr = reinterpret_cast<uint32_t *>(base + o_arr);
     h->W.dep = reinterpret_cast<uint32_t *>(base + o_dep);
     h->W.abase = reinterpret_cast<uint32_t *>(base + o_abase);
+    h->W.tsum = h->P.tile_sums;  // the entry scan leaves each tile's total there
     h->off[0] = h->P.offsets;
     h->off[1] = reinterpret_cast<uint32_t *>(base + o_off1);
     h->res_due = reinterpret_cast<uint32_t *>(base + o_due);

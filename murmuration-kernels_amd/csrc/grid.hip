@@ -447,34 +447,12 @@ __global__ __launch_bounds__(TILE) void k_r_flush(Geometry g, Pools p, Records r
     p.key[b] = rec.key[s];
 }
 
-// Rebuild, level 1: counts += arrivals - departures (kept), tile sums.
-__global__ __launch_bounds__(256) void k_r_scan_reduce(uint32_t *__restrict__ counts,
-                                                       const uint32_t *__restrict__ arr,
-                                                       const uint32_t *__restrict__ dep,
-                                                       uint32_t *__restrict__ tile_sums) {
-    const size_t base = (size_t)blockIdx.x * SCAN_TILE;
-    uint4 *c4 = reinterpret_cast<uint4 *>(counts + base);
-    const uint4 *a4 = reinterpret_cast<const uint4 *>(arr + base);
-    const uint4 *d4 = reinterpret_cast<const uint4 *>(dep + base);
-    uint32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t i = (uint32_t)k * 256u + threadIdx.x;
-        uint4 c = c4[i];
-        const uint4 a = a4[i], d = d4[i];
-        c = make_uint4(c.x - d.x + a.x, c.y - d.y + a.y, c.z - d.z + a.z, c.w - d.w + a.w);
-        c4[i] = c;
-        sum += c.x + c.y + c.z + c.w;
-    }
-    uint32_t total;
-    block_excl_scan<4>(sum, &total);
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
-}
-
-// Rebuild, level 2 (at most SCAN_MERGE_TILES tiles): the new offsets, the
-// first arrival slot of each bucket (its stayers come first), the occupancy
-// bits; arrivals and departures are zeroed for the next step.
-__global__ __launch_bounds__(256) void k_r_scan_apply(const uint32_t *__restrict__ counts,
+// Rebuild scan (one launch, at most SCAN_MERGE_TILES tiles): counts +=
+// arrivals - departures; the new offsets, from the tiles' totals that the
+// integrate pass kept current with its movers; the first arrival slot of each
+// bucket (its stayers come first); the occupancy bits. Arrivals and
+// departures are zeroed for the next step.
+__global__ __launch_bounds__(256) void k_r_scan_apply(uint32_t *__restrict__ counts,
                                                       uint32_t *__restrict__ arr, uint32_t *__restrict__ dep,
                                                       const uint32_t *__restrict__ tile_sums, uint32_t ntiles,
                                                       uint32_t *__restrict__ noff, uint32_t *__restrict__ abase,
@@ -484,15 +462,19 @@ __global__ __launch_bounds__(256) void k_r_scan_apply(const uint32_t *__restrict
     uint32_t prefix;
     block_excl_scan<4>(part, &prefix);
     const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * 16u;
-    const uint4 *c4 = reinterpret_cast<const uint4 *>(counts + base);
+    uint4 *c4 = reinterpret_cast<uint4 *>(counts + base);
     uint4 *a4 = reinterpret_cast<uint4 *>(arr + base);
     uint4 *d4 = reinterpret_cast<uint4 *>(dep + base);
     uint32_t c[16], a[16];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const uint4 v = c4[k], w = a4[k];
-        c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w;
+        const uint4 v = c4[k], w = a4[k], d = d4[k];
+        c[4 * k] = v.x - d.x + w.x;
+        c[4 * k + 1] = v.y - d.y + w.y;
+        c[4 * k + 2] = v.z - d.z + w.z;
+        c[4 * k + 3] = v.w - d.w + w.w;
         a[4 * k] = w.x; a[4 * k + 1] = w.y; a[4 * k + 2] = w.z; a[4 * k + 3] = w.w;
+        c4[k] = make_uint4(c[4 * k], c[4 * k + 1], c[4 * k + 2], c[4 * k + 3]);
     }
     uint32_t sum = 0, bits = 0;
 #pragma unroll
@@ -585,8 +567,7 @@ hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Reco
                                    const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, hipStream_t s) {
     const uint32_t ntiles = (g.n_buckets + SCAN_TILE - 1) / SCAN_TILE;
     if (ntiles == 0 || ntiles > SCAN_MERGE_TILES) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_r_scan_reduce, dim3(ntiles), dim3(256), 0, s, p.counts, w.arr, w.dep, p.tile_sums);
-    hipLaunchKernelGGL(k_r_scan_apply, dim3(ntiles), dim3(256), 0, s, p.counts, w.arr, w.dep, p.tile_sums, ntiles,
+    hipLaunchKernelGGL(k_r_scan_apply, dim3(ntiles), dim3(256), 0, s, p.counts, w.arr, w.dep, w.tsum, ntiles,
                        off_nxt, w.abase, p.occ);
     if (g.n_slots)
         hipLaunchKernelGGL(k_r_move, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, cur, nxt, w,

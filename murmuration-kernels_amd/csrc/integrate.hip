@@ -98,9 +98,10 @@ __global__ __launch_bounds__(TILE) void k_integrate(Geometry g, Pools p) {
 
 // Banked flight over resident records (slot order, one flock). A bird whose
 // new cell falls in another bucket is a mover: it counts itself out of its
-// old bucket and into its new one (keeping its arrival rank); every slot
-// records its new bucket and its mover bit. Workgroup 0 commits the step's
-// clock (segment time += DT, the time decide used).
+// old bucket and into its new one (keeping its arrival rank), and out of and
+// into the scan tiles' totals when the tile changes; every slot records its
+// new bucket and its mover bit. Workgroup 0 commits the step's clock (segment
+// time += DT, the time decide used).
 __global__ __launch_bounds__(TILE) void k_r_integrate(Geometry g, Records rec, ResWork w, Seg *seg) {
     const uint32_t s = blockIdx.x * TILE + threadIdx.x;
     const bool active = s < g.n_slots;
@@ -122,6 +123,10 @@ __global__ __launch_bounds__(TILE) void k_r_integrate(Geometry g, Records rec, R
             mover = true;
             w.arank[s] = atomicAdd(&w.arr[kn], 1u);
             atomicAdd(&w.dep[ko], 1u);
+            if (kn / SCAN_TILE != ko / SCAN_TILE) {
+                atomicAdd(&w.tsum[kn / SCAN_TILE], 1u);
+                atomicSub(&w.tsum[ko / SCAN_TILE], 1u);
+            }
         }
     }
     const uint64_t bits = __ballot(mover);

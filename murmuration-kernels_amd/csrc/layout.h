@@ -116,6 +116,7 @@ struct ResWork {
     uint32_t *mbits;  // per old slot: mover bit (32 slots per word)
     uint32_t *arr, *dep;  // per bucket: arrivals, departures (zero between steps)
     uint32_t *abase;      // per bucket: first slot of its arrivals in the new layout
+    uint32_t *tsum;       // per scan tile: its buckets' count total, kept from step to step
 };
 
 // Launch geometry of one build/step over the pools.
