@@ -202,12 +202,25 @@ def main() -> int:
     due_last = int(passes["due_last"])  # birds that decided in the last timed step
 
     nbytes = step_bytes(n, buckets, due_last, bool(flags & fh.FLAG_RESIDENT))
+    # With the fused flight, the decide launch also does the integrate pass's
+    # work (its event interval is then empty): the dominant kernel's bytes are
+    # both passes'. decide alone is timed in a run without the fusion.
+    fused = bool(flags & fh.FLAG_FLIGHT) and bool(flags & fh.FLAG_RESIDENT)
     dec_ms = passes["decide"]
-    achieved = nbytes["decide"] / (dec_ms * 1e-3) / 1e9
+    dec_bytes = nbytes["decide"] + (nbytes["integrate"] if fused else 0)
+    achieved = dec_bytes / (dec_ms * 1e-3) / 1e9
     step_gbs = sum(nbytes.values()) / (passes["step_total"] * 1e-3) / 1e9
-    pass_gbs = {k: b / (passes[k] * 1e-3) / 1e9 for k, b in nbytes.items() if passes[k] > 0}
+    if fused:
+        pass_gbs = {"decide+flight": achieved,
+                    "grid_scatter": nbytes["grid_scatter"] / (passes["grid_scatter"] * 1e-3) / 1e9}
+        with fh.Flock.from_words(words, flags=flags & ~(fh.FLAG_GRAPH | fh.FLAG_FLIGHT), **prm) as f:
+            f.step(args.warmup)
+            decide_alone_ms = f.step(args.steps, 0, timing=True)["decide"]
+    else:
+        pass_gbs = {k: b / (passes[k] * 1e-3) / 1e9 for k, b in nbytes.items() if passes[k] > 0}
+        decide_alone_ms = dec_ms
     dec_inst = (f"decide:{full_prm_block(prm)}:{prm.get('decide_group', 1)}:"
-                f"{'r' if flags & fh.FLAG_RESIDENT else 's'}")
+                f"{('f' if flags & fh.FLAG_FLIGHT else 'r') if flags & fh.FLAG_RESIDENT else 's'}")
     kinfo = fh.kernel_info(dec_inst)
     vgpr = kinfo["vgprs"]
     # waves per SIMD the register file allows (512 VGPRs per lane, granule 8)
@@ -240,7 +253,8 @@ def main() -> int:
             "device": dev["name"],
         },
         "ms_per_step_default_params": default_ms,
-        "decide_ms": dec_ms,
+        "decide_ms": decide_alone_ms,
+        "decide_kernel_ms": dec_ms,
         "passes_ms": {k: passes[k] for k in ("grid_histogram", "grid_scan", "grid_scatter", "decide",
                                             "integrate", "step_total")},
         "passes_bytes": nbytes,
@@ -252,7 +266,7 @@ def main() -> int:
                           "frac": step_gbs / HBM_PEAK_GBS, "bytes": sum(nbytes.values())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
-                     "kernel": "decide", "bytes_per_launch": nbytes["decide"],
+                     "kernel": "decide with fused flight" if fused else "decide", "bytes_per_launch": dec_bytes,
                      "traffic_profile": traffic["profile"] if traffic else None},
     }
     if not args.no_cpu:

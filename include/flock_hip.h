@@ -114,6 +114,14 @@ typedef struct fh_params {
  * fall in that window from LDS. Pays off with FH_FLAG_SPATIAL, whose bucket
  * order keeps most candidates near the chunk. Results are bit-identical. */
 #define FH_FLAG_WINDOW 1024u
+/* With FH_FLAG_RESIDENT: decide flies the birds of each chunk of slots right
+ * after deciding them, in the same launch (the new positions go to a third
+ * set of records, so other walks keep reading this step's positions). One
+ * launch less per step, and the flight's streaming overlaps the walks' memory
+ * latency; fh_pass_ms then counts the flight in `decide` (integrate 0). The
+ * results equal the unfused resident step's. Ignored without
+ * FH_FLAG_RESIDENT. */
+#define FH_FLAG_FLIGHT 2048u
 
 /* Per-pass GPU time of fh_step, mean ms per timed step (hipEvents on the
  * handle's stream), in the rows of the reference report
@@ -145,9 +153,10 @@ fh_status fh_params_default(fh_params *p);
 /* Device name, compute units and HBM bytes of device 0. */
 fh_status fh_device_info(char *name, int name_cap, int *compute_units, size_t *hbm_bytes);
 /* Resources of the kernel named `kernel` ("decide", "integrate", "scatter",
- * "hash_count", "scan"; "decide:BLOCK:G:r" or "decide:BLOCK:G:s" for the
- * decide instance of that workgroup size and lane group over resident
- * records or sorted copies): VGPRs per lane, static LDS bytes, scratch bytes. */
+ * "hash_count", "scan"; "decide:BLOCK:G:K" for the decide instance of that
+ * workgroup size and lane group over sorted copies (K = s), resident records
+ * (r) or resident records with the fused flight (f)): VGPRs per lane, static
+ * LDS bytes, scratch bytes. */
 fh_status fh_kernel_info(const char *kernel, int *vgprs, int *lds_bytes, int *scratch_bytes);
 
 /* ---- .flk files and generators (host only; state.rs:58-170) ------------- */

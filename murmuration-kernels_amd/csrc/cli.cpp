@@ -15,7 +15,8 @@
 static int usage() {
     std::fprintf(stderr,
                  "usage: flock_hip --in FILE [--steps K] [--warmup W] [--out FILE]\n"
-                 "       [--canonical] [--unfused] [--graph] [--resident] [--occ] [--spatial] [--persistent] [--xcd] [--window]\n"
+                 "       [--canonical] [--unfused] [--graph] [--resident] [--occ] [--spatial]\n"
+                 "       [--persistent] [--xcd] [--window] [--flight]\n"
                  "       [--cell M] [--block B] [--bpt K] [--group G] [--shift S]\n"
                  "(the first W steps are not timed)\n");
     return 2;
@@ -58,6 +59,7 @@ int main(int argc, char **argv) {
         else if (a == "--persistent") prm.flags |= FH_FLAG_PERSISTENT;
         else if (a == "--xcd") prm.flags |= FH_FLAG_XCD;
         else if (a == "--window") prm.flags |= FH_FLAG_WINDOW;
+        else if (a == "--flight") prm.flags |= FH_FLAG_FLIGHT;
         else return usage();
     }
     if (!in) {

@@ -17,7 +17,9 @@
 #include <hip/hip_ext.h>
 
 #include <cstdio>
+#include <type_traits>
 
+#include "flight.h"
 #include "layout.h"
 #include "model.h"
 
@@ -410,9 +412,10 @@ constexpr uint32_t NO_BIRD = 0xffffffffu;
 // G = 1-2 would spill.
 constexpr int decide_min_waves(int g) { return g >= 4 ? 5 : 1; }
 
-template <int BLOCK, int G, bool RES>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_min_waves(G), 8))) void k_decide(Geometry g, Pools p, Records rec, float time, uint32_t chunk,
-                                                  int walk_flags, uint32_t *due_clear, uint32_t *work_clear) {
+template <int BLOCK, int G, bool RES, bool FLY>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_min_waves(G), 8))) void k_decide(
+    Geometry g, Pools p, Records rec, float time, uint32_t chunk, int walk_flags, uint32_t *due_clear,
+    uint32_t *work_clear, ResWork w, Records flown) {
     const bool allow_tiled = (walk_flags & WALK_TILED) != 0, use_occ = (walk_flags & WALK_OCC) != 0;
     const bool persist = (walk_flags & WALK_PERSIST) != 0;
     // Cells whose bucket ranges a lane loads at once. The split walks (G >= 16)
@@ -607,6 +610,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
             p.cen[me] = d.centrality;
         }
     }
+    if constexpr (RES && FLY) {
+        // Fused flight: the chunk's decisions are in rec (visible to the whole
+        // workgroup after the barrier); every bird of the chunk flies into
+        // `flown`, so the walks of other workgroups keep reading the positions
+        // of this step from rec. A separate instance: the flight's code would
+        // otherwise cost the G = 1-2 walks a wave per SIMD.
+        __syncthreads();
+        const Seg sg0 = g.segs[0];
+        for (uint32_t b0 = start; b0 < end; b0 += BLOCK)
+            resident_flight(g, sg0, rec, flown, w, b0 + threadIdx.x, b0 + threadIdx.x < end);
+    }
     __syncthreads();  // due_slots and n_due are reused by the next chunk
     }
 }
@@ -639,7 +653,7 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
 // once (occupancy of this instance at this LDS size, times the CUs), and no
 // more than there are chunks. The occupancy is a property of the compiled
 // kernel; it is looked up once per instance and chunk size.
-template <int BLOCK, int G, bool RES>
+template <int BLOCK, int G, bool RES, bool FLY>
 static uint32_t persistent_blocks(uint32_t chunk, uint32_t n_chunks, size_t lds) {
     // by log2(chunk / BLOCK) (1..16 birds per thread), without and with a window
     static int per_cu[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
@@ -648,7 +662,7 @@ static uint32_t persistent_blocks(uint32_t chunk, uint32_t n_chunks, size_t lds)
     int *slot = &per_cu[lds > chunk * sizeof(uint32_t) ? 1 : 0][idx];
     int occ = __atomic_load_n(slot, __ATOMIC_RELAXED);
     if (occ <= 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_decide<BLOCK, G, RES>, BLOCK, lds) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_decide<BLOCK, G, RES, FLY>, BLOCK, lds) != hipSuccess ||
             occ <= 0)
             occ = 1;
         __atomic_store_n(slot, occ, __ATOMIC_RELAXED);
@@ -662,29 +676,31 @@ static uint32_t persistent_blocks(uint32_t chunk, uint32_t n_chunks, size_t lds)
 
 // With timing events, the launch itself records the kernel's start and end
 // (hipExtLaunchKernelGGL), so the measured interval is the kernel alone.
-template <int BLOCK, int G, bool RES>
+template <int BLOCK, int G, bool RES, bool FLY>
 static void launch_decide_block(const Geometry &g, const Pools &p, const Records &rec, float time, uint32_t chunk,
-                                int walk_flags, uint32_t *due_clear, uint32_t *work_clear, const DecideTiming &tm,
-                                hipStream_t s) {
+                                int walk_flags, uint32_t *due_clear, uint32_t *work_clear, const ResWork &w,
+                                const Records &flown, const DecideTiming &tm, hipStream_t s) {
     const uint32_t n_chunks = (g.n_slots + chunk - 1) / chunk;
     const size_t lds = decide_lds_bytes(chunk, walk_flags);
     const uint32_t blocks =
-        (walk_flags & WALK_PERSIST) ? persistent_blocks<BLOCK, G, RES>(chunk, n_chunks, lds) : n_chunks;
+        (walk_flags & WALK_PERSIST) ? persistent_blocks<BLOCK, G, RES, FLY>(chunk, n_chunks, lds) : n_chunks;
     if (tm.start)
-        hipExtLaunchKernelGGL((k_decide<BLOCK, G, RES>), dim3(blocks), dim3(BLOCK), lds, s, tm.start, tm.stop, 0u, g,
-                              p, rec, time, chunk, walk_flags, due_clear, work_clear);
+        hipExtLaunchKernelGGL((k_decide<BLOCK, G, RES, FLY>), dim3(blocks), dim3(BLOCK), lds, s, tm.start, tm.stop, 0u, g,
+                              p, rec, time, chunk, walk_flags, due_clear, work_clear, w, flown);
     else
-        hipLaunchKernelGGL((k_decide<BLOCK, G, RES>), dim3(blocks), dim3(BLOCK), lds, s, g, p, rec, time, chunk,
-                           walk_flags, due_clear, work_clear);
+        hipLaunchKernelGGL((k_decide<BLOCK, G, RES, FLY>), dim3(blocks), dim3(BLOCK), lds, s, g, p, rec, time, chunk,
+                           walk_flags, due_clear, work_clear, w, flown);
 }
 
-template <int BLOCK, bool RES>
+template <int BLOCK, bool RES, bool FLY>
 static hipError_t launch_decide_group(const Geometry &g, const Pools &p, const Records &rec, float time,
                                       uint32_t chunk, uint32_t group, int walk_flags, uint32_t *due_clear,
-                                      uint32_t *work_clear, const DecideTiming &tm, hipStream_t s) {
+                                      uint32_t *work_clear, const ResWork &w, const Records &flown,
+                                      const DecideTiming &tm, hipStream_t s) {
 #define FH_DECIDE_G(GV)                                                                                    \
     case GV:                                                                                               \
-        launch_decide_block<BLOCK, GV, RES>(g, p, rec, time, chunk, walk_flags, due_clear, work_clear, tm, s); \
+        launch_decide_block<BLOCK, GV, RES, FLY>(g, p, rec, time, chunk, walk_flags, due_clear, work_clear, w, flown, tm, \
+                                            s);                                                                    \
         break;
     switch (group) {
         FH_DECIDE_G(1)
@@ -699,27 +715,24 @@ static hipError_t launch_decide_group(const Geometry &g, const Pools &p, const R
     return hipSuccess;
 }
 
-template <bool RES>
+template <bool RES, bool FLY>
 static hipError_t launch_decide_any(const Geometry &g, const Pools &p, const Records &rec, float time,
                                     uint32_t block, uint32_t birds_per_thread, uint32_t group, int walk_flags,
-                                    uint32_t *due_clear, uint32_t *work_clear, const DecideTiming &tm,
-                                    hipStream_t s) {
+                                    uint32_t *due_clear, uint32_t *work_clear, const ResWork &w,
+                                    const Records &flown, const DecideTiming &tm, hipStream_t s) {
     if (g.n_slots == 0) return hipSuccess;
     const uint32_t chunk = block * birds_per_thread;
+    auto by_block = [&](auto block_c) {
+        constexpr int B = decltype(block_c)::value;
+        return launch_decide_group<B, RES, FLY>(g, p, rec, time, chunk, group, walk_flags, due_clear, work_clear, w, flown,
+                                           tm, s);
+    };
     hipError_t e;
     switch (block) {
-        case 64:
-            e = launch_decide_group<64, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, work_clear, tm, s);
-            break;
-        case 128:
-            e = launch_decide_group<128, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, work_clear, tm, s);
-            break;
-        case 256:
-            e = launch_decide_group<256, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, work_clear, tm, s);
-            break;
-        case 512:
-            e = launch_decide_group<512, RES>(g, p, rec, time, chunk, group, walk_flags, due_clear, work_clear, tm, s);
-            break;
+        case 64: e = by_block(std::integral_constant<int, 64>{}); break;
+        case 128: e = by_block(std::integral_constant<int, 128>{}); break;
+        case 256: e = by_block(std::integral_constant<int, 256>{}); break;
+        case 512: e = by_block(std::integral_constant<int, 512>{}); break;
         default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
@@ -729,15 +742,19 @@ static hipError_t launch_decide_any(const Geometry &g, const Pools &p, const Rec
 hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
                          uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s,
                          const DecideTiming &tm) {
-    return launch_decide_any<false>(g, p, Records{}, time, block, birds_per_thread, group, walk_flags, nullptr,
-                                    nullptr, tm, s);
+    return launch_decide_any<false, false>(g, p, Records{}, time, block, birds_per_thread, group,
+                                    walk_flags & ~WALK_FLIGHT, nullptr, nullptr, ResWork{}, Records{}, tm, s);
 }
 
 hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Records &rec, uint32_t block,
                                   uint32_t birds_per_thread, uint32_t group, int walk_flags, uint32_t *due_clear,
-                                  uint32_t *work_clear, hipStream_t s, const DecideTiming &tm) {
-    return launch_decide_any<true>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear,
-                                   work_clear, tm, s);
+                                  uint32_t *work_clear, const ResWork &w, const Records &flown, hipStream_t s,
+                                  const DecideTiming &tm) {
+    if (walk_flags & WALK_FLIGHT)
+        return launch_decide_any<true, true>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear,
+                                             work_clear, w, flown, tm, s);
+    return launch_decide_any<true, false>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear,
+                                          work_clear, w, flown, tm, s);
 }
 
 hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, int walk, uint32_t *ids,
@@ -746,40 +763,44 @@ hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, i
     return hipGetLastError();
 }
 
-template <int BLOCK, bool RES>
+template <int BLOCK, bool RES, bool FLY>
 static const void *decide_fn_group(uint32_t group) {
     switch (group) {
-        case 1: return reinterpret_cast<const void *>(k_decide<BLOCK, 1, RES>);
-        case 2: return reinterpret_cast<const void *>(k_decide<BLOCK, 2, RES>);
-        case 4: return reinterpret_cast<const void *>(k_decide<BLOCK, 4, RES>);
-        case 8: return reinterpret_cast<const void *>(k_decide<BLOCK, 8, RES>);
-        case 16: return reinterpret_cast<const void *>(k_decide<BLOCK, 16, RES>);
-        case 64: return reinterpret_cast<const void *>(k_decide<BLOCK, 64, RES>);
+        case 1: return reinterpret_cast<const void *>(k_decide<BLOCK, 1, RES, FLY>);
+        case 2: return reinterpret_cast<const void *>(k_decide<BLOCK, 2, RES, FLY>);
+        case 4: return reinterpret_cast<const void *>(k_decide<BLOCK, 4, RES, FLY>);
+        case 8: return reinterpret_cast<const void *>(k_decide<BLOCK, 8, RES, FLY>);
+        case 16: return reinterpret_cast<const void *>(k_decide<BLOCK, 16, RES, FLY>);
+        case 64: return reinterpret_cast<const void *>(k_decide<BLOCK, 64, RES, FLY>);
         default: return nullptr;
     }
 }
 
-template <bool RES>
+template <bool RES, bool FLY>
 static const void *decide_fn(uint32_t block, uint32_t group) {
     switch (block) {
-        case 64: return decide_fn_group<64, RES>(group);
-        case 128: return decide_fn_group<128, RES>(group);
-        case 256: return decide_fn_group<256, RES>(group);
-        case 512: return decide_fn_group<512, RES>(group);
+        case 64: return decide_fn_group<64, RES, FLY>(group);
+        case 128: return decide_fn_group<128, RES, FLY>(group);
+        case 256: return decide_fn_group<256, RES, FLY>(group);
+        case 512: return decide_fn_group<512, RES, FLY>(group);
         default: return nullptr;
     }
 }
 
 // "decide" is the default instance (block 256, G 1, sorted copies);
-// "decide:BLOCK:G:r" or "decide:BLOCK:G:s" names the instance for that
-// workgroup size and lane group over resident records or sorted copies.
+// "decide:BLOCK:G:K" names the instance for that workgroup size and lane
+// group over sorted copies (K = s), resident records (r), or resident records
+// with the fused flight (f).
 hipError_t decide_kernel_attributes(const char *name, hipFuncAttributes *out) {
     if (!__builtin_strcmp(name, "decide"))
-        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_decide<256, 1, false>));
+        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_decide<256, 1, false, false>));
     unsigned block = 0, group = 0;
     char kind = 0;
-    if (std::sscanf(name, "decide:%u:%u:%c", &block, &group, &kind) == 3 && (kind == 'r' || kind == 's')) {
-        const void *f = kind == 'r' ? decide_fn<true>(block, group) : decide_fn<false>(block, group);
+    if (std::sscanf(name, "decide:%u:%u:%c", &block, &group, &kind) == 3) {
+        const void *f = kind == 's'   ? decide_fn<false, false>(block, group)
+                        : kind == 'r' ? decide_fn<true, false>(block, group)
+                        : kind == 'f' ? decide_fn<true, true>(block, group)
+                                      : nullptr;
         if (f) return hipFuncGetAttributes(out, f);
     }
     return hipErrorInvalidValue;

@@ -42,6 +42,7 @@ struct fh_flock {
     // the current layout, q = parity; a step moves them to the other buffers.
     void *res_arena = nullptr;
     Records R[2]{};
+    Records F{};  // fused flight: the flown positions, velocities and roll rates
     ResWork W{};
     uint32_t *off[2]{};
     uint32_t *res_due = nullptr;  // due counter of each parity, then the persistent decide's chunk counters
@@ -183,6 +184,9 @@ static fh_status alloc_resident(fh_flock *h) {
     const size_t o_knew = a.take<uint32_t>(nn), o_arank = a.take<uint32_t>(nn);
     const size_t o_mbits = a.take<uint32_t>((nn + TILE - 1) / TILE * (TILE / 32));
     const size_t o_arr = a.take<uint32_t>(nb), o_dep = a.take<uint32_t>(nb), o_abase = a.take<uint32_t>(nb);
+    const bool flight = (h->prm.flags & FH_FLAG_FLIGHT) != 0;
+    const size_t o_fp4 = flight ? a.take<float4>(nn) : 0, o_fv4 = flight ? a.take<float4>(nn) : 0;
+    const size_t o_fspin = flight ? a.take<float>(nn) : 0;
     const size_t o_off1 = a.take<uint32_t>(nb + 1), o_due = a.take<uint32_t>(4);
     hipError_t e = hipMalloc(&h->res_arena, a.used);
     if (e != hipSuccess) {
@@ -205,6 +209,11 @@ static fh_status alloc_resident(fh_flock *h) {
     h->W.dep = reinterpret_cast<uint32_t *>(base + o_dep);
     h->W.abase = reinterpret_cast<uint32_t *>(base + o_abase);
     h->W.tsum = h->P.tile_sums;  // the entry scan leaves each tile's total there
+    if (flight) {
+        h->F.p4 = reinterpret_cast<float4 *>(base + o_fp4);
+        h->F.v4 = reinterpret_cast<float4 *>(base + o_fv4);
+        h->F.spin = reinterpret_cast<float *>(base + o_fspin);
+    }
     h->off[0] = h->P.offsets;
     h->off[1] = reinterpret_cast<uint32_t *>(base + o_off1);
     h->res_due = reinterpret_cast<uint32_t *>(base + o_due);
@@ -254,15 +263,25 @@ static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
     p.offsets = h->off[q];
     p.due_count = h->res_due + q;
     p.work = h->res_due + 2 + q;
+    const bool fused = (h->prm.flags & FH_FLAG_FLIGHT) != 0;
     if (ev) FH_HIP(hipEventRecord(ev[0], h->stream));
     FH_HIP(launch_decide_resident(h->geo, p, h->R[q], h->prm.decide_block, h->prm.birds_per_thread,
                                   h->prm.decide_group, walk_flags(h->prm.flags), h->res_due + (1u - q),
-                                  h->res_due + 2 + (1u - q), h->stream,
+                                  h->res_due + 2 + (1u - q), h->W, h->F, h->stream,
                                   ev ? DecideTiming{ev[6], ev[7]} : DecideTiming{}));
     if (ev) FH_HIP(hipEventRecord(ev[1], h->stream));
-    FH_HIP(launch_resident_integrate(h->geo, p, h->R[q], h->W, h->d_seg, h->stream));
+    if (!fused) FH_HIP(launch_resident_integrate(h->geo, p, h->R[q], h->W, h->d_seg, h->stream));
     if (ev) FH_HIP(hipEventRecord(ev[2], h->stream));
-    FH_HIP(launch_resident_rebuild(h->geo, p, h->R[q], h->R[1u - q], h->W, h->off[q], h->off[1u - q], h->stream));
+    // the move reads the flown positions, velocities and roll rates from F
+    // when the flight was fused, everything else from the current records
+    Records src = h->R[q];
+    if (fused) {
+        src.p4 = h->F.p4;
+        src.v4 = h->F.v4;
+        src.spin = h->F.spin;
+    }
+    FH_HIP(launch_resident_rebuild(h->geo, p, src, h->R[1u - q], h->W, h->off[q], h->off[1u - q],
+                                   fused ? h->d_seg : nullptr, h->stream));
     if (ev)
         for (int k = 3; k < 6; ++k) FH_HIP(hipEventRecord(ev[k], h->stream));
     return FH_OK;

@@ -449,14 +449,15 @@ __global__ __launch_bounds__(TILE) void k_r_flush(Geometry g, Pools p, Records r
 
 // Rebuild scan (one launch, at most SCAN_MERGE_TILES tiles): counts +=
 // arrivals - departures; the new offsets, from the tiles' totals that the
-// integrate pass kept current with its movers; the first arrival slot of each
-// bucket (its stayers come first); the occupancy bits. Arrivals and
-// departures are zeroed for the next step.
+// flight kept current with its movers; the first arrival slot of each bucket
+// (its stayers come first); the occupancy bits. Arrivals and departures are
+// zeroed for the next step. With `tick`, workgroup 0 commits the step's clock.
 __global__ __launch_bounds__(256) void k_r_scan_apply(uint32_t *__restrict__ counts,
                                                       uint32_t *__restrict__ arr, uint32_t *__restrict__ dep,
                                                       const uint32_t *__restrict__ tile_sums, uint32_t ntiles,
                                                       uint32_t *__restrict__ noff, uint32_t *__restrict__ abase,
-                                                      uint32_t *__restrict__ occ) {
+                                                      uint32_t *__restrict__ occ, Seg *tick) {
+    if (tick && blockIdx.x == 0 && threadIdx.x == 0) tick->time = tick->time + DT;
     uint32_t part = 0;
     for (uint32_t t = threadIdx.x; t < blockIdx.x; t += 256u) part += tile_sums[t];
     uint32_t prefix;
@@ -564,11 +565,12 @@ hipError_t launch_resident_flush(const Geometry &g, const Pools &p, const Record
 }
 
 hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
-                                   const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, hipStream_t s) {
+                                   const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, Seg *tick,
+                                   hipStream_t s) {
     const uint32_t ntiles = (g.n_buckets + SCAN_TILE - 1) / SCAN_TILE;
     if (ntiles == 0 || ntiles > SCAN_MERGE_TILES) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_r_scan_apply, dim3(ntiles), dim3(256), 0, s, p.counts, w.arr, w.dep, w.tsum, ntiles,
-                       off_nxt, w.abase, p.occ);
+                       off_nxt, w.abase, p.occ, tick);
     if (g.n_slots)
         hipLaunchKernelGGL(k_r_move, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, cur, nxt, w,
                            off_cur, off_nxt);

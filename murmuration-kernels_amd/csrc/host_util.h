@@ -46,7 +46,7 @@ inline uint32_t flock_buckets(uint32_t n, const fh_params &p) {
 inline int walk_flags(uint32_t flags) {
     return ((flags & FH_FLAG_NO_TILE) ? 0 : 1) | ((flags & FH_FLAG_OCC) ? 2 : 0) |
            ((flags & FH_FLAG_PERSISTENT) ? 4 : 0) | ((flags & FH_FLAG_XCD) ? 8 : 0) |
-           ((flags & FH_FLAG_WINDOW) ? 16 : 0);
+           ((flags & FH_FLAG_WINDOW) ? 16 : 0) | ((flags & FH_FLAG_FLIGHT) ? 32 : 0);
 }
 
 // .flk record words (crates/flock-ref/src/state.rs:4-7).

@@ -2,6 +2,7 @@
 // model.rs:149-176, cuda/flock.cu:293-328), optionally fused with the next
 // step's grid_histogram so the new position is hashed while it is still in
 // registers instead of being read back by a separate pass.
+#include "flight.h"
 #include "layout.h"
 #include "model.h"
 
@@ -25,46 +26,6 @@ __device__ __forceinline__ uint32_t count_key_i(uint32_t *counts, uint32_t k, bo
         left &= ~grp;
     }
     return rank;
-}
-
-// Banked flight of one bird (model.rs:149-176): new (pos, next), (vel, roll)
-// and the roll rate, from its (pos, next), (vel, roll) and (steering, radius).
-__device__ __forceinline__ void integrate_bird(float4 &p4, float4 &v4, const float4 s4, float &rate_out) {
-    f3 vel = mk3(v4.x, v4.y, v4.z);
-    const float roll = v4.w;
-    const f3 steering = mk3(s4.x, s4.y, s4.z);
-
-    const f3 fwd = forward_of(vel);
-    const f3 side = side_of(fwd);
-    const float speed = fmaxf(sqrtf(dot3(vel, vel)), 1e-6f);
-
-    const float a_lat = dot3(steering, side) / MASS;
-    const float target = clampf_(atanf(a_lat / G), -MAX_BANK, MAX_BANK);
-    const float rate = clampf_((target - roll) / TAU_ROLL, -MAX_ROLL_RATE, MAX_ROLL_RATE);
-    const float beta = clampf_(roll + rate * DT, -MAX_BANK, MAX_BANK);
-
-    // body_up (model.rs:34-38): (side x f) cos b + side sin b
-    const f3 up0 = cross3(side, fwd);
-    const float cb = cosf(beta);
-    const f3 up = up0 * cb + side * sinf(beta);
-    const float cos_beta = fmaxf(cb, 0.2f);
-    const float sr = speed / V0;
-    const float lift = sr * sr * L0 / cos_beta;
-    const float drag = sr * sr * D0;
-    const f3 flight = ((up * lift - fwd * drag) + fwd * D0) - (mk3(0.f, 0.f, 1.f) * MASS) * G;
-
-    const f3 control = steering - side * dot3(steering, side);
-    vel = vel + ((control + flight) / MASS) * DT;
-    const float sp = sqrtf(dot3(vel, vel));
-    if (sp > V_MAX)
-        vel = vel * (V_MAX / sp);
-    else if (sp < V_MIN && sp > 1e-6f)
-        vel = vel * (V_MIN / sp);
-    const f3 pos = mk3(p4.x, p4.y, p4.z) + vel * DT;
-
-    p4 = make_float4(pos.x, pos.y, pos.z, p4.w);
-    v4 = make_float4(vel.x, vel.y, vel.z, beta);
-    rate_out = rate;
 }
 
 template <bool HASH, bool WAVE_AGG>
@@ -96,43 +57,13 @@ __global__ __launch_bounds__(TILE) void k_integrate(Geometry g, Pools p) {
     }
 }
 
-// Banked flight over resident records (slot order, one flock). A bird whose
-// new cell falls in another bucket is a mover: it counts itself out of its
-// old bucket and into its new one (keeping its arrival rank), and out of and
-// into the scan tiles' totals when the tile changes; every slot records its
-// new bucket and its mover bit. Workgroup 0 commits the step's clock (segment
-// time += DT, the time decide used).
+// Banked flight over resident records (slot order, one flock), in place;
+// see resident_flight. Workgroup 0 commits the step's clock (segment time +=
+// DT, the time decide used).
 __global__ __launch_bounds__(TILE) void k_r_integrate(Geometry g, Records rec, ResWork w, Seg *seg) {
     const uint32_t s = blockIdx.x * TILE + threadIdx.x;
-    const bool active = s < g.n_slots;
     const Seg sg = g.segs[0];
-    bool mover = false;
-    if (active) {
-        float4 p4 = rec.p4[s];
-        float4 v4 = rec.v4[s];
-        float rate;
-        integrate_bird(p4, v4, rec.s4[s], rate);
-        rec.p4[s] = p4;
-        rec.v4[s] = v4;
-        rec.spin[s] = rate;
-        const uint32_t kn = bucket_of_cell(sg, cell_coord(p4.x, sg.inv_cell), cell_coord(p4.y, sg.inv_cell),
-                                           cell_coord(p4.z, sg.inv_cell));
-        const uint32_t ko = rec.key[s];
-        w.knew[s] = kn;
-        if (kn != ko) {
-            mover = true;
-            w.arank[s] = atomicAdd(&w.arr[kn], 1u);
-            atomicAdd(&w.dep[ko], 1u);
-            if (kn / SCAN_TILE != ko / SCAN_TILE) {
-                atomicAdd(&w.tsum[kn / SCAN_TILE], 1u);
-                atomicSub(&w.tsum[ko / SCAN_TILE], 1u);
-            }
-        }
-    }
-    const uint64_t bits = __ballot(mover);
-    const uint32_t lane = threadIdx.x & 63u;
-    if (lane == 0u) w.mbits[s >> 5] = (uint32_t)bits;  // s: first slot of this wave
-    if (lane == 32u) w.mbits[s >> 5] = (uint32_t)(bits >> 32);
+    resident_flight(g, sg, rec, rec, w, s, s < g.n_slots);
     if (blockIdx.x == 0 && threadIdx.x == 0) seg->time = sg.time + DT;
 }
 

@@ -147,6 +147,7 @@ constexpr int WALK_OCC = 2;    // skip empty buckets by the occupancy bitmap
 constexpr int WALK_PERSIST = 4;  // a fixed grid of workgroups takes chunks from a counter
 constexpr int WALK_XCD = 8;      // each XCD's workgroups take one contiguous run of chunks
 constexpr int WALK_WINDOW = 16;  // stage the positions of the chunk and a halo in LDS
+constexpr int WALK_FLIGHT = 32;  // resident: fly each chunk's birds right after deciding them
 // decide's LDS window: the chunk and a halo of half a chunk on each side, at
 // most WINDOW_MAX slots and within DECIDE_LDS_MAX bytes of dynamic LDS together
 // with the chunk's due-slot list (32 bytes per slot: position and velocity).
@@ -180,20 +181,27 @@ hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t
                          const DecideTiming &tm = DecideTiming{});
 // decide over resident records (p.offsets: the current offsets). Adds the
 // due birds to *p.due_count, takes chunks from *p.work (persistent), and
-// zeroes *due_clear and *work_clear (the other step's counters).
+// zeroes *due_clear and *work_clear (the other step's counters). With
+// WALK_FLIGHT, each chunk's birds then fly (resident_flight) from rec into
+// `flown` (positions, velocities, roll rates) with the mover accounting in w;
+// rec itself keeps the positions every walk reads.
 hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Records &rec, uint32_t block,
                                   uint32_t birds_per_thread, uint32_t group, int walk_flags, uint32_t *due_clear,
-                                  uint32_t *work_clear, hipStream_t s, const DecideTiming &tm = DecideTiming{});
+                                  uint32_t *work_clear, const ResWork &w, const Records &flown, hipStream_t s,
+                                  const DecideTiming &tm = DecideTiming{});
 // Resident passes (one flock): records from the bird-order state and a grid
 // built by hash + scan (counts kept); banked flight in slot order with the
 // movers counted (and the flock clock committed: += DT); the rebuild of the
 // grid and the move of every record to its slot in `nxt`; records back to the
-// bird-order state.
+// bird-order state. The rebuild reads the records to move from `cur`; when
+// the flight was fused into decide, `tick` is the flock's segment and the
+// rebuild commits its clock (+= DT) instead of the integrate pass.
 hipError_t launch_resident_enter(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s);
 hipError_t launch_resident_integrate(const Geometry &g, const Pools &p, const Records &rec, const ResWork &w,
                                      Seg *seg, hipStream_t s);
 hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
-                                   const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, hipStream_t s);
+                                   const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, Seg *tick,
+                                   hipStream_t s);
 hipError_t launch_resident_flush(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s);
 hipError_t launch_integrate(const Geometry &g, const Pools &p, bool hash, bool wave_agg,
                             hipStream_t s);

@@ -31,6 +31,7 @@ FLAG_SPATIAL = 128
 FLAG_PERSISTENT = 256
 FLAG_XCD = 512
 FLAG_WINDOW = 1024
+FLAG_FLIGHT = 2048
 
 OK = 0
 ERR_INVALID, ERR_FLK, ERR_NOMEM, ERR_HIP, ERR_BUDGET, ERR_STATE, ERR_IO, ERR_NODEVICE = range(1, 9)

@@ -6,7 +6,7 @@ workgroup size, the birds each decide workgroup scans per lane
 whether decide tests the bucket occupancy bitmap, runs as a persistent grid
 or maps chunks XCD by XCD, whether it stages each chunk's positions in LDS,
 whether cells are bucketed by the spatial hash and
-(single flocks) whether fh_step keeps the state in bucket order, by timing
+(single flocks) whether fh_step keeps the state in bucket order and flies each chunk inside decide, by timing
 short runs of the real workload. Each candidate is timed over `steps` steps
 after `warmup` untimed ones; the fastest few are timed again and the lowest
 median kept. Results can be cached in a JSON file keyed by
@@ -26,8 +26,8 @@ import time
 
 import numpy as np
 
-from . import (FLAG_CANONICAL, FLAG_GRAPH, FLAG_OCC, FLAG_PERSISTENT, FLAG_RESIDENT, FLAG_SPATIAL, FLAG_WINDOW,
-               FLAG_XCD, Batch, Flock, device_info)
+from . import (FLAG_CANONICAL, FLAG_FLIGHT, FLAG_GRAPH, FLAG_OCC, FLAG_PERSISTENT, FLAG_RESIDENT, FLAG_SPATIAL,
+               FLAG_WINDOW, FLAG_XCD, Batch, Flock, device_info)
 
 BPT = (1, 2, 4, 8, 16)
 BLOCKS = (64, 128, 256, 512)
@@ -36,11 +36,11 @@ CELLS = (4.0, 3.0, 5.0, 6.0, 8.0, 10.0, 12.0)
 SHIFTS = (0, 1, 2, 3, 4)
 # on/off tunables and the flag each one sets
 SWITCHES = {"occ": FLAG_OCC, "resident": FLAG_RESIDENT, "spatial": FLAG_SPATIAL, "persist": FLAG_PERSISTENT,
-            "xcd": FLAG_XCD, "window": FLAG_WINDOW}
+            "xcd": FLAG_XCD, "window": FLAG_WINDOW, "flight": FLAG_FLIGHT}
 # Part of every cache key: raised whenever the searched space or the kernels'
 # costs change, so an entry found for other kernels or a smaller space is not
 # reused.
-SEARCH_VERSION = 6
+SEARCH_VERSION = 7
 FINALISTS = 6
 CONFIRM = 3
 
@@ -122,7 +122,7 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
         return _time_plain(starts[0], times[0], prm, f, warmup, steps)
 
     best = {"cell": float(cells[0]), "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0,
-            "occ": 0, "resident": 0, "spatial": 0, "persist": 0, "xcd": 0, "window": 0}
+            "occ": 0, "resident": 0, "spatial": 0, "persist": 0, "xcd": 0, "window": 0, "flight": 0}
     trials = []
     seen = {}
 
@@ -159,8 +159,10 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
             sweep([nv])
     # the resident layout is a single-flock fh_step mode, not for canonical order
     resident = (0, 1) if kind == "plain" and not (flags & FLAG_CANONICAL) else (0,)
+    # the fused flight exists only in the resident layout
     for nv in (("resident", resident), ("spatial", spatial), ("cell", cells), ("bucket_shift", shifts),
-               ("occ", (0, 1)), ("persist", (0, 1)), ("xcd", (0, 1)), ("window", (0, 1)), ("cell", cells)):
+               ("occ", (0, 1)), ("persist", (0, 1)), ("xcd", (0, 1)), ("window", (0, 1)), ("flight", resident),
+               ("cell", cells)):
         sweep([nv])
     if thorough:
         sweep(shape[:2])

@@ -23,7 +23,7 @@ def rel_err(a, b):
     return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b))))
 
 
-@pytest.mark.parametrize("flags", [0, 1, 4, 64 | 4, 128 | 4, 512 | 256, 1024 | 128 | 64 | 4])
+@pytest.mark.parametrize("flags", [0, 1, 4, 64 | 4, 128 | 4, 512 | 256, 1024 | 128 | 64 | 4, 2048 | 64 | 4])
 def test_one_step_matches_the_fixture(fh, flags):
     # one step: neighbour counts exact, positions and velocities within 1e-4
     start, _ = load(fh, "s2000_seed1.flk")
@@ -38,7 +38,7 @@ def test_one_step_matches_the_fixture(fh, flags):
     assert rel_err(got[:, 0:6], want[:, 0:6]) <= 1e-4
 
 
-@pytest.mark.parametrize("flags", [0, 64 | 4])
+@pytest.mark.parametrize("flags", [0, 64 | 4, 2048 | 64 | 4])
 def test_twenty_steps_stay_near_the_fixture(fh, flags):
     start, _ = load(fh, "s2000_seed1.flk")
     want, tw = load(fh, "s2000_seed1_step20.flk")

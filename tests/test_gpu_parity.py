@@ -400,7 +400,7 @@ def test_decide_instances_fit_their_register_budget(fh):
     # (at most 96 VGPRs of the 512 per lane) without spilling to scratch.
     for block in (64, 256, 512):
         for group in (4, 8, 16, 64):
-            for kind in "rs":
+            for kind in "rsf":
                 k = fh.kernel_info(f"decide:{block}:{group}:{kind}")
                 assert 0 < k["vgprs"] <= 96 and k["scratch_bytes"] == 0, (block, group, kind, k)
     with pytest.raises(fh.FlockHipError):

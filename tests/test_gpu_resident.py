@@ -33,8 +33,10 @@ def per_bucket_sets_equal(g_sorted, offsets, o_sorted):
     return np.array_equal(g_sorted[offsets[:-1][single]], o_sorted[offsets[:-1][single]])
 
 
-@pytest.mark.parametrize("n,kind,cell", [(200_000, "cube", 4.0), (50_000, "ref", 4.0), (300_000, "clumps", 2.0)])
-def test_resident_grid_is_the_reference_grid(fh, fr, n, kind, cell):
+@pytest.mark.parametrize("n,kind,cell,extra", [(200_000, "cube", 4.0, 0), (50_000, "ref", 4.0, 0),
+                                               (300_000, "clumps", 2.0, 0), (200_000, "cube", 4.0, 2048),
+                                               (300_000, "clumps", 2.0, 2048)])
+def test_resident_grid_is_the_reference_grid(fh, fr, n, kind, cell, extra):
     # After every run of resident steps the layout is the counting sort of the
     # current positions; downloading leaves the layout and the next step
     # enters it again.
@@ -44,7 +46,7 @@ def test_resident_grid_is_the_reference_grid(fh, fr, n, kind, cell):
         w0 = fh.gen_reference(n, 3)
     else:
         w0 = fh.gen_clumps(n, 4, 4.0, 200.0, 3)
-    with fh.Flock.from_words(w0, cell=cell, flags=fh.FLAG_RESIDENT | fh.FLAG_GRAPH) as f:
+    with fh.Flock.from_words(w0, cell=cell, flags=fh.FLAG_RESIDENT | fh.FLAG_GRAPH | extra) as f:
         for k in range(3):
             f.step(5 + 3 * k)
             g = f.read_grid()
@@ -59,7 +61,9 @@ def test_resident_grid_is_the_reference_grid(fh, fr, n, kind, cell):
 
 @pytest.mark.parametrize("flags,group", [(64, 1), (64 | 4, 4), (64 | 4 | 32, 2), (64 | 4, 16), (64 | 4, 64),
                                          (64 | 4 | 256, 4), (64 | 256, 1), (64 | 4 | 128 | 512, 4),
-                                         (64 | 4 | 128 | 512 | 1024, 4), (64 | 1024, 1), (64 | 4 | 1024 | 256, 16)])
+                                         (64 | 4 | 128 | 512 | 1024, 4), (64 | 1024, 1), (64 | 4 | 1024 | 256, 16),
+                                         (64 | 4 | 2048, 2), (64 | 2048 | 128 | 1024, 4), (64 | 2048 | 256, 64),
+                                         (64 | 4 | 2048 | 512, 1)])
 def test_resident_step_matches_the_oracle(fh, fr, flags, group):
     w0 = cube(fh, 200_000, 5)
     with fh.Flock.from_words(w0, flags=flags, decide_group=group) as f:
@@ -169,6 +173,29 @@ def test_resident_persistent_after_leaving_at_odd_parity(fh):
     # decision clocks (advanced only by decide) must agree exactly
     assert np.array_equal(outs[0][:, 12], outs[1][:, 12])
     assert rel_err(outs[0][:, 0:6], outs[1][:, 0:6]) <= 1e-5
+
+
+@pytest.mark.parametrize("extra", [0, 1024 | 128, 256 | 512])
+def test_fused_flight_equals_the_separate_pass(fh, extra):
+    # The fused flight computes the same new state per bird as the integrate
+    # pass; slots of arrivals follow atomic order in both, so positions agree
+    # to rounding of reordered sums and decision clocks exactly.
+    w0 = cube(fh, 300_000, 21)
+    outs = []
+    for fl in (fh.FLAG_FLIGHT, 0):
+        with fh.Flock.from_words(w0, flags=fh.FLAG_RESIDENT | extra | fl) as f:  # per-pass events: no graph
+            f.step(6)
+            f.download()
+            f.step(9)
+            ms = f.step(3, 0, timing=True)
+            outs.append((f.download(), ms))
+    ((a, ta), ma), ((b, tb), mb) = outs
+    assert ta == tb
+    assert np.array_equal(a[:, 12], b[:, 12])
+    assert rel_err(a[:, 0:6], b[:, 0:6]) <= 1e-5
+    assert np.mean(a[:, 13].view(np.uint32) == b[:, 13].view(np.uint32)) > 0.999
+    assert ma["decide"] > 0 and mb["integrate"] > 0
+    assert ma["due_last"] == mb["due_last"]
 
 
 def test_resident_upload_replaces_the_records(fh):
