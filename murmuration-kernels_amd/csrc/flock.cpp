@@ -49,6 +49,8 @@ struct fh_flock {
     bool resident = false;
     uint32_t parity = 0;
     hipGraphExec_t rgraph[2]{};
+    hipGraphExec_t graph_run[2]{};   // GRAPH_RUNS[i] fused steps from the Hashed stage
+    hipGraphExec_t rgraph_run[2]{};  // GRAPH_RUNS[i] resident steps from parity 0
 };
 
 // Events per timed step with per-pass timing: six on the stream between the
@@ -287,6 +289,61 @@ static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
     return FH_OK;
 }
 
+// Replaying a graph costs a gap of several microseconds before its first
+// kernel (none between the kernels inside it), so fh_step replays runs of
+// GRAPH_RUNS[i] steps as one graph where it can, longest first. Even, so that
+// a resident run ends at the buffer parity it started from.
+constexpr uint32_t GRAPH_RUNS[2] = {16, 4};
+
+// Captures `body` (launches on h->stream) into *out, once.
+template <class Body>
+static fh_status capture_once(fh_flock *h, hipGraphExec_t *out, Body &&body) {
+    if (*out) return FH_OK;
+    hipGraph_t g = nullptr;
+    FH_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    fh_status st = body();
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    return FH_OK;
+}
+
+// GRAPH_RUNS[i] resident steps from parity 0 (the parity is restored).
+static fh_status capture_resident_run(fh_flock *h, int i) {
+    return capture_once(h, &h->rgraph_run[i], [&]() -> fh_status {
+        const uint32_t q0 = h->parity;
+        fh_status st = FH_OK;
+        for (uint32_t k = 0; k < GRAPH_RUNS[i] && !st; ++k) {
+            st = resident_step_launches(h, nullptr);
+            h->parity ^= 1u;
+        }
+        h->parity = q0;
+        return st;
+    });
+}
+
+// GRAPH_RUNS[i] fused steps from the Hashed stage (stage and counts state restored).
+static fh_status capture_graph_run(fh_flock *h, int i) {
+    return capture_once(h, &h->graph_run[i], [&]() -> fh_status {
+        const Stage stage0 = h->stage;
+        const bool dirty0 = h->counts_dirty;
+        fh_status st = FH_OK;
+        for (uint32_t k = 0; k < GRAPH_RUNS[i] && !st; ++k) st = step_launches(h, true, nullptr);
+        h->stage = stage0;
+        h->counts_dirty = dirty0;
+        return st;
+    });
+}
+
 static fh_status capture_resident_graph(fh_flock *h) {
     const uint32_t q = h->parity;
     if (h->rgraph[q]) return FH_OK;
@@ -413,6 +470,10 @@ fh_status fh_destroy(fh_flock *h) {
     if (!h) return FH_OK;
     (void)hipStreamSynchronize(h->stream);
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
+    for (int i = 0; i < 2; ++i) {
+        if (h->graph_run[i]) (void)hipGraphExecDestroy(h->graph_run[i]);
+        if (h->rgraph_run[i]) (void)hipGraphExecDestroy(h->rgraph_run[i]);
+    }
     for (auto &gx : h->rgraph)
         if (gx) (void)hipGraphExecDestroy(gx);
     if (h->res_arena) (void)hipFree(h->res_arena);
@@ -600,6 +661,30 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
             FH_HIP(hipStreamSynchronize(h->stream));
             t0 = std::chrono::steady_clock::now();
             if (out && graph) FH_HIP(hipEventRecord(ev[0], h->stream));
+        }
+        // A run of GRAPH_RUNS[i] steps as one graph, never across the warm-up
+        // boundary (the timed steps start on a step of their own).
+        const uint32_t seg_end = s < warmup ? warmup : steps;
+        int run = -1;
+        if (graph && (!res || h->parity == 0))
+            for (int i = 0; i < 2 && run < 0; ++i)
+                if (seg_end - s >= GRAPH_RUNS[i]) run = i;
+        if (run >= 0) {
+            if (!res && h->stage != Stage::Hashed && (st = do_hash(h))) break;
+            if ((st = res ? capture_resident_run(h, run) : capture_graph_run(h, run))) break;
+            hipError_t e = hipGraphLaunch(res ? h->rgraph_run[run] : h->graph_run[run], h->stream);
+            if (e != hipSuccess) {
+                st = hip_fail(e, "hipGraphLaunch");
+                break;
+            }
+            for (uint32_t k = 0; k < GRAPH_RUNS[run]; ++k) h->time += DT;  // as one step at a time
+            if (!res) {
+                h->stage = Stage::Hashed;
+                h->counts_dirty = true;
+                h->absolute_due = false;
+            }
+            s += GRAPH_RUNS[run] - 1;
+            continue;
         }
         h->time += DT;  // host f32 clock (cuda/flock.cu:455); the device adds DT the same way
         if (res) {

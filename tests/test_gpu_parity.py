@@ -378,6 +378,44 @@ def test_graph_and_unfused_replays_match_eager(fh):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
 
 
+@pytest.mark.parametrize("plan", [[40], [16, 16], [17, 33, 5], [3, 20]])
+def test_graph_runs_match_eager(fh, plan):
+    # fh_step replays runs of 16 steps as one graph (never across the
+    # warm-up boundary) and single-step graphs for the rest: every split of
+    # the same steps gives the eager state bit for bit (canonical order).
+    w0 = cube(fh, 40_000, 6)
+    out = {}
+    for name, flags in [("eager", 1), ("graph", 1 | 4)]:
+        with fh.Flock.from_words(w0, flags=flags) as f:
+            for k, n in enumerate(plan):
+                f.step(n, warmup=n // 3 if k == 0 else 0, timing=(k == 0))
+            out[name] = f.download()
+    (a, ta), (b, tb) = out["eager"], out["graph"]
+    assert ta == tb
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("plan", [[40], [1, 32, 7], [17, 16]])
+def test_resident_graph_runs_match_eager(fh, plan):
+    # Resident runs start at buffer parity 0: after an odd number of steps a
+    # single step comes first. Slots of arrivals follow atomic order, so sums
+    # are reordered from run to run: the check is the decision clocks exactly
+    # and positions to 1e-4 after 40 steps (rounding drift, about 1e-5).
+    w0 = cube(fh, 40_000, 7)
+    out = {}
+    for name, flags in [("eager", 64), ("graph", 64 | 4), ("fused", 64 | 4 | 2048)]:
+        with fh.Flock.from_words(w0, flags=flags) as f:
+            for n in plan:
+                f.step(n)
+            out[name] = f.download()
+    a, ta = out["eager"]
+    for name in ("graph", "fused"):
+        b, tb = out[name]
+        assert ta == tb, name
+        assert np.array_equal(a[:, 12], b[:, 12]), name
+        assert rel_err(b[:, 0:6], a[:, 0:6]) <= 1e-4, name
+
+
 def test_pass_order_is_enforced(fh):
     with fh.Flock.from_words(cube(fh, 100)) as f:
         with pytest.raises(fh.FlockHipError) as e:
