@@ -74,9 +74,9 @@ __device__ __forceinline__ void resident_flight(const Geometry &g, const Seg &sg
             mover = true;
             w.arank[s] = atomicAdd(&w.arr[kn], 1u);
             atomicAdd(&w.dep[ko], 1u);
-            if (kn / SCAN_TILE != ko / SCAN_TILE) {
-                atomicAdd(&w.tsum[kn / SCAN_TILE], 1u);
-                atomicSub(&w.tsum[ko / SCAN_TILE], 1u);
+            if (kn / RES_TILE != ko / RES_TILE) {
+                atomicAdd(&w.tsum[kn / RES_TILE], 1u);
+                atomicSub(&w.tsum[ko / RES_TILE], 1u);
             }
         }
     }

@@ -190,6 +190,7 @@ static fh_status alloc_resident(fh_flock *h) {
     const size_t o_fp4 = flight ? a.take<float4>(nn) : 0, o_fv4 = flight ? a.take<float4>(nn) : 0;
     const size_t o_fspin = flight ? a.take<float>(nn) : 0;
     const size_t o_off1 = a.take<uint32_t>(nb + 1), o_due = a.take<uint32_t>(4);
+    const size_t o_tsum = a.take<uint32_t>(nb / RES_TILE);
     hipError_t e = hipMalloc(&h->res_arena, a.used);
     if (e != hipSuccess) {
         h->res_arena = nullptr;
@@ -210,7 +211,7 @@ static fh_status alloc_resident(fh_flock *h) {
     h->W.arr = reinterpret_cast<uint32_t *>(base + o_arr);
     h->W.dep = reinterpret_cast<uint32_t *>(base + o_dep);
     h->W.abase = reinterpret_cast<uint32_t *>(base + o_abase);
-    h->W.tsum = h->P.tile_sums;  // the entry scan leaves each tile's total there
+    h->W.tsum = reinterpret_cast<uint32_t *>(base + o_tsum);
     if (flight) {
         h->F.p4 = reinterpret_cast<float4 *>(base + o_fp4);
         h->F.v4 = reinterpret_cast<float4 *>(base + o_fv4);
@@ -236,6 +237,7 @@ static fh_status enter_resident(fh_flock *h) {
     p.offsets = h->off[0];
     FH_HIP(launch_scan(h->geo, p, nullptr, 0u, 0.0f, h->stream, true));
     FH_HIP(launch_resident_enter(h->geo, p, h->R[0], h->stream));
+    FH_HIP(launch_resident_tile_sums(h->geo, p, h->W, h->stream));
     // parity 0's counters may hold a step taken before the last leave
     FH_HIP(launch_fill(h->res_due, 4, 0u, h->stream));
     h->parity = 0;
@@ -384,10 +386,6 @@ fh_status fh_create(uint32_t n, const fh_params *params, fh_flock **out) {
     h->buckets = flock_buckets(n, prm);
     const uint32_t ntiles = (h->buckets + SCAN_TILE - 1) / SCAN_TILE;
     h->padded_buckets = ntiles * SCAN_TILE;
-    if ((prm.flags & FH_FLAG_RESIDENT) && ntiles > 2048u) {
-        delete h;
-        return set_error(FH_ERR_INVALID, "fh_create: FH_FLAG_RESIDENT supports at most 2048 * 4096 buckets");
-    }
 
     Arena a;
     const size_t nn = n ? n : 1;

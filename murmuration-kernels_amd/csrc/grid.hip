@@ -447,64 +447,61 @@ __global__ __launch_bounds__(TILE) void k_r_flush(Geometry g, Pools p, Records r
     p.key[b] = rec.key[s];
 }
 
-// Rebuild scan (one launch, at most SCAN_MERGE_TILES tiles): counts +=
-// arrivals - departures; the new offsets, from the tiles' totals that the
-// flight kept current with its movers; the first arrival slot of each bucket
-// (its stayers come first); the occupancy bits. Arrivals and departures are
-// zeroed for the next step. With `tick`, workgroup 0 commits the step's clock.
+// Each resident scan tile's count total (RES_TILE buckets per workgroup).
+__global__ __launch_bounds__(256) void k_r_tile_sums(const uint32_t *__restrict__ counts, uint32_t *__restrict__ tsum) {
+    const uint4 c = reinterpret_cast<const uint4 *>(counts + (size_t)blockIdx.x * RES_TILE)[threadIdx.x];
+    uint32_t total;
+    block_excl_scan<4>(c.x + c.y + c.z + c.w, &total);
+    if (threadIdx.x == 0) tsum[blockIdx.x] = total;
+}
+
+// Rebuild scan (one launch; RES_TILE buckets per workgroup, 4 per lane):
+// counts += arrivals - departures; the new offsets, from the tiles' totals
+// that the flight kept current with its movers; the first arrival slot of
+// each bucket (its stayers come first); the occupancy bits. Arrivals and
+// departures are zeroed for the next step. With `tick`, workgroup 0 commits
+// the step's clock.
 __global__ __launch_bounds__(256) void k_r_scan_apply(uint32_t *__restrict__ counts,
                                                       uint32_t *__restrict__ arr, uint32_t *__restrict__ dep,
-                                                      const uint32_t *__restrict__ tile_sums, uint32_t ntiles,
+                                                      const uint32_t *__restrict__ tsum, uint32_t ntiles,
                                                       uint32_t *__restrict__ noff, uint32_t *__restrict__ abase,
                                                       uint32_t *__restrict__ occ, Seg *tick) {
     if (tick && blockIdx.x == 0 && threadIdx.x == 0) tick->time = tick->time + DT;
     uint32_t part = 0;
-    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += 256u) part += tile_sums[t];
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += 256u) part += tsum[t];
     uint32_t prefix;
     block_excl_scan<4>(part, &prefix);
-    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * 16u;
+    const size_t base = (size_t)blockIdx.x * RES_TILE + (size_t)threadIdx.x * 4u;
     uint4 *c4 = reinterpret_cast<uint4 *>(counts + base);
     uint4 *a4 = reinterpret_cast<uint4 *>(arr + base);
     uint4 *d4 = reinterpret_cast<uint4 *>(dep + base);
-    uint32_t c[16], a[16];
+    const uint4 v = *c4, w = *a4, d = *d4;
+    const uint32_t c[4] = {v.x - d.x + w.x, v.y - d.y + w.y, v.z - d.z + w.z, v.w - d.w + w.w};
+    const uint32_t a[4] = {w.x, w.y, w.z, w.w};
+    *c4 = make_uint4(c[0], c[1], c[2], c[3]);
+    uint32_t bits = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) bits |= (c[k] != 0u ? 1u : 0u) << k;
+    // one occupancy word per 8 lanes (32 buckets)
+    bits <<= 4u * (threadIdx.x & 7u);
+    bits |= __shfl_xor(bits, 1);
+    bits |= __shfl_xor(bits, 2);
+    bits |= __shfl_xor(bits, 4);
+    if ((threadIdx.x & 7u) == 0u) occ[base / 32u] = bits;
+    uint32_t total;
+    uint32_t run = prefix + block_excl_scan<4>(c[0] + c[1] + c[2] + c[3], &total);
+    if (blockIdx.x == ntiles - 1u && threadIdx.x == 0) noff[(size_t)ntiles * RES_TILE] = prefix + total;
+    uint32_t o[4], ab[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const uint4 v = c4[k], w = a4[k], d = d4[k];
-        c[4 * k] = v.x - d.x + w.x;
-        c[4 * k + 1] = v.y - d.y + w.y;
-        c[4 * k + 2] = v.z - d.z + w.z;
-        c[4 * k + 3] = v.w - d.w + w.w;
-        a[4 * k] = w.x; a[4 * k + 1] = w.y; a[4 * k + 2] = w.z; a[4 * k + 3] = w.w;
-        c4[k] = make_uint4(c[4 * k], c[4 * k + 1], c[4 * k + 2], c[4 * k + 3]);
-    }
-    uint32_t sum = 0, bits = 0;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        sum += c[k];
-        bits |= (c[k] != 0u ? 1u : 0u) << k;
-    }
-    const uint32_t hi = __shfl_xor(bits, 1);
-    if ((threadIdx.x & 1u) == 0u) occ[base / 32u] = bits | (hi << 16);
-    uint32_t total;
-    uint32_t run = prefix + block_excl_scan<4>(sum, &total);
-    if (blockIdx.x == ntiles - 1u && threadIdx.x == 0) noff[(size_t)ntiles * SCAN_TILE] = prefix + total;
-    uint32_t o[16], ab[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
         o[k] = run;
         ab[k] = run + c[k] - a[k];
         run += c[k];
     }
-    uint4 *o4 = reinterpret_cast<uint4 *>(noff + base);
-    uint4 *b4 = reinterpret_cast<uint4 *>(abase + base);
-    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        o4[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
-        b4[k] = make_uint4(ab[4 * k], ab[4 * k + 1], ab[4 * k + 2], ab[4 * k + 3]);
-        a4[k] = z;
-        d4[k] = z;
-    }
+    reinterpret_cast<uint4 *>(noff + base)[0] = make_uint4(o[0], o[1], o[2], o[3]);
+    reinterpret_cast<uint4 *>(abase + base)[0] = make_uint4(ab[0], ab[1], ab[2], ab[3]);
+    *a4 = make_uint4(0u, 0u, 0u, 0u);
+    *d4 = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // Movers among old slots [lo, hi) (lo <= hi), from the mover bits.
@@ -558,6 +555,12 @@ hipError_t launch_resident_enter(const Geometry &g, const Pools &p, const Record
     return hipGetLastError();
 }
 
+hipError_t launch_resident_tile_sums(const Geometry &g, const Pools &p, const ResWork &w, hipStream_t s) {
+    const uint32_t ntiles = (g.n_buckets + RES_TILE - 1) / RES_TILE;
+    if (ntiles) hipLaunchKernelGGL(k_r_tile_sums, dim3(ntiles), dim3(256), 0, s, p.counts, w.tsum);
+    return hipGetLastError();
+}
+
 hipError_t launch_resident_flush(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s) {
     if (g.n_slots == 0) return hipSuccess;
     hipLaunchKernelGGL(k_r_flush, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, rec);
@@ -567,8 +570,8 @@ hipError_t launch_resident_flush(const Geometry &g, const Pools &p, const Record
 hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
                                    const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, Seg *tick,
                                    hipStream_t s) {
-    const uint32_t ntiles = (g.n_buckets + SCAN_TILE - 1) / SCAN_TILE;
-    if (ntiles == 0 || ntiles > SCAN_MERGE_TILES) return hipErrorInvalidValue;
+    const uint32_t ntiles = (g.n_buckets + RES_TILE - 1) / RES_TILE;
+    if (ntiles == 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_r_scan_apply, dim3(ntiles), dim3(256), 0, s, p.counts, w.arr, w.dep, w.tsum, ntiles,
                        off_nxt, w.abase, p.occ, tick);
     if (g.n_slots)

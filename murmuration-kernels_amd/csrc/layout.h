@@ -35,6 +35,7 @@ namespace fh {
 
 constexpr uint32_t TILE = 256;        // birds per workgroup of the bird-order passes
 constexpr uint32_t SCAN_TILE = 4096;  // buckets per workgroup of the scan (256 lanes x 16)
+constexpr uint32_t RES_TILE = 1024;   // buckets per workgroup of the resident rebuild scan (256 lanes x 4)
 constexpr uint32_t CANON_CAP = 1024;  // largest bucket the per-slot canonical pass reorders
 constexpr uint32_t CANON_BIG_CAP = 16384;  // largest bucket the LDS sort of big buckets reorders
 constexpr uint32_t NO_SEG = 0xffffffffu;
@@ -116,7 +117,7 @@ struct ResWork {
     uint32_t *mbits;  // per old slot: mover bit (32 slots per word)
     uint32_t *arr, *dep;  // per bucket: arrivals, departures (zero between steps)
     uint32_t *abase;      // per bucket: first slot of its arrivals in the new layout
-    uint32_t *tsum;       // per scan tile: its buckets' count total, kept from step to step
+    uint32_t *tsum;       // per RES_TILE buckets: their count total, kept from step to step
 };
 
 // Launch geometry of one build/step over the pools.
@@ -197,6 +198,8 @@ hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Recor
 // the flight was fused into decide, `tick` is the flock's segment and the
 // rebuild commits its clock (+= DT) instead of the integrate pass.
 hipError_t launch_resident_enter(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s);
+// Each resident scan tile's count total (after the entry scan, counts kept).
+hipError_t launch_resident_tile_sums(const Geometry &g, const Pools &p, const ResWork &w, hipStream_t s);
 hipError_t launch_resident_integrate(const Geometry &g, const Pools &p, const Records &rec, const ResWork &w,
                                      Seg *seg, hipStream_t s);
 hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
