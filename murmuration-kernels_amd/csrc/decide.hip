@@ -36,22 +36,27 @@ struct Decision {
 // vel[STRIDE t] (STRIDE 2: interleaved sorted records; 1: resident arrays).
 // With a window, the positions and velocities of slots [wlo, wlo + wn) are
 // also held in LDS (winp, winv) and read from there.
+// The workgroup's dynamic LDS: the due-slot list, then (with a window) the
+// window's positions and velocities. Window reads index this array, so the
+// compiler sees LDS loads, never a generic pointer into LDS.
+extern __shared__ uint32_t decide_lds[];
+
 struct Cand {
     const float4 *pos, *vel;
-    const float4 *winp = nullptr, *winv = nullptr;
+    uint32_t wpos = 0, wvel = 0;  // word offsets of the window's positions and velocities in decide_lds
     uint32_t wlo = 0, wn = 0;
 };
 
 template <int STRIDE>
 __device__ __forceinline__ float4 cand_pos(const Cand &c, uint32_t t) {
     const uint32_t o = t - c.wlo;  // wraps for t < wlo
-    return o < c.wn ? c.winp[o] : c.pos[(size_t)STRIDE * t];
+    return o < c.wn ? reinterpret_cast<const float4 *>(decide_lds + c.wpos)[o] : c.pos[(size_t)STRIDE * t];
 }
 
 template <int STRIDE>
 __device__ __forceinline__ float4 cand_vel(const Cand &c, uint32_t t) {
     const uint32_t o = t - c.wlo;
-    return o < c.wn ? c.winv[o] : c.vel[(size_t)STRIDE * t];
+    return o < c.wn ? reinterpret_cast<const float4 *>(decide_lds + c.wvel)[o] : c.vel[(size_t)STRIDE * t];
 }
 
 // Per-bird walk. `visit(slot)` sees each candidate slot that passes the
@@ -429,10 +434,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
     constexpr int STRIDE = RES ? 1 : 2;
     Cand cand = RES ? Cand{rec.p4, rec.v4} : Cand{p.sbird, p.sbird + 1};
     const bool use_win = (walk_flags & WALK_WINDOW) != 0;
-    // [chunk] due slots, then with a window float4 positions and velocities
-    // [window_slots(chunk)] each
-    extern __shared__ uint32_t due_slots[];
-    float4 *const winp = reinterpret_cast<float4 *>(due_slots + chunk);
+    // decide_lds: [chunk] due slots, then with a window float4 positions and
+    // velocities [window_slots(chunk)] each
+    uint32_t *const due_slots = decide_lds;
+    float4 *const winp = reinterpret_cast<float4 *>(decide_lds + chunk);
     float4 *const winv = winp + window_slots(chunk);
     __shared__ uint32_t n_due;
     __shared__ int32_t u_lo[3], u_hi[3];
@@ -476,8 +481,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
             winp[t - wlo] = cand.pos[(size_t)STRIDE * t];
             winv[t - wlo] = cand.vel[(size_t)STRIDE * t];
         }
-        cand.winp = winp;
-        cand.winv = winv;
+        cand.wpos = chunk;
+        cand.wvel = chunk + 4u * window_slots(chunk);
         cand.wlo = wlo;
         cand.wn = whi - wlo;
         __syncthreads();
