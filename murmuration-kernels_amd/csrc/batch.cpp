@@ -66,6 +66,7 @@ static fh_status bcheck(const fh_batch *b, const char *fn) {
 // Geometry of the live flocks: bird range up to the highest live tile, bucket
 // range up to the highest live bucket.
 static void refresh_geometry(fh_batch *b) {
+    b->rgraph_valid = false;
     b->geo.segs = b->d_segs;
     b->geo.tile_seg = b->d_tile_seg;
     b->geo.n_birds = b->tiles.high_water() * TILE;
@@ -155,12 +156,171 @@ static fh_status batch_capture(fh_batch *b) {
     return FH_OK;
 }
 
+// ---- resident layout ------------------------------------------------------
+
+static fh_status batch_alloc_resident(fh_batch *b) {
+    if (b->res_arena) return FH_OK;
+    const size_t nn = b->bird_cap ? b->bird_cap : 1;
+    const size_t nb = (size_t)b->bucket_cap_units * SCAN_TILE;
+    const bool flight = (b->prm.flags & FH_FLAG_FLIGHT) != 0;
+    Arena a;
+    size_t o_p4[2], o_v4[2], o_s4[2], o_spin[2], o_id[2], o_key[2];
+    for (int q = 0; q < 2; ++q) {
+        o_p4[q] = a.take<float4>(nn);
+        o_v4[q] = a.take<float4>(nn);
+        o_s4[q] = a.take<float4>(nn);
+        o_spin[q] = a.take<float>(nn);
+        o_id[q] = a.take<uint32_t>(nn);
+        o_key[q] = a.take<uint32_t>(nn);
+    }
+    const size_t o_knew = a.take<uint32_t>(nn), o_arank = a.take<uint32_t>(nn);
+    const size_t o_mbits = a.take<uint32_t>((nn + TILE - 1) / TILE * (TILE / 32));
+    const size_t o_arr = a.take<uint32_t>(nb), o_dep = a.take<uint32_t>(nb), o_abase = a.take<uint32_t>(nb);
+    const size_t o_off1 = a.take<uint32_t>(nb + 1), o_due = a.take<uint32_t>(4);
+    const size_t o_tsum = a.take<uint32_t>(nb / RES_TILE);
+    const size_t o_fp4 = flight ? a.take<float4>(nn) : 0, o_fv4 = flight ? a.take<float4>(nn) : 0;
+    const size_t o_fspin = flight ? a.take<float>(nn) : 0;
+    hipError_t e = hipMalloc(&b->res_arena, a.used);
+    if (e != hipSuccess) {
+        b->res_arena = nullptr;
+        return hip_fail(e, "hipMalloc(batch resident arena)");
+    }
+    char *base = static_cast<char *>(b->res_arena);
+    for (int q = 0; q < 2; ++q) {
+        b->R[q].p4 = reinterpret_cast<float4 *>(base + o_p4[q]);
+        b->R[q].v4 = reinterpret_cast<float4 *>(base + o_v4[q]);
+        b->R[q].s4 = reinterpret_cast<float4 *>(base + o_s4[q]);
+        b->R[q].spin = reinterpret_cast<float *>(base + o_spin[q]);
+        b->R[q].id = reinterpret_cast<uint32_t *>(base + o_id[q]);
+        b->R[q].key = reinterpret_cast<uint32_t *>(base + o_key[q]);
+    }
+    b->W.knew = reinterpret_cast<uint32_t *>(base + o_knew);
+    b->W.arank = reinterpret_cast<uint32_t *>(base + o_arank);
+    b->W.mbits = reinterpret_cast<uint32_t *>(base + o_mbits);
+    b->W.arr = reinterpret_cast<uint32_t *>(base + o_arr);
+    b->W.dep = reinterpret_cast<uint32_t *>(base + o_dep);
+    b->W.abase = reinterpret_cast<uint32_t *>(base + o_abase);
+    b->W.tsum = reinterpret_cast<uint32_t *>(base + o_tsum);
+    if (flight) {
+        b->F.p4 = reinterpret_cast<float4 *>(base + o_fp4);
+        b->F.v4 = reinterpret_cast<float4 *>(base + o_fv4);
+        b->F.spin = reinterpret_cast<float *>(base + o_fspin);
+    }
+    b->off[0] = b->P.offsets;
+    b->off[1] = reinterpret_cast<uint32_t *>(base + o_off1);
+    b->res_due = reinterpret_cast<uint32_t *>(base + o_due);
+    // arrivals and departures start at zero and every rebuild zeroes them again
+    FH_HIP(hipMemsetAsync(b->res_arena, 0, a.used, b->stream));
+    FH_HIP(hipStreamSynchronize(b->stream));
+    return FH_OK;
+}
+
+// Bird-order state of every live flock -> resident records: the histogram,
+// the scan with the counts kept, every bird to its slot, the tiles' totals.
+static fh_status batch_enter_resident(fh_batch *b) {
+    fh_status st = batch_alloc_resident(b);
+    if (st) return st;
+    if (!b->hashed && (st = batch_hash(b))) return st;
+    Pools p = b->P;
+    p.offsets = b->off[0];
+    FH_HIP(launch_scan(b->geo, p, nullptr, 0u, 0.0f, b->stream, true));
+    FH_HIP(launch_resident_enter(b->geo, p, b->R[0], b->stream));
+    FH_HIP(launch_resident_tile_sums(b->geo, p, b->W, b->stream));
+    FH_HIP(launch_fill(b->res_due, 4, 0u, b->stream));  // parity 0's counters may hold an earlier step
+    b->parity = 0;
+    b->resident = true;
+    b->hashed = false;  // the counts now follow the layout; the ranks are stale
+    b->counts_dirty = false;
+    return FH_OK;
+}
+
+// Resident records -> bird-order state (before any call that reads or
+// changes the flocks).
+static fh_status batch_leave_resident(fh_batch *b) {
+    if (!b->resident) return FH_OK;
+    FH_HIP(launch_resident_flush(b->geo, b->P, b->R[b->parity], b->stream));
+    b->resident = false;
+    b->hashed = false;
+    b->counts_dirty = true;  // the counts hold the last layout, not a fresh histogram
+    return FH_OK;
+}
+
+// One resident step over every live flock (the events as for one flock).
+static fh_status batch_resident_launches(fh_batch *b, hipEvent_t *ev) {
+    const uint32_t q = b->parity;
+    const uint32_t nseg = (uint32_t)b->segs.size();
+    Pools p = b->P;
+    p.offsets = b->off[q];
+    p.due_count = b->res_due + q;
+    p.work = b->res_due + 2 + q;
+    const bool fused = (b->prm.flags & FH_FLAG_FLIGHT) != 0;
+    if (ev) FH_HIP(hipEventRecord(ev[0], b->stream));
+    FH_HIP(launch_decide_resident(b->geo, p, b->R[q], b->prm.decide_block, b->prm.birds_per_thread,
+                                  b->prm.decide_group, walk_flags(b->prm.flags), b->res_due + (1u - q),
+                                  b->res_due + 2 + (1u - q), b->W, b->F, b->stream,
+                                  ev ? DecideTiming{ev[6], ev[7]} : DecideTiming{}));
+    if (ev) FH_HIP(hipEventRecord(ev[1], b->stream));
+    if (!fused) FH_HIP(launch_resident_integrate(b->geo, p, b->R[q], b->W, b->d_segs, nseg, b->stream));
+    if (ev) FH_HIP(hipEventRecord(ev[2], b->stream));
+    Records src = b->R[q];
+    if (fused) {
+        src.p4 = b->F.p4;
+        src.v4 = b->F.v4;
+        src.spin = b->F.spin;
+    }
+    FH_HIP(launch_resident_rebuild(b->geo, p, src, b->R[1u - q], b->W, b->off[q], b->off[1u - q],
+                                   fused ? b->d_segs : nullptr, nseg, b->stream));
+    if (ev)
+        for (int k = 3; k < 6; ++k) FH_HIP(hipEventRecord(ev[k], b->stream));
+    return FH_OK;
+}
+
+static fh_status batch_capture_resident(fh_batch *b) {
+    if (!b->rgraph_valid) {
+        for (auto &gx : b->rgraph)
+            if (gx) {
+                (void)hipGraphExecDestroy(gx);
+                gx = nullptr;
+            }
+        b->rgraph_valid = true;
+    }
+    const uint32_t q = b->parity;
+    if (b->rgraph[q]) return FH_OK;
+    hipGraph_t g = nullptr;
+    FH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    fh_status st = batch_resident_launches(b, nullptr);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&b->rgraph[q], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        b->rgraph[q] = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    return FH_OK;
+}
+
 namespace fh {
+
+size_t resident_bytes_per_bird(bool flight) {
+    // two records (p4, v4, s4: 48; spin, id, key: 12), knew, arank, mover bits
+    return 2 * 60 + 8 + 1 + (flight ? 36 : 0);
+}
+size_t resident_bytes_per_bucket() { return 4 /*arr*/ + 4 /*dep*/ + 4 /*abase*/ + 4 /*offsets*/ + 1 /*tile totals*/; }
 
 fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *out) {
     if (warmup > steps) warmup = steps;
     const bool graph = (b->prm.flags & FH_FLAG_GRAPH) != 0;
+    const bool res = (b->prm.flags & FH_FLAG_RESIDENT) != 0;
     const uint32_t timed = steps - warmup;
+    if (res && steps && !b->resident && b->geo.n_slots) {
+        fh_status st0 = batch_enter_resident(b);
+        if (st0) return st0;
+    }
     const bool per_pass = out && !graph;
     std::vector<hipEvent_t> ev;
     if (out) {
@@ -180,7 +340,16 @@ fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *o
         }
         for (size_t f = 0; f < b->segs.size(); ++f)
             if (b->segs[f].live) b->times[f] += DT;  // mirrors the device clock add
-        if (graph) {
+        if (b->resident) {
+            if (graph) {
+                if ((st = batch_capture_resident(b))) break;
+                hipError_t e = hipGraphLaunch(b->rgraph[b->parity], b->stream);
+                if (e != hipSuccess) st = hip_fail(e, "hipGraphLaunch");
+            } else {
+                st = batch_resident_launches(b, (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 8] : nullptr);
+            }
+            b->parity ^= 1u;
+        } else if (graph) {
             if (!b->hashed && (st = batch_hash(b))) break;
             if ((st = batch_capture(b))) break;
             hipError_t e = hipGraphLaunch(b->graph, b->stream);
@@ -219,12 +388,17 @@ fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *o
                     FH_HIP(hipEventElapsedTime(&ms, e[0], e[5]));
                     total += ms;
                 }
-                out->grid_histogram = acc[0] / timed;
-                out->grid_scan = acc[1] / timed;
-                out->grid_scatter = acc[2] / timed;
                 out->decide = dec / timed;
-                out->integrate = acc[4] / timed;
                 out->step_total = total / timed;
+                if (b->resident) {  // decide, integrate, then the rebuild (scan and move, reported as scatter)
+                    out->grid_scatter = acc[2] / timed;
+                    out->integrate = acc[1] / timed;
+                } else {
+                    out->grid_histogram = acc[0] / timed;
+                    out->grid_scan = acc[1] / timed;
+                    out->grid_scatter = acc[2] / timed;
+                    out->integrate = acc[4] / timed;
+                }
             } else {
                 float ms = 0.0f;
                 FH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
@@ -232,7 +406,8 @@ fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *o
             }
         }
         uint32_t due = 0;
-        FH_HIP(hipMemcpy(&due, b->P.due_count, 4, hipMemcpyDeviceToHost));
+        FH_HIP(hipMemcpy(&due, b->resident ? b->res_due + (b->parity ^ 1u) : b->P.due_count, 4,
+                         hipMemcpyDeviceToHost));
         out->due_last = due;
     }
     destroy();
@@ -265,8 +440,6 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
     fh_params prm;
     fh_status st = resolve_params(params, &prm, "fh_batch_create");
     if (st) return st;
-    if (prm.flags & FH_FLAG_RESIDENT)
-        return set_error(FH_ERR_INVALID, "fh_batch_create: FH_FLAG_RESIDENT is for single flocks (fh_step)");
     st = require_device();
     if (st) return st;
 
@@ -354,6 +527,9 @@ fh_status fh_batch_destroy(fh_batch *b) {
     if (!b) return FH_OK;
     (void)hipStreamSynchronize(b->stream);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (auto &gx : b->rgraph)
+        if (gx) (void)hipGraphExecDestroy(gx);
+    if (b->res_arena) (void)hipFree(b->res_arena);
     (void)hipFree(b->arena);
     (void)hipStreamDestroy(b->stream);
     delete b;
@@ -366,6 +542,7 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
     if (st) return st;
     if (!flock_id || (n && !words) || n == 0)
         return set_error(FH_ERR_INVALID, "fh_batch_add: need n > 0, words and flock_id");
+    if ((st = batch_leave_resident(b))) return st;
     const uint32_t buckets = flock_buckets(n, b->prm);
     const uint32_t ntile = (n + TILE - 1) / TILE;
     const uint32_t nunit = (buckets + SCAN_TILE - 1) / SCAN_TILE;
@@ -442,6 +619,7 @@ fh_status fh_batch_remove(fh_batch *b, uint32_t flock_id) {
     if (st) return st;
     if (flock_id >= b->segs.size() || !b->segs[flock_id].live)
         return set_error(FH_ERR_INVALID, "fh_batch_remove: no live flock %u", flock_id);
+    if ((st = batch_leave_resident(b))) return st;
     Seg &s = b->segs[flock_id];
     if (b->counts_dirty)  // drop this flock's birds from the pending histogram
         FH_HIP(launch_fill(b->P.counts + s.bucket_base, s.mask + 1, 0u, b->stream));
@@ -464,6 +642,7 @@ fh_status fh_batch_download(fh_batch *b, uint32_t flock_id, float *words, float 
     const Seg &s = b->segs[flock_id];
     const uint32_t n = s.n;
     if (!words) return set_error(FH_ERR_INVALID, "fh_batch_download: NULL words");
+    if ((st = batch_leave_resident(b))) return st;
     std::vector<float4> pos(n), vel(n), steer(n);
     std::vector<float> spin(n), cen(n);
     std::vector<uint32_t> nbv(n);
@@ -517,7 +696,10 @@ size_t fh_flock_bytes(uint32_t n, const fh_params *params) {
     const uint32_t buckets = flock_buckets(n, prm);
     const size_t birds = (size_t)((n + TILE - 1) / TILE) * TILE;
     const size_t bk = (size_t)((buckets + SCAN_TILE - 1) / SCAN_TILE) * SCAN_TILE;
-    return birds * batch_bytes_per_bird() + bk * batch_bytes_per_bucket() + bk / 8 /*occupancy bits*/;
+    size_t bytes = birds * batch_bytes_per_bird() + bk * batch_bytes_per_bucket() + bk / 8 /*occupancy bits*/;
+    if (prm.flags & FH_FLAG_RESIDENT)
+        bytes += birds * resident_bytes_per_bird((prm.flags & FH_FLAG_FLIGHT) != 0) + bk * resident_bytes_per_bucket();
+    return bytes;
 }
 
 }  // extern "C"

@@ -29,6 +29,9 @@ class RangeAlloc {
 
 size_t batch_bytes_per_bird();
 size_t batch_bytes_per_bucket();
+// Extra device bytes of the resident layout per bird and per bucket.
+size_t resident_bytes_per_bird(bool flight);
+size_t resident_bytes_per_bucket();
 
 }  // namespace fh
 
@@ -50,6 +53,18 @@ struct fh_batch {
     bool counts_dirty = false;  // histogram filled and not yet consumed by a scan
     hipGraphExec_t graph = nullptr;
     bool graph_valid = false;
+    // Resident layout (FH_FLAG_RESIDENT): every flock's state kept in bucket
+    // order between steps, as for one flock (records R[parity], flown
+    // records F for the fused flight, work arrays W, offsets per parity).
+    void *res_arena = nullptr;
+    fh::Records R[2]{}, F{};
+    fh::ResWork W{};
+    uint32_t *off[2]{};
+    uint32_t *res_due = nullptr;  // due counter per parity, then the persistent decide's chunk counters
+    bool resident = false;
+    uint32_t parity = 0;
+    hipGraphExec_t rgraph[2]{};
+    bool rgraph_valid = false;
 };
 
 namespace fh {

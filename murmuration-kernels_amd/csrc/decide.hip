@@ -488,11 +488,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
         __syncthreads();
     }
     // Sorted copies: time is relative (0) and snext holds next - time. Resident
-    // records: this step's clock is the segment clock + DT (committed by the
-    // integrate pass that follows) and p4.w holds next_decision.
-    const float tnow = RES ? g.segs[0].time + DT : time;
-    for (uint32_t s = start + threadIdx.x; s < end; s += BLOCK)
-        if (tnow >= (RES ? cand_pos<STRIDE>(cand, s).w : p.snext[s])) due_slots[atomicAdd(&n_due, 1u)] = s;
+    // records: this step's clock is the bird's flock clock + DT (committed by
+    // the pass that follows) and p4.w holds next_decision.
+    for (uint32_t s = start + threadIdx.x; s < end; s += BLOCK) {
+        const bool due = RES ? g.segs[slot_segment(g, rec, s)].time + DT >= cand_pos<STRIDE>(cand, s).w
+                             : time >= p.snext[s];
+        if (due) due_slots[atomicAdd(&n_due, 1u)] = s;
+    }
     __syncthreads();
     const uint32_t nd = n_due;
     if (threadIdx.x == 0 && nd) atomicAdd(p.due_count, nd);
@@ -622,9 +624,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
         // of this step from rec. A separate instance: the flight's code would
         // otherwise cost the G = 1-2 walks a wave per SIMD.
         __syncthreads();
-        const Seg sg0 = g.segs[0];
         for (uint32_t b0 = start; b0 < end; b0 += BLOCK)
-            resident_flight(g, sg0, rec, flown, w, b0 + threadIdx.x, b0 + threadIdx.x < end);
+            resident_flight(g, rec, flown, w, b0 + threadIdx.x, b0 + threadIdx.x < end);
     }
     __syncthreads();  // due_slots and n_due are reused by the next chunk
     }

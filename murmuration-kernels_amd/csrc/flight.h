@@ -48,17 +48,18 @@ __device__ __forceinline__ void integrate_bird(float4 &p4, float4 &v4, const flo
 }
 
 // Resident flight of slot s: banked flight of in[s] into out[s] (out may be
-// in), its new bucket, and the mover accounting. A bird whose new cell falls
+// in), its new bucket (by its flock's hash), and the mover accounting. A bird whose new cell falls
 // in another bucket is a mover: it counts itself out of its old bucket and
 // into its new one (keeping its arrival rank), and out of and into the scan
 // tiles' totals when the tile changes. Every lane of a wave calls this
 // (active or not) with consecutive slots starting at a multiple of 64: the
 // wave's mover bits are written with one ballot. Words past the flock's last
 // slot are not written (nothing reads them).
-__device__ __forceinline__ void resident_flight(const Geometry &g, const Seg &sg, const Records &in,
-                                                const Records &out, const ResWork &w, uint32_t s, bool active) {
+__device__ __forceinline__ void resident_flight(const Geometry &g, const Records &in, const Records &out,
+                                                const ResWork &w, uint32_t s, bool active) {
     bool mover = false;
     if (active) {
+        const Seg &sg = g.segs[slot_segment(g, in, s)];
         float4 p4 = in.p4[s];
         float4 v4 = in.v4[s];
         float rate;

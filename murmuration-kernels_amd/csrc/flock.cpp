@@ -274,7 +274,7 @@ static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
                                   h->res_due + 2 + (1u - q), h->W, h->F, h->stream,
                                   ev ? DecideTiming{ev[6], ev[7]} : DecideTiming{}));
     if (ev) FH_HIP(hipEventRecord(ev[1], h->stream));
-    if (!fused) FH_HIP(launch_resident_integrate(h->geo, p, h->R[q], h->W, h->d_seg, h->stream));
+    if (!fused) FH_HIP(launch_resident_integrate(h->geo, p, h->R[q], h->W, h->d_seg, 1u, h->stream));
     if (ev) FH_HIP(hipEventRecord(ev[2], h->stream));
     // the move reads the flown positions, velocities and roll rates from F
     // when the flight was fused, everything else from the current records
@@ -285,7 +285,7 @@ static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
         src.spin = h->F.spin;
     }
     FH_HIP(launch_resident_rebuild(h->geo, p, src, h->R[1u - q], h->W, h->off[q], h->off[1u - q],
-                                   fused ? h->d_seg : nullptr, h->stream));
+                                   fused ? h->d_seg : nullptr, 1u, h->stream));
     if (ev)
         for (int k = 3; k < 6; ++k) FH_HIP(hipEventRecord(ev[k], h->stream));
     return FH_OK;

@@ -417,6 +417,10 @@ hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, boo
 __global__ __launch_bounds__(TILE) void k_r_enter(Geometry g, Pools p, Records rec) {
     const uint32_t b = blockIdx.x * TILE + threadIdx.x;
     if (b >= g.n_birds) return;
+    if (g.tile_seg) {  // a batch: padding birds and dead flocks hold no slot
+        const uint32_t si = g.tile_seg[b / TILE];
+        if (si == NO_SEG || !g.segs[si].live || b - g.segs[si].bird_base >= g.segs[si].n) return;
+    }
     const uint32_t k = p.key[b];
     const uint32_t s = p.offsets[k] + p.rank[b];
     if (s >= g.n_slots) {
@@ -465,8 +469,8 @@ __global__ __launch_bounds__(256) void k_r_scan_apply(uint32_t *__restrict__ cou
                                                       uint32_t *__restrict__ arr, uint32_t *__restrict__ dep,
                                                       const uint32_t *__restrict__ tsum, uint32_t ntiles,
                                                       uint32_t *__restrict__ noff, uint32_t *__restrict__ abase,
-                                                      uint32_t *__restrict__ occ, Seg *tick) {
-    if (tick && blockIdx.x == 0 && threadIdx.x == 0) tick->time = tick->time + DT;
+                                                      uint32_t *__restrict__ occ, Seg *tick, uint32_t ntick) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) commit_clocks(tick, ntick);
     uint32_t part = 0;
     for (uint32_t t = threadIdx.x; t < blockIdx.x; t += 256u) part += tsum[t];
     uint32_t prefix;
@@ -569,11 +573,11 @@ hipError_t launch_resident_flush(const Geometry &g, const Pools &p, const Record
 
 hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
                                    const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, Seg *tick,
-                                   hipStream_t s) {
+                                   uint32_t ntick, hipStream_t s) {
     const uint32_t ntiles = (g.n_buckets + RES_TILE - 1) / RES_TILE;
     if (ntiles == 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_r_scan_apply, dim3(ntiles), dim3(256), 0, s, p.counts, w.arr, w.dep, w.tsum, ntiles,
-                       off_nxt, w.abase, p.occ, tick);
+                       off_nxt, w.abase, p.occ, tick, tick ? ntick : 0u);
     if (g.n_slots)
         hipLaunchKernelGGL(k_r_move, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, cur, nxt, w,
                            off_cur, off_nxt);

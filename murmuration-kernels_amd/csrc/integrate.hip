@@ -57,20 +57,19 @@ __global__ __launch_bounds__(TILE) void k_integrate(Geometry g, Pools p) {
     }
 }
 
-// Banked flight over resident records (slot order, one flock), in place;
-// see resident_flight. Workgroup 0 commits the step's clock (segment time +=
-// DT, the time decide used).
-__global__ __launch_bounds__(TILE) void k_r_integrate(Geometry g, Records rec, ResWork w, Seg *seg) {
+// Banked flight over resident records (slot order), in place; see
+// resident_flight. Workgroup 0 commits the step's clocks (each live segment's
+// time += DT, the time decide used).
+__global__ __launch_bounds__(TILE) void k_r_integrate(Geometry g, Records rec, ResWork w, Seg *segs, uint32_t nseg) {
     const uint32_t s = blockIdx.x * TILE + threadIdx.x;
-    const Seg sg = g.segs[0];
-    resident_flight(g, sg, rec, rec, w, s, s < g.n_slots);
-    if (blockIdx.x == 0 && threadIdx.x == 0) seg->time = sg.time + DT;
+    resident_flight(g, rec, rec, w, s, s < g.n_slots);
+    if (blockIdx.x == 0 && threadIdx.x == 0) commit_clocks(segs, nseg);
 }
 
 hipError_t launch_resident_integrate(const Geometry &g, const Pools &, const Records &rec, const ResWork &w,
-                                     Seg *seg, hipStream_t s) {
+                                     Seg *segs, uint32_t nseg, hipStream_t s) {
     if (g.n_slots == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_r_integrate, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, rec, w, seg);
+    hipLaunchKernelGGL(k_r_integrate, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, rec, w, segs, nseg);
     return hipGetLastError();
 }
 

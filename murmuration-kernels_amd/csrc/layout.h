@@ -131,6 +131,18 @@ struct Geometry {
     uint32_t tile_count;       // tiles of a bird-order launch (0 = up to n_birds)
 };
 
+// Segment (flock) of the bird in slot s of resident records: through its bird
+// id and the tile -> segment map in a batch, segment 0 for one flock.
+__device__ __forceinline__ uint32_t slot_segment(const Geometry &g, const Records &rec, uint32_t s) {
+    return g.tile_seg ? g.tile_seg[rec.id[s] / TILE] : 0u;
+}
+
+// Advances the clock of every live segment by DT (one thread).
+__device__ __forceinline__ void commit_clocks(Seg *segs, uint32_t nseg) {
+    for (uint32_t i = 0; i < nseg; ++i)
+        if (segs[i].live) segs[i].time = segs[i].time + DT;
+}
+
 // Launchers (stream-ordered, no host synchronisation). Each returns the
 // hipError_t of its launch.
 hipError_t launch_hash_count(const Geometry &g, const Pools &p, bool wave_agg, hipStream_t s);
@@ -195,16 +207,18 @@ hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Recor
 // movers counted (and the flock clock committed: += DT); the rebuild of the
 // grid and the move of every record to its slot in `nxt`; records back to the
 // bird-order state. The rebuild reads the records to move from `cur`; when
-// the flight was fused into decide, `tick` is the flock's segment and the
-// rebuild commits its clock (+= DT) instead of the integrate pass.
+// the flight was fused into decide, `tick` holds the ntick segments whose
+// clocks the rebuild commits (+= DT each live one) instead of the integrate
+// pass. In a batch (g.tile_seg set) every slot's flock is found through its
+// bird id; the enter pass skips the padding birds of each flock's range.
 hipError_t launch_resident_enter(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s);
 // Each resident scan tile's count total (after the entry scan, counts kept).
 hipError_t launch_resident_tile_sums(const Geometry &g, const Pools &p, const ResWork &w, hipStream_t s);
 hipError_t launch_resident_integrate(const Geometry &g, const Pools &p, const Records &rec, const ResWork &w,
-                                     Seg *seg, hipStream_t s);
+                                     Seg *segs, uint32_t nseg, hipStream_t s);
 hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
                                    const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, Seg *tick,
-                                   hipStream_t s);
+                                   uint32_t ntick, hipStream_t s);
 hipError_t launch_resident_flush(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s);
 hipError_t launch_integrate(const Geometry &g, const Pools &p, bool hash, bool wave_agg,
                             hipStream_t s);

@@ -343,10 +343,11 @@ class Flock:
         return ids[: min(cnt.value, cap)].copy() if cnt.value <= cap else ids.copy()
 
 
-def flock_bytes(n: int, buckets: int = 0, bucket_shift: int = 0) -> int:
-    """Device bytes one flock of n birds occupies in a batch (admission unit)."""
+def flock_bytes(n: int, buckets: int = 0, bucket_shift: int = 0, flags: int = 0) -> int:
+    """Device bytes one flock of n birds occupies in a batch (admission unit);
+    with FLAG_RESIDENT it includes the resident layout's arrays."""
     p = Params()
-    p.buckets, p.bucket_shift = buckets, bucket_shift
+    p.buckets, p.bucket_shift, p.flags = buckets, bucket_shift, flags
     return int(lib().fh_flock_bytes(n, C.addressof(p)))
 
 

@@ -1,0 +1,116 @@
+"""Batched stepping in the resident layout (FH_FLAG_RESIDENT on a batch):
+every flock's state stays in bucket order between steps, as for one flock.
+Arrivals take their slots in atomic order, so results are compared with
+stepping alone by the decision clocks (exactly) and positions, velocities
+and neighbour counts (to rounding of reordered sums); the oracle agrees
+with a batched flock like with a flock stepped alone."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RES, GRAPH, CANON = 64, 4, 1
+
+
+def cube(fh, n, seed):
+    return fh.gen_cube(n, fh.cube_side_for(n), seed)
+
+
+def alone(fh, w, steps, flags, time=0.0):
+    with fh.Flock.from_words(w, time, flags=flags) as f:
+        f.step(steps)
+        return f.download()
+
+
+def rel_err(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)))) if a.size else 0.0
+
+
+def close(got, want, tol=1e-4):
+    assert np.array_equal(got[:, 12], want[:, 12])  # decision times
+    assert rel_err(got[:, 0:6], want[:, 0:6]) <= tol
+    assert np.mean(got[:, 13].view(np.uint32) == want[:, 13].view(np.uint32)) >= 0.99
+
+
+@pytest.mark.parametrize("flags", [RES, RES | GRAPH, RES | GRAPH | 2048, RES | 128 | 512 | GRAPH,
+                                   RES | 256 | 1024, RES | 2048 | 128])
+def test_resident_batch_equals_alone(fh, flags):
+    sizes = [1000, 3000, 257, 20_000, 1, 64_000]
+    starts = [cube(fh, n, 30 + i) for i, n in enumerate(sizes)]
+    with fh.Batch(200_000, 1 << 22, 16, flags=flags, decide_group=4) as b:
+        ids = [b.add(w) for w in starts]
+        b.step(12)
+        for fid, w in zip(ids, starts):
+            got, t = b.download(fid)
+            want, tw = alone(fh, w, 12, CANON)
+            assert t == tw
+            close(got, want)
+
+
+def test_resident_batch_matches_the_oracle(fh, fr):
+    w0 = fh.gen_reference(2000, 1)
+    other = cube(fh, 30_000, 2)
+    with fh.Batch(100_000, 1 << 21, 4, flags=RES | GRAPH | 2048) as b:
+        b.add(other)
+        fid = b.add(w0)
+        b.step(10)
+        got, t = b.download(fid)
+    cpu, tc = fr.step(w0, 0.0, 10)
+    d = fr.diff(cpu, got)
+    assert t == tc and d["ok"], d
+
+
+def test_resident_batch_join_and_leave(fh):
+    # Adding, removing and downloading leave the resident layout; the next
+    # step enters it again with the flocks then live.
+    a, c, d = cube(fh, 5000, 1), cube(fh, 9000, 2), cube(fh, 700, 3)
+    out = {}
+    for name, flags in (("resident", RES | GRAPH), ("canonical", CANON)):
+        with fh.Batch(50_000, 1 << 21, 8, flags=flags) as b:
+            ia = b.add(a)
+            b.step(5)
+            ic = b.add(c, 0.0)
+            b.step(7)
+            b.download(ia)
+            b.step(3)  # an odd number of steps: the next entry restarts at parity 0
+            b.remove(ia)
+            idd = b.add(d, 0.0)
+            b.step(4)
+            out[name] = (b.download(ic), b.download(idd))
+    for (g, tg), (w, tw) in zip(out["resident"], out["canonical"]):
+        assert tg == tw
+        close(g, w)
+
+
+def test_resident_batch_clock_per_flock(fh):
+    w = cube(fh, 2000, 4)
+    with fh.Batch(10_000, 1 << 18, 4, flags=RES) as b:
+        i0 = b.add(w, 0.0)
+        i1 = b.add(w, 1.25)
+        b.step(3)
+        _, t0 = b.download(i0)
+        got1, t1 = b.download(i1)
+    want1, tw1 = alone(fh, w, 3, CANON, time=1.25)
+    assert t0 == np.float32(np.float32(np.float32(0.005) + np.float32(0.005)) + np.float32(0.005))
+    assert t1 == tw1
+    close(got1, want1)
+
+
+def test_resident_batch_timing_and_bytes(fh):
+    starts = [cube(fh, n, 40 + n) for n in (30_000, 50_000)]
+    with fh.Batch(100_000, 1 << 21, 4, flags=RES) as b:
+        for w in starts:
+            b.add(w)
+        ms = b.step(4, 1, timing=True)
+    assert ms["timed_steps"] == 3
+    assert ms["decide"] > 0 and ms["integrate"] > 0 and ms["grid_scatter"] > 0
+    assert 4_000 < ms["due_last"] < 12_000  # about a tenth of the 80,000 birds decide each step
+    assert fh.flock_bytes(1000, flags=RES) > fh.flock_bytes(1000)
+
+
+def test_resident_batch_rejects_canonical(fh):
+    with pytest.raises(fh.FlockHipError) as e:
+        fh.Batch(1000, 1 << 16, 2, flags=RES | CANON)
+    assert e.value.status == fh.ERR_INVALID
