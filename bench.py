@@ -366,7 +366,8 @@ def clumps_leg(args) -> dict:
 def batched_leg(args) -> dict:
     """256 flocks of log-uniform sizes 1k-200k at 0.64 birds per cell: all of
     them in one batch (one launch per stage, hipGraph, autotuned) against the
-    same flocks stepped one flock per launch sequence (each its own graph)."""
+    same flocks stepped one flock per launch sequence, each with its own
+    graph and the same tunables and flags."""
     import flock_hip as fh
     from flock_hip import autotune, workloads
 
@@ -374,7 +375,7 @@ def batched_leg(args) -> dict:
     starts = workloads.mix(args.flocks, seed=args.seed)
     sizes = [int(w.shape[0]) for w in starts]
     total = sum(sizes)
-    tuned = autotune.tune(starts, "batch", flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, warmup=2, steps=5,
+    tuned = autotune.tune(starts, "batch", flags=fh.FLAG_GRAPH, warmup=2, steps=5,
                           progress=log, thorough=False, **tune_cache(args, f"mix-{args.flocks}-seed{args.seed}"))
     prm, flags = tuned["params"], tuned["flags"]
     buckets = sum(autotune._buckets(s) for s in sizes)
@@ -385,7 +386,7 @@ def batched_leg(args) -> dict:
         tb = time.perf_counter()
         b.step(args.batch_steps)
         batched_s = time.perf_counter() - tb
-    flocks = [fh.Flock.from_words(w, flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL) for w in starts]
+    flocks = [fh.Flock.from_words(w, flags=flags | fh.FLAG_GRAPH, **prm) for w in starts]
     for f in flocks:
         f.step(3)
     tp = time.perf_counter()
@@ -397,7 +398,7 @@ def batched_leg(args) -> dict:
     bs = total * args.batch_steps
     return {
         "workload": f"{len(starts)} flocks, sizes log-uniform 1e3..2e5 (seed {args.seed}), {total} birds, "
-                    f"0.64 birds per 4 m cell, {args.batch_steps} steps, canonical order",
+                    f"0.64 birds per 4 m cell, {args.batch_steps} steps",
         "birds": total,
         "steps": args.batch_steps,
         "batched_bird_steps_per_s": bs / batched_s,
@@ -465,12 +466,12 @@ def served_leg(args, batched: dict) -> dict:
 
     t0 = time.perf_counter()
     warm_req = workloads.stream(32, 1.0, seed=args.seed + 7)
-    with fh.Batch(40 << 20, 64 << 20, 64, flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL) as b:
+    with fh.Batch(40 << 20, 64 << 20, 64, flags=fh.FLAG_GRAPH) as b:
         ids = [b.add(fh.gen_request(int(n), int(s))) for n, s in zip(warm_req["birds"], warm_req["seeds"])]
         b.step(250)
         warm = [b.download(i) for i in ids]
     log("served leg: autotune on 32 warmed request flocks")
-    tuned = autotune.tune([w for w, _ in warm], "batch", flags=fh.FLAG_GRAPH | fh.FLAG_CANONICAL, warmup=1, steps=4,
+    tuned = autotune.tune([w for w, _ in warm], "batch", flags=fh.FLAG_GRAPH, warmup=1, steps=4,
                           times=[t for _, t in warm],
                           cells=(4.0, 6.0, 8.0, 10.0, 12.0), progress=log,
                           **tune_cache(args, f"served-warm250-seed{args.seed + 7}"))

@@ -6,7 +6,7 @@ workgroup size, the birds each decide workgroup scans per lane
 whether decide tests the bucket occupancy bitmap, runs as a persistent grid
 or maps chunks XCD by XCD, whether it stages each chunk's positions in LDS,
 whether cells are bucketed by the spatial hash and
-(single flocks) whether fh_step keeps the state in bucket order and flies each chunk inside decide, by timing
+whether the state stays in bucket order between steps and flies each chunk inside decide, by timing
 short runs of the real workload. Each candidate is timed over `steps` steps
 after `warmup` untimed ones; the fastest few are timed again and the lowest
 median kept. Results can be cached in a JSON file keyed by
@@ -40,7 +40,7 @@ SWITCHES = {"occ": FLAG_OCC, "resident": FLAG_RESIDENT, "spatial": FLAG_SPATIAL,
 # Part of every cache key: raised whenever the searched space or the kernels'
 # costs change, so an entry found for other kernels or a smaller space is not
 # reused.
-SEARCH_VERSION = 7
+SEARCH_VERSION = 8
 FINALISTS = 6
 CONFIRM = 3
 
@@ -157,8 +157,8 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
     else:
         for nv in shape:
             sweep([nv])
-    # the resident layout is a single-flock fh_step mode, not for canonical order
-    resident = (0, 1) if kind == "plain" and not (flags & FLAG_CANONICAL) else (0,)
+    # the resident layout keeps no canonical order
+    resident = (0, 1) if not (flags & FLAG_CANONICAL) else (0,)
     # the fused flight exists only in the resident layout
     for nv in (("resident", resident), ("spatial", spatial), ("cell", cells), ("bucket_shift", shifts),
                ("occ", (0, 1)), ("persist", (0, 1)), ("xcd", (0, 1)), ("window", (0, 1)), ("flight", resident),

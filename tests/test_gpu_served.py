@@ -45,6 +45,34 @@ def test_each_result_equals_the_flock_alone(fh):
                 assert r["checksum"] == fh.state_checksum(want), rid
 
 
+def test_resident_served_results_match_alone(fh):
+    # The served loop over a resident batch (arrivals take their slots in
+    # atomic order): each result has the clock and decision times of the
+    # flock stepped alone, positions within 1e-4, and the checksum of the
+    # state the server returns.
+    rng = np.random.default_rng(6)
+    reqs = [(int(b), int(s), 200 + i) for i, (b, s) in
+            enumerate(zip(rng.integers(2000, 30_000, 10), rng.integers(10, 50, 10)))]
+    with fh.Server(max_active=4, steps_per_iter=4, keep_states=1, bird_capacity=1 << 20,
+                   bucket_capacity=1 << 23, flags=fh.FLAG_RESIDENT | fh.FLAG_GRAPH | fh.FLAG_FLIGHT) as srv:
+        ids = [srv.submit(b, s, seed) for b, s, seed in reqs]
+        srv.drain(300)
+        res = {r["id"]: r for r in srv.poll()}
+        for rid, (b, s, seed) in zip(ids, reqs):
+            r = res[rid]
+            assert r["status"] == 0 and r["final_time"] == clock_after(s)
+            got, t = srv.take_state(rid)
+            with fh.Flock.from_words(start_state(fh, b, seed), flags=fh.FLAG_CANONICAL) as f:
+                f.step(s)
+                want, tw = f.download()
+            assert t == tw
+            assert np.array_equal(got[:, 12], want[:, 12]), rid
+            rel = np.abs(got[:, 0:6].astype(np.float64) - want[:, 0:6]) / np.maximum(1.0, np.abs(want[:, 0:6]))
+            assert float(rel.max()) <= 1e-4, rid
+            if b < 6000:
+                assert r["checksum"] == fh.state_checksum(got), rid
+
+
 def test_served_request_matches_the_oracle(fh, fr):
     with fh.Server(max_active=4, keep_states=1, bird_capacity=1 << 20, bucket_capacity=1 << 22) as srv:
         other = srv.submit(20_000, 30, 7)
