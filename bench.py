@@ -39,7 +39,7 @@ def step_bytes(n: int, buckets: int, due: int, resident: bool) -> dict:
     counting the movers, then scan and move as one 'grid_scatter' pass)."""
     if resident:
         return {"decide": 28 * n + 4 * buckets + 32 * due, "integrate": 72 * n,
-                "grid_scatter": 32 * buckets + 120 * n}
+                "grid_scatter": 20 * buckets + 120 * n}
     return {
         "grid_histogram": 16 * n + 8 * buckets,
         "grid_scan": 8 * buckets,

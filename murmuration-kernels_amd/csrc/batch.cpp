@@ -175,7 +175,7 @@ static fh_status batch_alloc_resident(fh_batch *b) {
     }
     const size_t o_knew = a.take<uint32_t>(nn), o_arank = a.take<uint32_t>(nn);
     const size_t o_mbits = a.take<uint32_t>((nn + TILE - 1) / TILE * (TILE / 32));
-    const size_t o_arr = a.take<uint32_t>(nb), o_dep = a.take<uint32_t>(nb), o_abase = a.take<uint32_t>(nb);
+    const size_t o_arr = a.take<uint32_t>(nb), o_abase = a.take<uint32_t>(nb);
     const size_t o_off1 = a.take<uint32_t>(nb + 1), o_due = a.take<uint32_t>(4);
     const size_t o_tsum = a.take<uint32_t>(nb / RES_TILE);
     const size_t o_fp4 = flight ? a.take<float4>(nn) : 0, o_fv4 = flight ? a.take<float4>(nn) : 0;
@@ -198,7 +198,7 @@ static fh_status batch_alloc_resident(fh_batch *b) {
     b->W.arank = reinterpret_cast<uint32_t *>(base + o_arank);
     b->W.mbits = reinterpret_cast<uint32_t *>(base + o_mbits);
     b->W.arr = reinterpret_cast<uint32_t *>(base + o_arr);
-    b->W.dep = reinterpret_cast<uint32_t *>(base + o_dep);
+    b->W.counts = b->P.counts;
     b->W.abase = reinterpret_cast<uint32_t *>(base + o_abase);
     b->W.tsum = reinterpret_cast<uint32_t *>(base + o_tsum);
     if (flight) {
@@ -209,7 +209,7 @@ static fh_status batch_alloc_resident(fh_batch *b) {
     b->off[0] = b->P.offsets;
     b->off[1] = reinterpret_cast<uint32_t *>(base + o_off1);
     b->res_due = reinterpret_cast<uint32_t *>(base + o_due);
-    // arrivals and departures start at zero and every rebuild zeroes them again
+    // arrivals start at zero and every rebuild zeroes them again
     FH_HIP(hipMemsetAsync(b->res_arena, 0, a.used, b->stream));
     FH_HIP(hipStreamSynchronize(b->stream));
     return FH_OK;
@@ -310,7 +310,7 @@ size_t resident_bytes_per_bird(bool flight) {
     // two records (p4, v4, s4: 48; spin, id, key: 12), knew, arank, mover bits
     return 2 * 60 + 8 + 1 + (flight ? 36 : 0);
 }
-size_t resident_bytes_per_bucket() { return 4 /*arr*/ + 4 /*dep*/ + 4 /*abase*/ + 4 /*offsets*/ + 1 /*tile totals*/; }
+size_t resident_bytes_per_bucket() { return 4 /*arr*/ + 4 /*abase*/ + 4 /*offsets*/ + 1 /*tile totals*/; }
 
 fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *out) {
     if (warmup > steps) warmup = steps;

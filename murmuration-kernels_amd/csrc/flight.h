@@ -49,8 +49,8 @@ __device__ __forceinline__ void integrate_bird(float4 &p4, float4 &v4, const flo
 
 // Resident flight of slot s: banked flight of in[s] into out[s] (out may be
 // in), its new bucket (by its flock's hash), and the mover accounting. A bird whose new cell falls
-// in another bucket is a mover: it counts itself out of its old bucket and
-// into its new one (keeping its arrival rank), and out of and into the scan
+// in another bucket is a mover: it moves its count from its old bucket to its
+// new one, takes an arrival rank there, and moves its count between the scan
 // tiles' totals when the tile changes. Every lane of a wave calls this
 // (active or not) with consecutive slots starting at a multiple of 64: the
 // wave's mover bits are written with one ballot. Words past the flock's last
@@ -74,7 +74,8 @@ __device__ __forceinline__ void resident_flight(const Geometry &g, const Records
         if (kn != ko) {
             mover = true;
             w.arank[s] = atomicAdd(&w.arr[kn], 1u);
-            atomicAdd(&w.dep[ko], 1u);
+            atomicAdd(&w.counts[kn], 1u);
+            atomicSub(&w.counts[ko], 1u);
             if (kn / RES_TILE != ko / RES_TILE) {
                 atomicAdd(&w.tsum[kn / RES_TILE], 1u);
                 atomicSub(&w.tsum[ko / RES_TILE], 1u);

@@ -185,7 +185,7 @@ static fh_status alloc_resident(fh_flock *h) {
     }
     const size_t o_knew = a.take<uint32_t>(nn), o_arank = a.take<uint32_t>(nn);
     const size_t o_mbits = a.take<uint32_t>((nn + TILE - 1) / TILE * (TILE / 32));
-    const size_t o_arr = a.take<uint32_t>(nb), o_dep = a.take<uint32_t>(nb), o_abase = a.take<uint32_t>(nb);
+    const size_t o_arr = a.take<uint32_t>(nb), o_abase = a.take<uint32_t>(nb);
     const bool flight = (h->prm.flags & FH_FLAG_FLIGHT) != 0;
     const size_t o_fp4 = flight ? a.take<float4>(nn) : 0, o_fv4 = flight ? a.take<float4>(nn) : 0;
     const size_t o_fspin = flight ? a.take<float>(nn) : 0;
@@ -209,7 +209,7 @@ static fh_status alloc_resident(fh_flock *h) {
     h->W.arank = reinterpret_cast<uint32_t *>(base + o_arank);
     h->W.mbits = reinterpret_cast<uint32_t *>(base + o_mbits);
     h->W.arr = reinterpret_cast<uint32_t *>(base + o_arr);
-    h->W.dep = reinterpret_cast<uint32_t *>(base + o_dep);
+    h->W.counts = h->P.counts;
     h->W.abase = reinterpret_cast<uint32_t *>(base + o_abase);
     h->W.tsum = reinterpret_cast<uint32_t *>(base + o_tsum);
     if (flight) {
@@ -220,7 +220,7 @@ static fh_status alloc_resident(fh_flock *h) {
     h->off[0] = h->P.offsets;
     h->off[1] = reinterpret_cast<uint32_t *>(base + o_off1);
     h->res_due = reinterpret_cast<uint32_t *>(base + o_due);
-    // arrivals and departures start at zero and every rebuild zeroes them again
+    // arrivals start at zero and every rebuild zeroes them again
     FH_HIP(hipMemsetAsync(h->res_arena, 0, a.used, h->stream));
     FH_HIP(hipStreamSynchronize(h->stream));
     return FH_OK;

@@ -459,14 +459,13 @@ __global__ __launch_bounds__(256) void k_r_tile_sums(const uint32_t *__restrict_
     if (threadIdx.x == 0) tsum[blockIdx.x] = total;
 }
 
-// Rebuild scan (one launch; RES_TILE buckets per workgroup, 4 per lane):
-// counts += arrivals - departures; the new offsets, from the tiles' totals
-// that the flight kept current with its movers; the first arrival slot of
-// each bucket (its stayers come first); the occupancy bits. Arrivals and
-// departures are zeroed for the next step. With `tick`, workgroup 0 commits
-// the step's clock.
-__global__ __launch_bounds__(256) void k_r_scan_apply(uint32_t *__restrict__ counts,
-                                                      uint32_t *__restrict__ arr, uint32_t *__restrict__ dep,
+// Rebuild scan (one launch; RES_TILE buckets per workgroup, 4 per lane): the
+// new offsets from the counts the flight kept current, and from the tiles'
+// totals before this tile; the first arrival slot of each bucket (its stayers
+// come first); the occupancy bits. Arrivals are zeroed for the next step.
+// With `tick`, workgroup 0 commits the step's clocks.
+__global__ __launch_bounds__(256) void k_r_scan_apply(const uint32_t *__restrict__ counts,
+                                                      uint32_t *__restrict__ arr,
                                                       const uint32_t *__restrict__ tsum, uint32_t ntiles,
                                                       uint32_t *__restrict__ noff, uint32_t *__restrict__ abase,
                                                       uint32_t *__restrict__ occ, Seg *tick, uint32_t ntick) {
@@ -476,13 +475,10 @@ __global__ __launch_bounds__(256) void k_r_scan_apply(uint32_t *__restrict__ cou
     uint32_t prefix;
     block_excl_scan<4>(part, &prefix);
     const size_t base = (size_t)blockIdx.x * RES_TILE + (size_t)threadIdx.x * 4u;
-    uint4 *c4 = reinterpret_cast<uint4 *>(counts + base);
     uint4 *a4 = reinterpret_cast<uint4 *>(arr + base);
-    uint4 *d4 = reinterpret_cast<uint4 *>(dep + base);
-    const uint4 v = *c4, w = *a4, d = *d4;
-    const uint32_t c[4] = {v.x - d.x + w.x, v.y - d.y + w.y, v.z - d.z + w.z, v.w - d.w + w.w};
+    const uint4 v = reinterpret_cast<const uint4 *>(counts + base)[0], w = *a4;
+    const uint32_t c[4] = {v.x, v.y, v.z, v.w};
     const uint32_t a[4] = {w.x, w.y, w.z, w.w};
-    *c4 = make_uint4(c[0], c[1], c[2], c[3]);
     uint32_t bits = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) bits |= (c[k] != 0u ? 1u : 0u) << k;
@@ -505,7 +501,6 @@ __global__ __launch_bounds__(256) void k_r_scan_apply(uint32_t *__restrict__ cou
     reinterpret_cast<uint4 *>(noff + base)[0] = make_uint4(o[0], o[1], o[2], o[3]);
     reinterpret_cast<uint4 *>(abase + base)[0] = make_uint4(ab[0], ab[1], ab[2], ab[3]);
     *a4 = make_uint4(0u, 0u, 0u, 0u);
-    *d4 = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // Movers among old slots [lo, hi) (lo <= hi), from the mover bits.
@@ -576,7 +571,7 @@ hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Reco
                                    uint32_t ntick, hipStream_t s) {
     const uint32_t ntiles = (g.n_buckets + RES_TILE - 1) / RES_TILE;
     if (ntiles == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_r_scan_apply, dim3(ntiles), dim3(256), 0, s, p.counts, w.arr, w.dep, w.tsum, ntiles,
+    hipLaunchKernelGGL(k_r_scan_apply, dim3(ntiles), dim3(256), 0, s, w.counts, w.arr, w.tsum, ntiles,
                        off_nxt, w.abase, p.occ, tick, tick ? ntick : 0u);
     if (g.n_slots)
         hipLaunchKernelGGL(k_r_move, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, cur, nxt, w,

@@ -115,7 +115,8 @@ struct ResWork {
     uint32_t *knew;   // per old slot: bucket of the new position
     uint32_t *arank;  // per old slot (movers): arrival rank in the new bucket
     uint32_t *mbits;  // per old slot: mover bit (32 slots per word)
-    uint32_t *arr, *dep;  // per bucket: arrivals, departures (zero between steps)
+    uint32_t *arr;     // per bucket: arrivals (zero between steps)
+    uint32_t *counts;  // per bucket: birds in the layout (the flight moves each mover's count)
     uint32_t *abase;      // per bucket: first slot of its arrivals in the new layout
     uint32_t *tsum;       // per RES_TILE buckets: their count total, kept from step to step
 };
