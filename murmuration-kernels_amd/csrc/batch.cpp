@@ -94,6 +94,47 @@ static fh_status push_tables(fh_batch *b, uint32_t t_lo, uint32_t t_hi) {
     return FH_OK;
 }
 
+// Pinned staging of at least `bytes` (hipHostMalloc). The stream is
+// synchronised first, so no copy of an earlier call still reads or writes the
+// buffer when the caller fills it or when it is replaced by a larger one.
+static fh_status pinned(fh_batch *b, size_t bytes) {
+    FH_HIP(hipStreamSynchronize(b->stream));
+    if (b->pin_bytes >= bytes) return FH_OK;
+    if (b->pin) {
+        (void)hipHostFree(b->pin);
+        b->pin = nullptr;
+        b->pin_bytes = 0;
+    }
+    const size_t want = std::max<size_t>(bytes, (size_t)1 << 20);
+    hipError_t e = hipHostMalloc(&b->pin, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        b->pin = nullptr;
+        return hip_fail(e, "hipHostMalloc(batch staging)");
+    }
+    b->pin_bytes = want;
+    return FH_OK;
+}
+
+// The six per-bird arrays of n birds laid out one after another in the
+// staging buffer (pos, vel, steer: float4; spin, cen: float; nb: uint32).
+struct Staged {
+    float4 *pos, *vel, *steer;
+    float *spin, *cen;
+    uint32_t *nb;
+};
+static Staged staged(void *pin, uint32_t n) {
+    char *q = static_cast<char *>(pin);
+    Staged st;
+    st.pos = reinterpret_cast<float4 *>(q);
+    st.vel = st.pos + n;
+    st.steer = st.vel + n;
+    st.spin = reinterpret_cast<float *>(st.steer + n);
+    st.cen = st.spin + n;
+    st.nb = reinterpret_cast<uint32_t *>(st.cen + n);
+    return st;
+}
+static size_t staged_bytes(uint32_t n) { return (size_t)n * (3 * sizeof(float4) + 3 * 4); }
+
 static fh_status batch_hash(fh_batch *b) {
     if (b->counts_dirty) FH_HIP(launch_fill(b->P.counts, b->bucket_cap_units * SCAN_TILE, 0u, b->stream));
     FH_HIP(launch_hash_count(b->geo, b->P, (b->prm.flags & FH_FLAG_WAVE_AGG) != 0, b->stream));
@@ -530,6 +571,7 @@ fh_status fh_batch_destroy(fh_batch *b) {
     for (auto &gx : b->rgraph)
         if (gx) (void)hipGraphExecDestroy(gx);
     if (b->res_arena) (void)hipFree(b->res_arena);
+    if (b->pin) (void)hipHostFree(b->pin);
     (void)hipFree(b->arena);
     (void)hipStreamDestroy(b->stream);
     delete b;
@@ -580,26 +622,25 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
     b->times[f] = time;
     for (uint32_t t = 0; t < ntile; ++t) b->tile_seg[t0 + t] = f;
 
-    std::vector<float4> pos(n), vel(n), steer(n);
-    std::vector<float> spin(n), cen(n);
-    std::vector<uint32_t> nbv(n);
+    if ((st = pinned(b, staged_bytes(n)))) return st;
+    const Staged h = staged(b->pin, n);
     for (uint32_t i = 0; i < n; ++i) {
         const float *w = words + (size_t)i * FH_WORDS;
-        pos[i] = make_float4(w[PX], w[PY], w[PZ], w[NEXT]);
-        vel[i] = make_float4(w[VX], w[VY], w[VZ], w[ROLL]);
-        steer[i] = make_float4(w[SX], w[SY], w[SZ], w[RADIUS]);
-        spin[i] = w[SPIN];
-        std::memcpy(&nbv[i], &w[NB], 4);
-        cen[i] = w[CEN];
+        h.pos[i] = make_float4(w[PX], w[PY], w[PZ], w[NEXT]);
+        h.vel[i] = make_float4(w[VX], w[VY], w[VZ], w[ROLL]);
+        h.steer[i] = make_float4(w[SX], w[SY], w[SZ], w[RADIUS]);
+        h.spin[i] = w[SPIN];
+        std::memcpy(&h.nb[i], &w[NB], 4);
+        h.cen[i] = w[CEN];
     }
     const size_t o = s.bird_base;
-    FH_HIP(hipMemcpyAsync(b->P.pos4 + o, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
-    FH_HIP(hipMemcpyAsync(b->P.vel4 + o, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
-    FH_HIP(hipMemcpyAsync(b->P.steer4 + o, steer.data(), n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
-    FH_HIP(hipMemcpyAsync(b->P.spin + o, spin.data(), n * 4, hipMemcpyHostToDevice, b->stream));
-    FH_HIP(hipMemcpyAsync(b->P.cen + o, cen.data(), n * 4, hipMemcpyHostToDevice, b->stream));
-    FH_HIP(hipMemcpyAsync(b->P.nb + o, nbv.data(), n * 4, hipMemcpyHostToDevice, b->stream));
-    if ((st = push_tables(b, t0, t0 + ntile))) return st;
+    FH_HIP(hipMemcpyAsync(b->P.pos4 + o, h.pos, n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
+    FH_HIP(hipMemcpyAsync(b->P.vel4 + o, h.vel, n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
+    FH_HIP(hipMemcpyAsync(b->P.steer4 + o, h.steer, n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
+    FH_HIP(hipMemcpyAsync(b->P.spin + o, h.spin, n * 4, hipMemcpyHostToDevice, b->stream));
+    FH_HIP(hipMemcpyAsync(b->P.cen + o, h.cen, n * 4, hipMemcpyHostToDevice, b->stream));
+    FH_HIP(hipMemcpyAsync(b->P.nb + o, h.nb, n * 4, hipMemcpyHostToDevice, b->stream));
+    if ((st = push_tables(b, t0, t0 + ntile))) return st;  // synchronises: the staging is free again
     // Count the new flock's birds into the histogram if the others are already
     // counted; otherwise the next step hashes everything.
     refresh_geometry(b);
@@ -643,28 +684,27 @@ fh_status fh_batch_download(fh_batch *b, uint32_t flock_id, float *words, float 
     const uint32_t n = s.n;
     if (!words) return set_error(FH_ERR_INVALID, "fh_batch_download: NULL words");
     if ((st = batch_leave_resident(b))) return st;
-    std::vector<float4> pos(n), vel(n), steer(n);
-    std::vector<float> spin(n), cen(n);
-    std::vector<uint32_t> nbv(n);
+    if ((st = pinned(b, staged_bytes(n)))) return st;
+    const Staged h = staged(b->pin, n);
     const size_t o = s.bird_base;
-    FH_HIP(hipMemcpyAsync(pos.data(), b->P.pos4 + o, n * sizeof(float4), hipMemcpyDeviceToHost, b->stream));
-    FH_HIP(hipMemcpyAsync(vel.data(), b->P.vel4 + o, n * sizeof(float4), hipMemcpyDeviceToHost, b->stream));
-    FH_HIP(hipMemcpyAsync(steer.data(), b->P.steer4 + o, n * sizeof(float4), hipMemcpyDeviceToHost, b->stream));
-    FH_HIP(hipMemcpyAsync(spin.data(), b->P.spin + o, n * 4, hipMemcpyDeviceToHost, b->stream));
-    FH_HIP(hipMemcpyAsync(cen.data(), b->P.cen + o, n * 4, hipMemcpyDeviceToHost, b->stream));
-    FH_HIP(hipMemcpyAsync(nbv.data(), b->P.nb + o, n * 4, hipMemcpyDeviceToHost, b->stream));
+    FH_HIP(hipMemcpyAsync(h.pos, b->P.pos4 + o, n * sizeof(float4), hipMemcpyDeviceToHost, b->stream));
+    FH_HIP(hipMemcpyAsync(h.vel, b->P.vel4 + o, n * sizeof(float4), hipMemcpyDeviceToHost, b->stream));
+    FH_HIP(hipMemcpyAsync(h.steer, b->P.steer4 + o, n * sizeof(float4), hipMemcpyDeviceToHost, b->stream));
+    FH_HIP(hipMemcpyAsync(h.spin, b->P.spin + o, n * 4, hipMemcpyDeviceToHost, b->stream));
+    FH_HIP(hipMemcpyAsync(h.cen, b->P.cen + o, n * 4, hipMemcpyDeviceToHost, b->stream));
+    FH_HIP(hipMemcpyAsync(h.nb, b->P.nb + o, n * 4, hipMemcpyDeviceToHost, b->stream));
     FH_HIP(hipStreamSynchronize(b->stream));
     for (uint32_t i = 0; i < n; ++i) {
         float *w = words + (size_t)i * FH_WORDS;
-        w[PX] = pos[i].x; w[PY] = pos[i].y; w[PZ] = pos[i].z;
-        w[VX] = vel[i].x; w[VY] = vel[i].y; w[VZ] = vel[i].z;
-        w[ROLL] = vel[i].w;
-        w[SPIN] = spin[i];
-        w[RADIUS] = steer[i].w;
-        w[SX] = steer[i].x; w[SY] = steer[i].y; w[SZ] = steer[i].z;
-        w[NEXT] = pos[i].w;
-        std::memcpy(&w[NB], &nbv[i], 4);
-        w[CEN] = cen[i];
+        w[PX] = h.pos[i].x; w[PY] = h.pos[i].y; w[PZ] = h.pos[i].z;
+        w[VX] = h.vel[i].x; w[VY] = h.vel[i].y; w[VZ] = h.vel[i].z;
+        w[ROLL] = h.vel[i].w;
+        w[SPIN] = h.spin[i];
+        w[RADIUS] = h.steer[i].w;
+        w[SX] = h.steer[i].x; w[SY] = h.steer[i].y; w[SZ] = h.steer[i].z;
+        w[NEXT] = h.pos[i].w;
+        std::memcpy(&w[NB], &h.nb[i], 4);
+        w[CEN] = h.cen[i];
     }
     if (time) *time = b->times[flock_id];
     return FH_OK;

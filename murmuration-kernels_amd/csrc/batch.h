@@ -65,6 +65,11 @@ struct fh_batch {
     uint32_t parity = 0;
     hipGraphExec_t rgraph[2]{};
     bool rgraph_valid = false;
+    // Pinned host staging for fh_batch_add / fh_batch_download (grown on
+    // demand, reused): copies from pageable memory are staged by the runtime
+    // and run at a fraction of the link's speed.
+    void *pin = nullptr;
+    size_t pin_bytes = 0;
 };
 
 namespace fh {
