@@ -236,8 +236,18 @@ fh_status fh_batch_destroy(fh_batch *b);
 fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, uint32_t *flock_id);
 fh_status fh_batch_remove(fh_batch *b, uint32_t flock_id);
 fh_status fh_batch_download(fh_batch *b, uint32_t flock_id, float *words, float *time);
-/* `steps` steps of every live flock (time += DT each). */
+/* `steps` steps of every live flock (time += DT each), except the flocks of a
+ * resident batch whose step budget is used up. */
 fh_status fh_batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *out);
+/* Resident batches (FH_FLAG_RESIDENT): the flock takes at most `steps` more
+ * steps of the coming fh_batch_step calls; after them it is frozen (state and
+ * clock kept) until its budget is set again or it is removed, so a serving
+ * loop can step many flocks of different lengths together and collect them
+ * once in a while. FH_STEPS_UNLIMITED lifts the budget (the default).
+ * FH_ERR_INVALID on a batch without FH_FLAG_RESIDENT. No counterpart in the
+ * reference, which steps one flock per process. */
+#define FH_STEPS_UNLIMITED 0xffffffffu
+fh_status fh_batch_limit(fh_batch *b, uint32_t flock_id, uint32_t steps);
 fh_status fh_batch_info(const fh_batch *b, uint32_t *live_flocks, uint32_t *live_birds, uint32_t *bucket_range,
                         size_t *device_bytes);
 /* Device bytes one flock of n birds occupies in a batch (state, sorted copies,

@@ -379,8 +379,11 @@ fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *o
             t0 = std::chrono::steady_clock::now();
             if (out && graph) (void)hipEventRecord(ev[0], b->stream);
         }
-        for (size_t f = 0; f < b->segs.size(); ++f)
-            if (b->segs[f].live) b->times[f] += DT;  // mirrors the device clock add
+        for (size_t f = 0; f < b->segs.size(); ++f)  // mirrors the device clock add (and budget)
+            if (b->segs[f].live && (!b->resident || b->segs[f].left)) {
+                b->times[f] += DT;
+                if (b->resident && b->segs[f].left != SEG_UNLIMITED) b->segs[f].left -= 1u;
+            }
         if (b->resident) {
             if (graph) {
                 if ((st = batch_capture_resident(b))) break;
@@ -708,6 +711,18 @@ fh_status fh_batch_download(fh_batch *b, uint32_t flock_id, float *words, float 
     }
     if (time) *time = b->times[flock_id];
     return FH_OK;
+}
+
+fh_status fh_batch_limit(fh_batch *b, uint32_t flock_id, uint32_t steps) {
+    clear_error();
+    fh_status st = bcheck(b, "fh_batch_limit");
+    if (st) return st;
+    if (!(b->prm.flags & FH_FLAG_RESIDENT))
+        return set_error(FH_ERR_INVALID, "fh_batch_limit: step budgets need FH_FLAG_RESIDENT");
+    if (flock_id >= b->segs.size() || !b->segs[flock_id].live)
+        return set_error(FH_ERR_INVALID, "fh_batch_limit: no live flock %u", flock_id);
+    b->segs[flock_id].left = steps;
+    return push_tables(b, 0, 0);
 }
 
 fh_status fh_batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *out) {

@@ -491,8 +491,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
     // records: this step's clock is the bird's flock clock + DT (committed by
     // the pass that follows) and p4.w holds next_decision.
     for (uint32_t s = start + threadIdx.x; s < end; s += BLOCK) {
-        const bool due = RES ? g.segs[slot_segment(g, rec, s)].time + DT >= cand_pos<STRIDE>(cand, s).w
-                             : time >= p.snext[s];
+        bool due;
+        if (RES) {
+            const Seg &sg = g.segs[slot_segment(g, rec, s)];
+            due = sg.left && sg.time + DT >= cand_pos<STRIDE>(cand, s).w;  // frozen flocks never decide
+        } else {
+            due = time >= p.snext[s];
+        }
         if (due) due_slots[atomicAdd(&n_due, 1u)] = s;
     }
     __syncthreads();

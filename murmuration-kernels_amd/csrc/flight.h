@@ -63,7 +63,10 @@ __device__ __forceinline__ void resident_flight(const Geometry &g, const Records
         float4 p4 = in.p4[s];
         float4 v4 = in.v4[s];
         float rate;
-        integrate_bird(p4, v4, in.s4[s], rate);
+        if (sg.left)
+            integrate_bird(p4, v4, in.s4[s], rate);
+        else
+            rate = in.spin[s];  // a frozen flock keeps its state
         out.p4[s] = p4;
         out.v4[s] = v4;
         out.spin[s] = rate;

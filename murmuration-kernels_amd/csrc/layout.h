@@ -59,9 +59,11 @@ struct Seg {
     float time;            // clock of the step in flight
     uint32_t live;         // 0 for a freed segment
     uint32_t mode;         // SEG_* bits
+    uint32_t left = 0xffffffffu;  // resident steps left (SEG_UNLIMITED: no budget); 0: frozen
 };
 
 constexpr uint32_t SEG_SPATIAL = 1u;  // buckets by spatial_hash instead of the reference hash
+constexpr uint32_t SEG_UNLIMITED = 0xffffffffu;
 
 // Bucket of cell (cx, cy, cz) in segment sg.
 __device__ __forceinline__ uint32_t bucket_of_cell(const Seg &sg, int32_t cx, int32_t cy, int32_t cz) {
@@ -138,10 +140,15 @@ __device__ __forceinline__ uint32_t slot_segment(const Geometry &g, const Record
     return g.tile_seg ? g.tile_seg[rec.id[s] / TILE] : 0u;
 }
 
-// Advances the clock of every live segment by DT (one thread).
+// Advances the clock of every live segment with steps left by DT and spends
+// one step of its budget (one thread). A segment with no steps left is
+// frozen: its birds are neither due nor moved, and its clock stays.
 __device__ __forceinline__ void commit_clocks(Seg *segs, uint32_t nseg) {
     for (uint32_t i = 0; i < nseg; ++i)
-        if (segs[i].live) segs[i].time = segs[i].time + DT;
+        if (segs[i].live && segs[i].left) {
+            segs[i].time = segs[i].time + DT;
+            if (segs[i].left != SEG_UNLIMITED) segs[i].left -= 1u;
+        }
 }
 
 // Launchers (stream-ordered, no host synchronisation). Each returns the

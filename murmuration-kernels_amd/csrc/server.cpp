@@ -129,6 +129,9 @@ static void fail_server(fh_server *s, fh_status st) {
     s->fatal_msg = fh_last_error();
 }
 
+// Resident batches take per-flock step budgets (fh_batch_limit).
+static bool budgets(const fh_server *s) { return (s->cfg.params.flags & FH_FLAG_RESIDENT) != 0; }
+
 static void worker_loop(fh_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     const double busy0 = s->now();
@@ -154,6 +157,10 @@ static void worker_loop(fh_server *s) {
             }
             j.flock = fid;
             j.left = j.res.steps;
+            if (budgets(s) && (st = fh_batch_limit(s->batch, fid, j.left))) {
+                fail_server(s, st);
+                break;
+            }
             j.res.admit_s = s->now();
             if (!s->cfg.keep_states) std::vector<float>().swap(j.words);
             s->bytes_active += bytes;
@@ -166,13 +173,15 @@ static void worker_loop(fh_server *s) {
             s->cv_work.wait_for(lk, std::chrono::milliseconds(2));
             continue;
         }
-        // Step every active flock together; stop at the first completion.
+        // Step every active flock together. With step budgets (resident
+        // batch) each flock freezes on its own last step and the iteration
+        // runs its full length; otherwise it stops at the first completion.
         uint32_t k = s->cfg.steps_per_iter ? s->cfg.steps_per_iter : 8;
-        uint64_t birds = 0;
-        for (uint64_t id : s->active) {
-            k = std::min(k, s->jobs[id].left);
-            birds += s->jobs[id].res.birds;
-        }
+        if (!budgets(s))
+            for (uint64_t id : s->active) k = std::min(k, s->jobs[id].left);
+        double bird_steps = 0.0;
+        for (uint64_t id : s->active)
+            bird_steps += (double)s->jobs[id].res.birds * std::min(k, s->jobs[id].left);
         lk.unlock();
         const double t_a = s->now();
         fh_status st = batch_step(s->batch, k, 0, nullptr);
@@ -184,12 +193,12 @@ static void worker_loop(fh_server *s) {
         }
         s->st.busy_s += t_b - t_a;
         s->st.iterations += 1;
-        s->st.bird_steps += (double)birds * k;
+        s->st.bird_steps += bird_steps;
         s->st.steps += k;
         // Retire finished flocks.
         for (size_t i = 0; i < s->active.size();) {
             Job &j = s->jobs[s->active[i]];
-            j.left -= k;
+            j.left -= std::min(k, j.left);
             if (j.left) {
                 ++i;
                 continue;

@@ -116,6 +116,7 @@ _SIGS = {
     "fh_batch_destroy": ([_P], C.c_int32),
     "fh_batch_add": ([_P, _P, _u32, C.c_float, _P], C.c_int32),
     "fh_batch_remove": ([_P, _u32], C.c_int32),
+    "fh_batch_limit": ([_P, _u32, _u32], C.c_int32),
     "fh_batch_download": ([_P, _u32, _P, _P], C.c_int32),
     "fh_batch_step": ([_P, _u32, _u32, _P], C.c_int32),
     "fh_batch_info": ([_P, _P, _P, _P, _P], C.c_int32),
@@ -394,6 +395,10 @@ class Batch:
     def remove(self, flock_id: int) -> None:
         _check(lib().fh_batch_remove(self._h, flock_id))
         self.sizes.pop(flock_id, None)
+
+    def limit(self, flock_id: int, steps: int = 0xFFFFFFFF) -> None:
+        """Resident batches: at most `steps` more steps for this flock, then frozen."""
+        _check(lib().fh_batch_limit(self._h, flock_id, steps))
 
     def download(self, flock_id: int) -> tuple[np.ndarray, float]:
         w = np.zeros((self.sizes[flock_id], WORDS), np.float32)

@@ -114,3 +114,35 @@ def test_resident_batch_rejects_canonical(fh):
     with pytest.raises(fh.FlockHipError) as e:
         fh.Batch(1000, 1 << 16, 2, flags=RES | CANON)
     assert e.value.status == fh.ERR_INVALID
+
+
+def test_step_budgets_freeze_each_flock_on_its_last_step(fh):
+    # Budgets of 3, 7 and unlimited steps, then 10 steps of the batch: each
+    # flock matches that many steps alone, and a frozen flock continues once
+    # its budget is raised.
+    starts = [cube(fh, n, 50 + n) for n in (4000, 9000, 2500)]
+    with fh.Batch(50_000, 1 << 21, 4, flags=RES | GRAPH | 2048) as b:
+        ids = [b.add(w) for w in starts]
+        b.limit(ids[0], 3)
+        b.limit(ids[1], 7)
+        b.step(6)
+        b.step(4)
+        got = [b.download(i) for i in ids]
+        b.limit(ids[0], 2)  # thawed for two more steps
+        b.step(5)
+        again, t_again = b.download(ids[0])
+    for (g, t), w, k in zip(got, starts, (3, 7, 10)):
+        want, tw = alone(fh, w, k, CANON)
+        assert t == tw, k
+        close(g, want)
+    want, tw = alone(fh, starts[0], 5, CANON)
+    assert t_again == tw
+    close(again, want)
+
+
+def test_step_budgets_need_the_resident_layout(fh):
+    with fh.Batch(10_000, 1 << 18, 2, flags=CANON) as b:
+        fid = b.add(cube(fh, 100, 1))
+        with pytest.raises(fh.FlockHipError) as e:
+            b.limit(fid, 5)
+        assert e.value.status == fh.ERR_INVALID
