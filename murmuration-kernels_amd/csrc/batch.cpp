@@ -214,7 +214,7 @@ static fh_status batch_alloc_resident(fh_batch *b) {
         o_id[q] = a.take<uint32_t>(nn);
         o_key[q] = a.take<uint32_t>(nn);
     }
-    const size_t o_knew = a.take<uint32_t>(nn), o_arank = a.take<uint32_t>(nn);
+    const size_t o_arank = a.take<uint32_t>(nn);
     const size_t o_mbits = a.take<uint32_t>((nn + TILE - 1) / TILE * (TILE / 32));
     const size_t o_arr = a.take<uint32_t>(nb), o_abase = a.take<uint32_t>(nb);
     const size_t o_off1 = a.take<uint32_t>(nb + 1), o_due = a.take<uint32_t>(4);
@@ -235,7 +235,6 @@ static fh_status batch_alloc_resident(fh_batch *b) {
         b->R[q].id = reinterpret_cast<uint32_t *>(base + o_id[q]);
         b->R[q].key = reinterpret_cast<uint32_t *>(base + o_key[q]);
     }
-    b->W.knew = reinterpret_cast<uint32_t *>(base + o_knew);
     b->W.arank = reinterpret_cast<uint32_t *>(base + o_arank);
     b->W.mbits = reinterpret_cast<uint32_t *>(base + o_mbits);
     b->W.arr = reinterpret_cast<uint32_t *>(base + o_arr);
@@ -348,8 +347,8 @@ static fh_status batch_capture_resident(fh_batch *b) {
 namespace fh {
 
 size_t resident_bytes_per_bird(bool flight) {
-    // two records (p4, v4, s4: 48; spin, id, key: 12), knew, arank, mover bits
-    return 2 * 60 + 8 + 1 + (flight ? 36 : 0);
+    // two records (p4, v4, s4: 48; spin, id, key: 12), arank, mover bits
+    return 2 * 60 + 4 + 1 + (flight ? 36 : 0);
 }
 size_t resident_bytes_per_bucket() { return 4 /*arr*/ + 4 /*abase*/ + 4 /*offsets*/ + 1 /*tile totals*/; }
 

@@ -73,7 +73,6 @@ __device__ __forceinline__ void resident_flight(const Geometry &g, const Records
         const uint32_t kn = bucket_of_cell(sg, cell_coord(p4.x, sg.inv_cell), cell_coord(p4.y, sg.inv_cell),
                                            cell_coord(p4.z, sg.inv_cell));
         const uint32_t ko = in.key[s];
-        w.knew[s] = kn;
         if (kn != ko) {
             mover = true;
             w.arank[s] = atomicAdd(&w.arr[kn], 1u);

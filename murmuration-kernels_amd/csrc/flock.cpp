@@ -183,7 +183,7 @@ static fh_status alloc_resident(fh_flock *h) {
         o_id[q] = a.take<uint32_t>(nn);
         o_key[q] = a.take<uint32_t>(nn);
     }
-    const size_t o_knew = a.take<uint32_t>(nn), o_arank = a.take<uint32_t>(nn);
+    const size_t o_arank = a.take<uint32_t>(nn);
     const size_t o_mbits = a.take<uint32_t>((nn + TILE - 1) / TILE * (TILE / 32));
     const size_t o_arr = a.take<uint32_t>(nb), o_abase = a.take<uint32_t>(nb);
     const bool flight = (h->prm.flags & FH_FLAG_FLIGHT) != 0;
@@ -205,7 +205,6 @@ static fh_status alloc_resident(fh_flock *h) {
         h->R[q].id = reinterpret_cast<uint32_t *>(base + o_id[q]);
         h->R[q].key = reinterpret_cast<uint32_t *>(base + o_key[q]);
     }
-    h->W.knew = reinterpret_cast<uint32_t *>(base + o_knew);
     h->W.arank = reinterpret_cast<uint32_t *>(base + o_arank);
     h->W.mbits = reinterpret_cast<uint32_t *>(base + o_mbits);
     h->W.arr = reinterpret_cast<uint32_t *>(base + o_arr);

@@ -524,7 +524,12 @@ __global__ __launch_bounds__(TILE) void k_r_move(Geometry g, Pools p, Records cu
                                                  const uint32_t *__restrict__ noff) {
     const uint32_t s = blockIdx.x * TILE + threadIdx.x;
     if (s >= g.n_slots) return;
-    const uint32_t ko = cur.key[s], kn = w.knew[s];
+    // the bucket of the new position, by its flock's hash (as the flight found it)
+    const float4 p4 = cur.p4[s];
+    const Seg &sg = g.segs[slot_segment(g, cur, s)];
+    const uint32_t ko = cur.key[s],
+                   kn = bucket_of_cell(sg, cell_coord(p4.x, sg.inv_cell), cell_coord(p4.y, sg.inv_cell),
+                                       cell_coord(p4.z, sg.inv_cell));
     uint32_t ns;
     if (kn == ko) {
         const uint32_t o = off_cur[ko];
@@ -540,7 +545,7 @@ __global__ __launch_bounds__(TILE) void k_r_move(Geometry g, Pools p, Records cu
         atomicOr(p.err, ERR_SLOT);
         return;
     }
-    nxt.p4[ns] = cur.p4[s];
+    nxt.p4[ns] = p4;
     nxt.v4[ns] = cur.v4[s];
     nxt.s4[ns] = cur.s4[s];
     nxt.spin[ns] = cur.spin[s];

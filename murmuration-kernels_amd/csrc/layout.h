@@ -114,7 +114,6 @@ struct Records {
 // differs from its old one ("mover") counts itself out of the old bucket
 // (dep) and into the new one (arr, returning its arrival rank).
 struct ResWork {
-    uint32_t *knew;   // per old slot: bucket of the new position
     uint32_t *arank;  // per old slot (movers): arrival rank in the new bucket
     uint32_t *mbits;  // per old slot: mover bit (32 slots per word)
     uint32_t *arr;     // per bucket: arrivals (zero between steps)
