@@ -432,6 +432,11 @@ def _replay(fh, req, args, prm, flags) -> dict:
         "peak_active": st["peak_active"],
         "peak_hbm_bytes": st["peak_bytes"],
         "iterations": st["iterations"],
+        # where the server's wall time went: batched steps, and the GPU-idle
+        # admissions (add) and retirements (download + remove)
+        "busy_s": st["busy_s"],
+        "admit_s": st["admit_s"],
+        "retire_s": st["retire_s"],
     }
 
 

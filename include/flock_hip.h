@@ -290,6 +290,8 @@ typedef struct fh_server_stats {
     double wall_s;             /* time since the server started */
     double peak_bytes;         /* largest admitted HBM bytes */
     uint32_t peak_active, queued, active;
+    double admit_s;            /* time adding admitted flocks to the batch (GPU idle meanwhile) */
+    double retire_s;           /* time downloading and removing finished flocks (GPU idle meanwhile) */
 } fh_server_stats;
 
 typedef struct fh_server fh_server;

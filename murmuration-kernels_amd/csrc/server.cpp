@@ -144,7 +144,9 @@ static void worker_loop(fh_server *s) {
             const double bytes = flock_bytes_d(s, j.res.birds);
             if (s->bytes_active + bytes > s->cfg.hbm_budget_bytes) break;
             uint32_t fid = 0;
+            const double t_add = s->now();
             fh_status st = fh_batch_add(s->batch, j.words.data(), j.res.birds, 0.0f, &fid);
+            s->st.admit_s += s->now() - t_add;
             if (st == FH_ERR_BUDGET && !s->active.empty()) break;  // pools fragmented: wait for room
             if (st) {
                 j.res.status = st;
@@ -203,10 +205,12 @@ static void worker_loop(fh_server *s) {
                 ++i;
                 continue;
             }
+            const double t_ret = s->now();
             std::vector<float> out((size_t)j.res.birds * FH_WORDS);
             float t = 0.0f;
             st = fh_batch_download(s->batch, j.flock, out.data(), &t);
             if (!st) st = fh_batch_remove(s->batch, j.flock);
+            s->st.retire_s += s->now() - t_ret;
             if (st) {
                 fail_server(s, st);
                 break;

@@ -463,6 +463,8 @@ class ServerStats(C.Structure):
         ("peak_active", C.c_uint32),
         ("queued", C.c_uint32),
         ("active", C.c_uint32),
+        ("admit_s", C.c_double),
+        ("retire_s", C.c_double),
     ]
 
     def as_dict(self) -> dict:
