@@ -77,10 +77,23 @@ def tune_cache(args, label: str) -> dict:
     return {"cache_path": args.tune_cache or None, "label": label, "refresh": args.retune}
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fp:
+            for line in fp:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(words: np.ndarray, n: int, sample: int) -> dict:
     """The oracle's brute-force step (single thread) on a bounded sample:
     `sample` due birds' decisions plus step_bird for every bird, extrapolated
-    to one whole step (all due birds)."""
+    to one whole step (all due birds). Beside it, labelled as a variant, the
+    same decisions in parallel over the due birds on up to 16 host threads
+    (threads x `sample` of them)."""
     from oracle import flockref as fr
 
     t = np.float32(0.0) + np.float32(0.005)
@@ -95,6 +108,12 @@ def cpu_baseline(words: np.ndarray, n: int, sample: int) -> dict:
         fr.step_bird(words[i])
     per_bird = (time.perf_counter() - t0) / len(range(0, n, max(1, n // 200_000)))
     step_s = t_dec * len(due) + per_bird * n
+    threads = max(1, min(16, os.cpu_count() or 1))
+    many = due[: min(sample * threads, len(due))]
+    t0 = time.perf_counter()
+    fr.decide_many(words, many, threads=threads)
+    t_dec_mt = (time.perf_counter() - t0) / max(1, len(many))
+    step_mt = t_dec_mt * len(due) + per_bird * n / threads
     return {
         "value": n / step_s,
         "unit": "bird-steps/s",
@@ -103,6 +122,14 @@ def cpu_baseline(words: np.ndarray, n: int, sample: int) -> dict:
         "sample": (f"one step of the C oracle (brute force, single thread): decide timed on {len(pick)} of the "
                    f"{len(due)} due birds, step_bird timed on a stride sample of the {n} birds, "
                    f"extrapolated to the whole step ({step_s:.1f} s)"),
+        "multicore_variant": {
+            "value": n / step_mt,
+            "threads": threads,
+            "sample": (f"variant: the same brute-force decide in parallel over {len(many)} due birds (OpenMP, "
+                       f"{threads} threads), step_bird's single-thread time divided by the threads, "
+                       f"extrapolated to the whole step ({step_mt:.1f} s)"),
+        },
+        "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model()},
     }
 
 

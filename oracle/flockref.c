@@ -593,6 +593,17 @@ void fr_decide_grid(const float *words, uint32_t n, uint32_t i, float z0, float 
     finish_decision(&a, me, z0, out);
 }
 
+/* fr_decide (model.rs:53-145, brute force) for count birds, in parallel over
+ * the birds: the multi-core CPU baseline. out[k * 6 ...] as fr_decide's. */
+void fr_decide_many(const float *words, uint32_t n, const uint32_t *birds, uint32_t count, float z0, float *out,
+                    int threads) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads > 0 ? threads : 1)
+#endif
+    for (int64_t k = 0; k < (int64_t)count; ++k) fr_decide(words, n, birds[k], z0, out + (size_t)k * 6);
+    (void)threads;
+}
+
 void fr_decide_grid_many(const float *words, uint32_t n, const uint32_t *birds, uint32_t count, float z0,
                          float cell, uint32_t buckets, const uint32_t *offsets, const uint32_t *sorted,
                          float *out, int threads) {

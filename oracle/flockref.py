@@ -50,6 +50,7 @@ _SIGS = {
     "fr_neighbours_brute": ([_P, _u32, _u32, C.c_int, _P, _u32], _u32),
     "fr_decide_grid": ([_P, _u32, _u32, C.c_float, C.c_float, _u32, _P, _P, _P], None),
     "fr_step_grid": ([_P, _u32, _P, _u32, C.c_float, _u32, C.c_int], None),
+    "fr_decide_many": ([_P, _u32, _P, _u32, C.c_float, _P, C.c_int], None),
     "fr_decide_grid_many": ([_P, _u32, _P, _u32, C.c_float, C.c_float, _u32, _P, _P, _P, C.c_int], None),
 }
 
@@ -258,6 +259,18 @@ def decide_grid(words: np.ndarray, i: int, offsets, sorted_, buckets: int, cell:
                          o.ctypes.data, s.ctypes.data, out.ctypes.data)
     return {"force": out[:3].copy(), "radius": out[3], "neighbours": int(out[4:5].view(np.uint32)[0]),
             "centrality": out[5]}
+
+
+def decide_many(words: np.ndarray, birds, z0: float = Z0, threads: int = 1) -> dict:
+    """fr_decide (brute force) for many birds on `threads` threads: arrays
+    force[k, 3], radius, neighbours, centrality."""
+    w = _w(words)
+    idx = np.ascontiguousarray(birds, np.uint32)
+    out = np.zeros((idx.size, 6), np.float32)
+    lib().fr_decide_many(w.ctypes.data, w.shape[0], idx.ctypes.data, idx.size, np.float32(z0), out.ctypes.data,
+                         threads)
+    return {"force": out[:, :3].copy(), "radius": out[:, 3].copy(),
+            "neighbours": out[:, 4].copy().view(np.uint32), "centrality": out[:, 5].copy()}
 
 
 def decide_grid_many(words: np.ndarray, birds, offsets, sorted_, buckets: int, cell: float = 4.0,

@@ -250,6 +250,20 @@ def test_threads_do_not_change_the_result(fr):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_decide_many_equals_decide_per_bird(fr):
+    # the multi-core baseline's entry point: the same brute-force decisions,
+    # bit for bit, whatever the thread count
+    w = fr.scatter(2500, 6)
+    birds = np.arange(0, 2500, 9)
+    for threads in (1, 4):
+        m = fr.decide_many(w, birds, threads=threads)
+        for k, i in enumerate(birds):
+            d = fr.decide(w, int(i))
+            assert np.array_equal(m["force"][k].view(np.uint32), d["force"].view(np.uint32))
+            assert m["neighbours"][k] == d["neighbours"]
+            assert m["radius"][k] == d["radius"] and m["centrality"][k] == d["centrality"]
+
+
 def test_diff_and_statdiff_semantics(fr):
     a = fr.scatter(100, 1)
     b = a.copy()
