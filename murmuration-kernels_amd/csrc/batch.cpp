@@ -122,6 +122,61 @@ struct Staged {
     float *spin, *cen;
     uint32_t *nb;
 };
+ParFor::ParFor(unsigned helpers) : parts_(helpers + 1) {
+    for (unsigned k = 1; k <= helpers; ++k) th_.emplace_back([this, k] { loop(k); });
+}
+
+ParFor::~ParFor() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    go_.notify_all();
+    for (auto &t : th_) t.join();
+}
+
+void ParFor::loop(unsigned part) {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+        go_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        const size_t n = n_;
+        const auto *body = body_;
+        lk.unlock();
+        (*body)(n * part / parts_, n * (part + 1) / parts_);
+        lk.lock();
+        if (--pending_ == 0) done_.notify_one();
+    }
+}
+
+void ParFor::run(size_t n, const std::function<void(size_t, size_t)> &body) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        body_ = &body;
+        n_ = n;
+        pending_ = parts_ - 1;
+        ++gen_;
+    }
+    go_.notify_all();
+    body(0, n / parts_);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [&] { return pending_ == 0; });
+}
+
+// The .flk word <-> pool conversions of one flock: on the calling thread for
+// small flocks, split four ways for large ones (they run while the GPU idles
+// in the served loop's admissions and retirements).
+static void convert(fh_batch *b, uint32_t n, const std::function<void(size_t, size_t)> &body) {
+    if (n < (1u << 15)) {
+        body(0, n);
+        return;
+    }
+    if (!b->par) b->par = std::make_unique<ParFor>(3u);
+    b->par->run(n, body);
+}
+
 static Staged staged(void *pin, uint32_t n) {
     char *q = static_cast<char *>(pin);
     Staged st;
@@ -626,15 +681,17 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
 
     if ((st = pinned(b, staged_bytes(n)))) return st;
     const Staged h = staged(b->pin, n);
-    for (uint32_t i = 0; i < n; ++i) {
-        const float *w = words + (size_t)i * FH_WORDS;
-        h.pos[i] = make_float4(w[PX], w[PY], w[PZ], w[NEXT]);
-        h.vel[i] = make_float4(w[VX], w[VY], w[VZ], w[ROLL]);
-        h.steer[i] = make_float4(w[SX], w[SY], w[SZ], w[RADIUS]);
-        h.spin[i] = w[SPIN];
-        std::memcpy(&h.nb[i], &w[NB], 4);
-        h.cen[i] = w[CEN];
-    }
+    convert(b, n, [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+            const float *w = words + i * FH_WORDS;
+            h.pos[i] = make_float4(w[PX], w[PY], w[PZ], w[NEXT]);
+            h.vel[i] = make_float4(w[VX], w[VY], w[VZ], w[ROLL]);
+            h.steer[i] = make_float4(w[SX], w[SY], w[SZ], w[RADIUS]);
+            h.spin[i] = w[SPIN];
+            std::memcpy(&h.nb[i], &w[NB], 4);
+            h.cen[i] = w[CEN];
+        }
+    });
     const size_t o = s.bird_base;
     FH_HIP(hipMemcpyAsync(b->P.pos4 + o, h.pos, n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
     FH_HIP(hipMemcpyAsync(b->P.vel4 + o, h.vel, n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
@@ -696,18 +753,20 @@ fh_status fh_batch_download(fh_batch *b, uint32_t flock_id, float *words, float 
     FH_HIP(hipMemcpyAsync(h.cen, b->P.cen + o, n * 4, hipMemcpyDeviceToHost, b->stream));
     FH_HIP(hipMemcpyAsync(h.nb, b->P.nb + o, n * 4, hipMemcpyDeviceToHost, b->stream));
     FH_HIP(hipStreamSynchronize(b->stream));
-    for (uint32_t i = 0; i < n; ++i) {
-        float *w = words + (size_t)i * FH_WORDS;
-        w[PX] = h.pos[i].x; w[PY] = h.pos[i].y; w[PZ] = h.pos[i].z;
-        w[VX] = h.vel[i].x; w[VY] = h.vel[i].y; w[VZ] = h.vel[i].z;
-        w[ROLL] = h.vel[i].w;
-        w[SPIN] = h.spin[i];
-        w[RADIUS] = h.steer[i].w;
-        w[SX] = h.steer[i].x; w[SY] = h.steer[i].y; w[SZ] = h.steer[i].z;
-        w[NEXT] = h.pos[i].w;
-        std::memcpy(&w[NB], &h.nb[i], 4);
-        w[CEN] = h.cen[i];
-    }
+    convert(b, n, [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+            float *w = words + i * FH_WORDS;
+            w[PX] = h.pos[i].x; w[PY] = h.pos[i].y; w[PZ] = h.pos[i].z;
+            w[VX] = h.vel[i].x; w[VY] = h.vel[i].y; w[VZ] = h.vel[i].z;
+            w[ROLL] = h.vel[i].w;
+            w[SPIN] = h.spin[i];
+            w[RADIUS] = h.steer[i].w;
+            w[SX] = h.steer[i].x; w[SY] = h.steer[i].y; w[SZ] = h.steer[i].z;
+            w[NEXT] = h.pos[i].w;
+            std::memcpy(&w[NB], &h.nb[i], 4);
+            w[CEN] = h.cen[i];
+        }
+    });
     if (time) *time = b->times[flock_id];
     return FH_OK;
 }

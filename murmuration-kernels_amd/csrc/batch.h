@@ -1,7 +1,12 @@
 // The batched multi-flock engine shared by fh_batch_* and the served loop.
 #pragma once
 
+#include <condition_variable>
+#include <functional>
 #include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/flock_hip.h"
@@ -25,6 +30,28 @@ class RangeAlloc {
 
    private:
     std::map<uint32_t, uint32_t> free_, used_;  // start -> length
+};
+
+// Splits body(lo, hi) over [0, n) between the calling thread and `helpers`
+// threads that sleep on a condition variable between calls. One caller at a
+// time (a batch is used by one thread at a time).
+class ParFor {
+   public:
+    explicit ParFor(unsigned helpers);
+    ~ParFor();
+    void run(size_t n, const std::function<void(size_t, size_t)> &body);
+
+   private:
+    void loop(unsigned part);
+    const unsigned parts_;
+    std::mutex mu_;
+    std::condition_variable go_, done_;
+    const std::function<void(size_t, size_t)> *body_ = nullptr;
+    size_t n_ = 0;
+    uint64_t gen_ = 0;
+    unsigned pending_ = 0;
+    bool stop_ = false;
+    std::vector<std::thread> th_;
 };
 
 size_t batch_bytes_per_bird();
@@ -70,6 +97,9 @@ struct fh_batch {
     // and run at a fraction of the link's speed.
     void *pin = nullptr;
     size_t pin_bytes = 0;
+    // Helper threads for the record conversions of large flocks (made on
+    // first use).
+    std::unique_ptr<fh::ParFor> par;
 };
 
 namespace fh {

@@ -27,9 +27,32 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
+// Records of one flock. Elements are default-initialised (left as they are):
+// every word is written before it is read, and a zero fill of a fresh
+// multi-megabyte buffer would fault in its pages on the serving thread.
+template <class T>
+struct NoFill : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoFill<U>;
+    };
+    NoFill() = default;
+    template <class U>
+    NoFill(const NoFill<U> &) noexcept {}
+    template <class U>
+    void construct(U *p) noexcept {
+        ::new (static_cast<void *>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+using Words = std::vector<float, NoFill<float>>;
+
 struct Job {
     fh_result res{};
-    std::vector<float> words;  // start state, then final state when kept
+    Words words;  // start state, then final state when kept
     bool ready = false;        // start state generated
     uint32_t flock = 0;        // batch slot while active
     uint32_t left = 0;         // steps still to run
@@ -50,7 +73,7 @@ uint64_t fnv1a_words(const float *w, size_t count) {
 // A retired flock whose checksum is still to be computed.
 struct Finish {
     fh_result res;
-    std::vector<float> words;
+    Words words;
 };
 
 }  // namespace
@@ -69,7 +92,7 @@ struct fh_server {
     std::vector<uint64_t> active;                // in the batch
     std::deque<fh_result> done;
     struct Kept {
-        std::vector<float> words;
+        Words words;
         float time;
     };
     std::unordered_map<uint64_t, Kept> kept;  // final states (keep_states)
@@ -112,7 +135,7 @@ static void prep_loop(fh_server *s) {
         const uint32_t n = it->second.res.birds;
         const uint64_t seed = it->second.res.seed;
         lk.unlock();
-        std::vector<float> w((size_t)n * FH_WORDS);
+        Words w((size_t)n * FH_WORDS);
         fh_gen_request(n, seed, s->cfg.birds_per_cell, s->cfg.cell_for_density, w.data());
         lk.lock();
         it = s->jobs.find(id);
@@ -164,7 +187,7 @@ static void worker_loop(fh_server *s) {
                 break;
             }
             j.res.admit_s = s->now();
-            if (!s->cfg.keep_states) std::vector<float>().swap(j.words);
+            if (!s->cfg.keep_states) Words().swap(j.words);
             s->bytes_active += bytes;
             s->active.push_back(s->queue.front());
             s->queue.pop_front();
@@ -206,7 +229,7 @@ static void worker_loop(fh_server *s) {
                 continue;
             }
             const double t_ret = s->now();
-            std::vector<float> out((size_t)j.res.birds * FH_WORDS);
+            Words out((size_t)j.res.birds * FH_WORDS);
             float t = 0.0f;
             st = fh_batch_download(s->batch, j.flock, out.data(), &t);
             if (!st) st = fh_batch_remove(s->batch, j.flock);
