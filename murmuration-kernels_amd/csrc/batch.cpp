@@ -276,6 +276,7 @@ static fh_status batch_alloc_resident(fh_batch *b) {
     const size_t o_tsum = a.take<uint32_t>(nb / RES_TILE);
     const size_t o_fp4 = flight ? a.take<float4>(nn) : 0, o_fv4 = flight ? a.take<float4>(nn) : 0;
     const size_t o_fspin = flight ? a.take<float>(nn) : 0;
+    const size_t o_shift = a.take<int32_t>(b->flock_cap ? b->flock_cap : 1);
     hipError_t e = hipMalloc(&b->res_arena, a.used);
     if (e != hipSuccess) {
         b->res_arena = nullptr;
@@ -304,6 +305,7 @@ static fh_status batch_alloc_resident(fh_batch *b) {
     b->off[0] = b->P.offsets;
     b->off[1] = reinterpret_cast<uint32_t *>(base + o_off1);
     b->res_due = reinterpret_cast<uint32_t *>(base + o_due);
+    b->d_shift = reinterpret_cast<int32_t *>(base + o_shift);
     // arrivals start at zero and every rebuild zeroes them again
     FH_HIP(hipMemsetAsync(b->res_arena, 0, a.used, b->stream));
     FH_HIP(hipStreamSynchronize(b->stream));
@@ -329,14 +331,92 @@ static fh_status batch_enter_resident(fh_batch *b) {
     return FH_OK;
 }
 
-// Resident records -> bird-order state (before any call that reads or
-// changes the flocks).
-static fh_status batch_leave_resident(fh_batch *b) {
-    if (!b->resident) return FH_OK;
-    FH_HIP(launch_resident_flush(b->geo, b->P, b->R[b->parity], b->stream));
-    b->resident = false;
-    b->hashed = false;
-    b->counts_dirty = true;  // the counts hold the last layout, not a fresh histogram
+// Flocks join and leave a resident batch in place. Each flock's records fill
+// as many consecutive slots as it has birds, flocks in the order of their
+// bucket ranges; a splice moves the records of the flocks after the one that
+// joins or leaves, keeping every record's order, and the offsets follow from
+// a scan of the counts (kept). This replaces a flush of every flock to bird
+// order and a full re-entry (hash, scan and placement of every bird).
+
+// The slot where each live flock's records begin.
+static std::vector<uint32_t> resident_slot_starts(const fh_batch *b) {
+    std::vector<uint32_t> order;
+    for (uint32_t i = 0; i < b->segs.size(); ++i)
+        if (b->segs[i].live) order.push_back(i);
+    std::sort(order.begin(), order.end(),
+              [&](uint32_t x, uint32_t y) { return b->segs[x].bucket_base < b->segs[y].bucket_base; });
+    std::vector<uint32_t> start(b->segs.size(), 0u);
+    uint32_t at = 0;
+    for (uint32_t i : order) {
+        start[i] = at;
+        at += b->segs[i].n;
+    }
+    return start;
+}
+
+// Splice shifts: the live flocks whose buckets lie after flock f's move by
+// `by` slots, f itself by `own`, the rest stay.
+static fh_status push_shifts(fh_batch *b, uint32_t f, int32_t by, int32_t own) {
+    std::vector<int32_t> sh(b->segs.size(), 0);
+    for (uint32_t i = 0; i < b->segs.size(); ++i)
+        if (i != f && b->segs[i].live && b->segs[i].bucket_base > b->segs[f].bucket_base) sh[i] = by;
+    sh[f] = own;
+    FH_HIP(hipMemcpyAsync(b->d_shift, sh.data(), sh.size() * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+    FH_HIP(hipStreamSynchronize(b->stream));  // sh is freed on return
+    return FH_OK;
+}
+
+// Buckets of a flock's range (whole scan tiles).
+static uint32_t seg_bucket_span(const Seg &s) { return (s.mask + SCAN_TILE) / SCAN_TILE * SCAN_TILE; }
+
+// The offsets of the current parity from the counts, and fresh chunk
+// counters for the next step's decide.
+static fh_status resident_rescan(fh_batch *b) {
+    Pools p = b->P;
+    p.offsets = b->off[b->parity];
+    FH_HIP(launch_scan(b->geo, p, nullptr, 0u, 0.0f, b->stream, true));
+    FH_HIP(launch_fill(b->res_due, 4, 0u, b->stream));
+    return FH_OK;
+}
+
+// Flock f leaves (before its tables entries are cleared): the flocks after it
+// move down by its birds; its counts and tile totals become zero.
+static fh_status resident_leave_one(fh_batch *b, uint32_t f) {
+    const Seg s = b->segs[f];
+    const uint32_t q = b->parity;
+    fh_status st = push_shifts(b, f, -(int32_t)s.n, SPLICE_DROP);
+    if (st) return st;
+    FH_HIP(launch_resident_splice(b->geo, b->P, b->R[q], b->R[1u - q], b->d_shift, b->geo.n_slots,
+                                  b->geo.n_slots - s.n, b->stream));
+    const uint32_t span = seg_bucket_span(s);
+    FH_HIP(launch_fill(b->P.counts + s.bucket_base, span, 0u, b->stream));
+    FH_HIP(launch_fill(b->W.tsum + s.bucket_base / RES_TILE, span / RES_TILE, 0u, b->stream));
+    b->parity = 1u - q;
+    return FH_OK;
+}
+
+// Flock f joins (its birds uploaded, its table entries pushed, the geometry
+// refreshed): its histogram, the flocks after it move up by its birds, the
+// offsets and tile totals, and its birds to their slots.
+static fh_status resident_join_one(fh_batch *b, uint32_t f) {
+    const Seg s = b->segs[f];
+    const uint32_t q = b->parity;
+    const uint32_t span = seg_bucket_span(s);
+    FH_HIP(launch_fill(b->P.counts + s.bucket_base, span, 0u, b->stream));
+    Geometry g = b->geo;
+    g.tile_begin = s.bird_base / TILE;
+    g.tile_count = (s.n + TILE - 1) / TILE;
+    FH_HIP(launch_hash_count(g, b->P, (b->prm.flags & FH_FLAG_WAVE_AGG) != 0, b->stream));
+    fh_status st = push_shifts(b, f, (int32_t)s.n, 0);
+    if (st) return st;
+    FH_HIP(launch_resident_splice(b->geo, b->P, b->R[q], b->R[1u - q], b->d_shift, b->geo.n_slots - s.n,
+                                  b->geo.n_slots, b->stream));
+    b->parity = 1u - q;
+    if ((st = resident_rescan(b))) return st;
+    Pools p = b->P;
+    p.offsets = b->off[b->parity];
+    FH_HIP(launch_resident_tile_sums(b->geo, p, b->W, b->stream));
+    FH_HIP(launch_resident_enter_range(b->geo, p, b->R[b->parity], s.bird_base, s.n, b->stream));
     return FH_OK;
 }
 
@@ -641,7 +721,6 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
     if (st) return st;
     if (!flock_id || (n && !words) || n == 0)
         return set_error(FH_ERR_INVALID, "fh_batch_add: need n > 0, words and flock_id");
-    if ((st = batch_leave_resident(b))) return st;
     const uint32_t buckets = flock_buckets(n, b->prm);
     const uint32_t ntile = (n + TILE - 1) / TILE;
     const uint32_t nunit = (buckets + SCAN_TILE - 1) / SCAN_TILE;
@@ -703,7 +782,9 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
     // Count the new flock's birds into the histogram if the others are already
     // counted; otherwise the next step hashes everything.
     refresh_geometry(b);
-    if (b->hashed) {
+    if (b->resident) {  // joins the resident layout in place
+        if ((st = resident_join_one(b, f))) return st;
+    } else if (b->hashed) {
         Geometry g = b->geo;
         g.tile_begin = t0;
         g.tile_count = ntile;
@@ -719,7 +800,8 @@ fh_status fh_batch_remove(fh_batch *b, uint32_t flock_id) {
     if (st) return st;
     if (flock_id >= b->segs.size() || !b->segs[flock_id].live)
         return set_error(FH_ERR_INVALID, "fh_batch_remove: no live flock %u", flock_id);
-    if ((st = batch_leave_resident(b))) return st;
+    const bool splice = b->resident;  // leaves the resident layout in place
+    if (splice && (st = resident_leave_one(b, flock_id))) return st;
     Seg &s = b->segs[flock_id];
     if (b->counts_dirty)  // drop this flock's birds from the pending histogram
         FH_HIP(launch_fill(b->P.counts + s.bucket_base, s.mask + 1, 0u, b->stream));
@@ -730,6 +812,7 @@ fh_status fh_batch_remove(fh_batch *b, uint32_t flock_id) {
     s.live = 0;
     if ((st = push_tables(b, t0, t0 + ntile))) return st;
     refresh_geometry(b);
+    if (splice && (st = resident_rescan(b))) return st;
     return FH_OK;
 }
 
@@ -742,7 +825,9 @@ fh_status fh_batch_download(fh_batch *b, uint32_t flock_id, float *words, float 
     const Seg &s = b->segs[flock_id];
     const uint32_t n = s.n;
     if (!words) return set_error(FH_ERR_INVALID, "fh_batch_download: NULL words");
-    if ((st = batch_leave_resident(b))) return st;
+    if (b->resident)  // this flock's slots back to bird order; the layout stays
+        FH_HIP(launch_resident_flush_range(b->geo, b->P, b->R[b->parity], resident_slot_starts(b)[flock_id], n,
+                                           b->stream));
     if ((st = pinned(b, staged_bytes(n)))) return st;
     const Staged h = staged(b->pin, n);
     const size_t o = s.bird_base;

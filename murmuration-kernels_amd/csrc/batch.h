@@ -88,6 +88,7 @@ struct fh_batch {
     fh::ResWork W{};
     uint32_t *off[2]{};
     uint32_t *res_due = nullptr;  // due counter per parity, then the persistent decide's chunk counters
+    int32_t *d_shift = nullptr;   // per flock: slot shift of a splice (flocks joining or leaving)
     bool resident = false;
     uint32_t parity = 0;
     hipGraphExec_t rgraph[2]{};

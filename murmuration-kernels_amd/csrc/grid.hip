@@ -414,9 +414,10 @@ hipError_t launch_scatter(const Geometry &g, const Pools &p, bool canonical, boo
 
 // Records from the bird-order state: slot = offsets[key] + rank, the same slot
 // the scatter gives the bird.
-__global__ __launch_bounds__(TILE) void k_r_enter(Geometry g, Pools p, Records rec) {
-    const uint32_t b = blockIdx.x * TILE + threadIdx.x;
-    if (b >= g.n_birds) return;
+__global__ __launch_bounds__(TILE) void k_r_enter(Geometry g, Pools p, Records rec, uint32_t bird_lo,
+                                                  uint32_t bird_hi) {
+    const uint32_t b = bird_lo + blockIdx.x * TILE + threadIdx.x;
+    if (b >= bird_hi || b >= g.n_birds) return;
     if (g.tile_seg) {  // a batch: padding birds and dead flocks hold no slot
         const uint32_t si = g.tile_seg[b / TILE];
         if (si == NO_SEG || !g.segs[si].live || b - g.segs[si].bird_base >= g.segs[si].n) return;
@@ -436,9 +437,10 @@ __global__ __launch_bounds__(TILE) void k_r_enter(Geometry g, Pools p, Records r
 }
 
 // Records back to the bird-order state.
-__global__ __launch_bounds__(TILE) void k_r_flush(Geometry g, Pools p, Records rec) {
-    const uint32_t s = blockIdx.x * TILE + threadIdx.x;
-    if (s >= g.n_slots) return;
+__global__ __launch_bounds__(TILE) void k_r_flush(Geometry g, Pools p, Records rec, uint32_t slot_lo,
+                                                  uint32_t slot_hi) {
+    const uint32_t s = slot_lo + blockIdx.x * TILE + threadIdx.x;
+    if (s >= slot_hi || s >= g.n_slots) return;
     const uint32_t b = rec.id[s];
     if (b >= g.n_birds) {
         atomicOr(p.err, ERR_SLOT);
@@ -449,6 +451,34 @@ __global__ __launch_bounds__(TILE) void k_r_flush(Geometry g, Pools p, Records r
     p.steer4[b] = rec.s4[s];
     p.spin[b] = rec.spin[s];
     p.key[b] = rec.key[s];
+}
+
+// Flocks joining or leaving a resident batch: each record keeps its order and
+// moves by its flock's shift; a leaving flock's records are dropped.
+__global__ __launch_bounds__(TILE) void k_r_splice(Geometry g, Pools p, Records cur, Records nxt,
+                                                   const int32_t *__restrict__ shift, uint32_t n_cur,
+                                                   uint32_t n_nxt) {
+    const uint32_t s = blockIdx.x * TILE + threadIdx.x;
+    if (s >= n_cur) return;
+    const uint32_t b = cur.id[s];
+    const uint32_t si = b >= g.n_birds ? NO_SEG : g.tile_seg ? g.tile_seg[b / TILE] : 0u;
+    if (si == NO_SEG) {
+        atomicOr(p.err, ERR_SLOT);
+        return;
+    }
+    const int32_t d = shift[si];
+    if (d == SPLICE_DROP) return;
+    const uint32_t ns = (uint32_t)((int64_t)s + d);
+    if (ns >= n_nxt) {
+        atomicOr(p.err, ERR_SLOT);
+        return;
+    }
+    nxt.p4[ns] = cur.p4[s];
+    nxt.v4[ns] = cur.v4[s];
+    nxt.s4[ns] = cur.s4[s];
+    nxt.spin[ns] = cur.spin[s];
+    nxt.id[ns] = b;
+    nxt.key[ns] = cur.key[s];
 }
 
 // Each resident scan tile's count total (RES_TILE buckets per workgroup).
@@ -555,7 +585,23 @@ __global__ __launch_bounds__(TILE) void k_r_move(Geometry g, Pools p, Records cu
 
 hipError_t launch_resident_enter(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s) {
     if (g.n_birds == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_r_enter, dim3((g.n_birds + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, rec);
+    hipLaunchKernelGGL(k_r_enter, dim3((g.n_birds + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, rec, 0u, g.n_birds);
+    return hipGetLastError();
+}
+
+hipError_t launch_resident_enter_range(const Geometry &g, const Pools &p, const Records &rec, uint32_t bird_lo,
+                                       uint32_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_r_enter, dim3((count + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, rec, bird_lo,
+                       bird_lo + count);
+    return hipGetLastError();
+}
+
+hipError_t launch_resident_splice(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
+                                  const int32_t *shift, uint32_t n_cur, uint32_t n_nxt, hipStream_t s) {
+    if (n_cur == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_r_splice, dim3((n_cur + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, cur, nxt, shift, n_cur,
+                       n_nxt);
     return hipGetLastError();
 }
 
@@ -567,7 +613,15 @@ hipError_t launch_resident_tile_sums(const Geometry &g, const Pools &p, const Re
 
 hipError_t launch_resident_flush(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s) {
     if (g.n_slots == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_r_flush, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, rec);
+    hipLaunchKernelGGL(k_r_flush, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, rec, 0u, g.n_slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_resident_flush_range(const Geometry &g, const Pools &p, const Records &rec, uint32_t slot_lo,
+                                       uint32_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_r_flush, dim3((count + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, rec, slot_lo,
+                       slot_lo + count);
     return hipGetLastError();
 }
 

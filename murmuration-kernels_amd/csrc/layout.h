@@ -219,6 +219,16 @@ hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Recor
 // pass. In a batch (g.tile_seg set) every slot's flock is found through its
 // bird id; the enter pass skips the padding birds of each flock's range.
 hipError_t launch_resident_enter(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s);
+// The enter pass for the birds [bird_lo, bird_lo + count) only (a flock that
+// joins a resident batch).
+hipError_t launch_resident_enter_range(const Geometry &g, const Pools &p, const Records &rec, uint32_t bird_lo,
+                                       uint32_t count, hipStream_t s);
+// A batch's resident records when flocks join or leave: slot s of `cur`
+// (s < n_cur) moves to s + shift[its flock] in `nxt`, and the records of a
+// flock whose shift is SPLICE_DROP are left out; n_nxt bounds the new slots.
+constexpr int32_t SPLICE_DROP = INT32_MIN;
+hipError_t launch_resident_splice(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
+                                  const int32_t *shift, uint32_t n_cur, uint32_t n_nxt, hipStream_t s);
 // Each resident scan tile's count total (after the entry scan, counts kept).
 hipError_t launch_resident_tile_sums(const Geometry &g, const Pools &p, const ResWork &w, hipStream_t s);
 hipError_t launch_resident_integrate(const Geometry &g, const Pools &p, const Records &rec, const ResWork &w,
@@ -227,6 +237,9 @@ hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Reco
                                    const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, Seg *tick,
                                    uint32_t ntick, hipStream_t s);
 hipError_t launch_resident_flush(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s);
+// The flush of the slots [slot_lo, slot_lo + count) only (one flock of a batch).
+hipError_t launch_resident_flush_range(const Geometry &g, const Pools &p, const Records &rec, uint32_t slot_lo,
+                                       uint32_t count, hipStream_t s);
 hipError_t launch_integrate(const Geometry &g, const Pools &p, bool hash, bool wave_agg,
                             hipStream_t s);
 hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, int walk,

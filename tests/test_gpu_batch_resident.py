@@ -63,8 +63,8 @@ def test_resident_batch_matches_the_oracle(fh, fr):
 
 
 def test_resident_batch_join_and_leave(fh):
-    # Adding, removing and downloading leave the resident layout; the next
-    # step enters it again with the flocks then live.
+    # Adding and removing splice the flock into or out of the resident layout
+    # in place; a download flushes that flock's slots only.
     a, c, d = cube(fh, 5000, 1), cube(fh, 9000, 2), cube(fh, 700, 3)
     out = {}
     for name, flags in (("resident", RES | GRAPH), ("canonical", CANON)):
@@ -79,6 +79,36 @@ def test_resident_batch_join_and_leave(fh):
             idd = b.add(d, 0.0)
             b.step(4)
             out[name] = (b.download(ic), b.download(idd))
+    for (g, tg), (w, tw) in zip(out["resident"], out["canonical"]):
+        assert tg == tw
+        close(g, w)
+
+
+@pytest.mark.parametrize("flags", [RES | GRAPH, RES | GRAPH | 2048 | 128, RES | 256 | 512])
+def test_resident_batch_splices_follow_the_canonical_batch(fh, flags):
+    # Flocks join and leave a resident batch in place: at the end of the
+    # bucket space, in the ranges of flocks that left, the first and a middle
+    # flock leaving, two changes between the same steps, at both buffer
+    # parities. Every flock still live must follow the canonical batch.
+    ws = [cube(fh, n, 20 + k) for k, n in enumerate((3000, 40_000, 700, 12_000, 2500, 5000, 60_000))]
+    out = {}
+    for name, fl in (("resident", flags), ("canonical", CANON)):
+        with fh.Batch(300_000, 1 << 22, 8, flags=fl) as b:
+            ids = [b.add(ws[0]), b.add(ws[1]), b.add(ws[2])]
+            b.step(3)
+            ids.append(b.add(ws[3], 0.0))  # joins after every flock
+            b.step(2)
+            b.remove(ids[0])  # the first flock leaves
+            b.step(1)
+            ids.append(b.add(ws[4], 0.0))  # fits the first flock's ranges
+            b.remove(ids[2])  # a middle flock leaves
+            ids.append(b.add(ws[5], 0.0))
+            b.step(5)
+            mid = b.download(ids[3])  # a download leaves the layout as it is
+            b.step(2)
+            ids.append(b.add(ws[6], 0.0))
+            b.step(3)
+            out[name] = [mid] + [b.download(i) for i in (ids[1], ids[3], ids[4], ids[5], ids[6])]
     for (g, tg), (w, tw) in zip(out["resident"], out["canonical"]):
         assert tg == tw
         close(g, w)
