@@ -413,9 +413,12 @@ constexpr uint32_t NO_BIRD = 0xffffffffu;
 // launch is the last chunk's walk, not the slowest workgroup's share.
 // Waves per SIMD the compiler keeps room for (1: no floor). With G >= 4 the
 // walk fits 95 VGPRs without spilling, so 5 waves per SIMD instead of the 4
-// that its natural 105-115 VGPRs allow (measured: -2% to -9% decide time);
-// G = 1-2 would spill.
-constexpr int decide_min_waves(int g) { return g >= 4 ? 5 : 1; }
+// that its natural 105-115 VGPRs allow (measured: -2% to -9% decide time).
+// G = 2 is held to 128 VGPRs (4 waves): it fits without the fused flight, and
+// with it (136 VGPRs, 3 waves, left alone) one spilled register outside the
+// walk buys the fourth wave (fused step at 1M birds: 0.127 -> 0.099 ms).
+// G = 1 would spill inside the walk.
+constexpr int decide_min_waves(int g) { return g >= 4 ? 5 : g == 2 ? 4 : 1; }
 
 template <int BLOCK, int G, bool RES, bool FLY>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_min_waves(G), 8))) void k_decide(

@@ -91,15 +91,17 @@ def test_resident_c1_20_steps(fh, fr):
     assert drift <= 1e-3
 
 
-def test_resident_parity_ladder_2000(fh, fr):
+@pytest.mark.parametrize("group,extra", [(2, 32), (4, 0)])
+def test_resident_parity_ladder_2000(fh, fr, group, extra):
     # The ladder of scripts/parity.sh:23-34 (docs/SPEC.md:66-72) on the path of
     # the tuned 1M-bird step: resident layout, fused flight, spatial hash,
-    # lane groups of 4. Same bars as the canonical ladder: pointwise 1e-3 up to
-    # 30 steps, at most 2 outliers under 1e-2 at 100, aggregates at 1000.
+    # lane groups of 2 with the occupancy bits (and of 4). Same bars as the
+    # canonical ladder: pointwise 1e-3 up to 30 steps, at most 2 outliers
+    # under 1e-2 at 100, aggregates at 1000.
     w0 = fh.gen_reference(2000, 1)
-    flags = fh.FLAG_RESIDENT | fh.FLAG_FLIGHT | fh.FLAG_SPATIAL | fh.FLAG_GRAPH
+    flags = fh.FLAG_RESIDENT | fh.FLAG_FLIGHT | fh.FLAG_SPATIAL | fh.FLAG_GRAPH | extra
     for k in (1, 10, 30, 100, 1000):
-        with fh.Flock.from_words(w0, flags=flags, decide_group=4) as f:
+        with fh.Flock.from_words(w0, flags=flags, decide_group=group) as f:
             f.step(k)
             gpu, t = f.download()
         cpu, tc = fr.step(w0, 0.0, k, threads=THREADS)
