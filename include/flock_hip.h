@@ -91,8 +91,10 @@ typedef struct fh_params {
  * pass moves every record to its new slot (instead of hashing every bird and
  * scattering sorted copies each step). Keys, counts and offsets stay the
  * reference's; within a bucket, birds that stay keep their order and birds
- * that arrive follow them. Single flocks only; not with FH_FLAG_CANONICAL.
- * Any other call on the flock first puts the state back in bird order. */
+ * that arrive follow them. Single flocks (fh_step) and batches
+ * (fh_batch_step); not with FH_FLAG_CANONICAL. Any other call on a single
+ * flock first puts its state back in bird order; in a batch, flocks join and
+ * leave the layout in place and a download flushes that flock's slots only. */
 #define FH_FLAG_RESIDENT 64u
 /* Bucket cells by a locality-preserving hash (the Morton order of the cell
  * coordinates within 1024-cell blocks) instead of the reference's hash: nearby
@@ -232,9 +234,13 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
                           const fh_params *params, fh_batch **out);
 fh_status fh_batch_destroy(fh_batch *b);
 /* Adds a flock (n > 0 .flk records at clock `time`); FH_ERR_BUDGET when the
- * pools or slots are full. */
+ * pools or slots are full. In a resident batch that has stepped, the flock
+ * joins the layout in place: the records of the flocks after it move by n
+ * slots. */
 fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, uint32_t *flock_id);
+/* Removes a flock; in a resident batch its records leave the layout in place. */
 fh_status fh_batch_remove(fh_batch *b, uint32_t flock_id);
+/* The flock's .flk records (in its bird order) and clock. */
 fh_status fh_batch_download(fh_batch *b, uint32_t flock_id, float *words, float *time);
 /* `steps` steps of every live flock (time += DT each), except the flocks of a
  * resident batch whose step budget is used up. */
