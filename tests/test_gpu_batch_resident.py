@@ -114,6 +114,49 @@ def test_resident_batch_splices_follow_the_canonical_batch(fh, flags):
         close(g, w)
 
 
+def test_resident_batch_random_joins_and_leaves(fh):
+    # A seeded sequence of adds, removes, downloads and steps over up to 8
+    # flocks of mixed sizes; the resident batch (splicing flocks in and out)
+    # must keep every live flock equal to the canonical batch.
+    rng = np.random.default_rng(5)
+    sizes = [int(x) for x in rng.integers(300, 30_000, 14)]
+    ws = [cube(fh, n, 40 + k) for k, n in enumerate(sizes)]
+    ops = []
+    live, nxt = [], 0
+    for _ in range(24):
+        r = rng.random()
+        if (r < 0.4 or not live) and nxt < len(ws) and len(live) < 8:
+            ops.append(("add", nxt))
+            live.append(nxt)
+            nxt += 1
+        elif r < 0.6 and len(live) > 1:
+            k = live.pop(int(rng.integers(len(live))))
+            ops.append(("remove", k))
+        elif r < 0.7 and live:
+            ops.append(("download", live[int(rng.integers(len(live)))]))
+        else:
+            ops.append(("step", int(rng.integers(1, 4))))
+    ops.append(("step", 3))
+    out = {}
+    for name, fl in (("resident", RES | 128 | 2048), ("canonical", CANON)):
+        with fh.Batch(300_000, 1 << 22, 8, flags=fl) as b:
+            ids, mids = {}, []
+            for op, arg in ops:
+                if op == "add":
+                    ids[arg] = b.add(ws[arg], 0.0)
+                elif op == "remove":
+                    b.remove(ids.pop(arg))
+                elif op == "download":
+                    mids.append(b.download(ids[arg]))
+                else:
+                    b.step(arg)
+            out[name] = mids + [b.download(ids[k]) for k in sorted(ids)]
+    assert len(out["resident"]) == len(out["canonical"]) >= 3
+    for (g, tg), (w, tw) in zip(out["resident"], out["canonical"]):
+        assert tg == tw
+        close(g, w)
+
+
 def test_resident_batch_clock_per_flock(fh):
     w = cube(fh, 2000, 4)
     with fh.Batch(10_000, 1 << 18, 4, flags=RES) as b:
