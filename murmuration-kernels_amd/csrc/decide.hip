@@ -632,8 +632,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
         // of this step from rec. A separate instance: the flight's code would
         // otherwise cost the G = 1-2 walks a wave per SIMD.
         __syncthreads();
-        for (uint32_t b0 = start; b0 < end; b0 += BLOCK)
-            resident_flight(g, rec, flown, w, b0 + threadIdx.x, b0 + threadIdx.x < end);
+        for (uint32_t b0 = start; b0 < end; b0 += 2u * BLOCK)
+            resident_flight2(g, rec, flown, w, b0 + threadIdx.x, b0 + BLOCK + threadIdx.x, end, start);
     }
     __syncthreads();  // due_slots and n_due are reused by the next chunk
     }

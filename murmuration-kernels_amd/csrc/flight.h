@@ -47,47 +47,77 @@ __device__ __forceinline__ void integrate_bird(float4 &p4, float4 &v4, const flo
     rate_out = rate;
 }
 
-// Resident flight of slot s: banked flight of in[s] into out[s] (out may be
-// in), its new bucket (by its flock's hash), and the mover accounting. A bird whose new cell falls
-// in another bucket is a mover: it moves its count from its old bucket to its
-// new one, takes an arrival rank there, and moves its count between the scan
-// tiles' totals when the tile changes. Every lane of a wave calls this
-// (active or not) with consecutive slots starting at a multiple of 64: the
-// wave's mover bits are written with one ballot. Words past the flock's last
-// slot are not written (nothing reads them).
+// Resident flight of slot s from its loaded records (position and decision
+// time p4, velocity and roll v4, steering and radius s4, old bucket ko):
+// banked flight into out[s] (out may be in), its new bucket by its flock's
+// hash, and the mover accounting. A bird whose new cell falls in another
+// bucket is a mover: it moves its count from its old bucket to its new one,
+// takes an arrival rank there, and moves its count between the scan tiles'
+// totals when the tile changes. Returns whether the bird is a mover.
+__device__ __forceinline__ bool fly_loaded(const Seg &sg, const Records &in, const Records &out, const ResWork &w,
+                                           uint32_t s, float4 p4, float4 v4, const float4 s4, uint32_t ko) {
+    float rate;
+    if (sg.left)
+        integrate_bird(p4, v4, s4, rate);
+    else
+        rate = in.spin[s];  // a frozen flock keeps its state
+    out.p4[s] = p4;
+    out.v4[s] = v4;
+    out.spin[s] = rate;
+    const uint32_t kn = bucket_of_cell(sg, cell_coord(p4.x, sg.inv_cell), cell_coord(p4.y, sg.inv_cell),
+                                       cell_coord(p4.z, sg.inv_cell));
+    if (kn == ko) return false;
+    w.arank[s] = atomicAdd(&w.arr[kn], 1u);
+    atomicAdd(&w.counts[kn], 1u);
+    atomicSub(&w.counts[ko], 1u);
+    if (kn / RES_TILE != ko / RES_TILE) {
+        atomicAdd(&w.tsum[kn / RES_TILE], 1u);
+        atomicSub(&w.tsum[ko / RES_TILE], 1u);
+    }
+    return true;
+}
+
+// The mover bits of a wave whose lanes hold 64 consecutive slots from a
+// multiple of 64 (every lane calls this): one ballot, written by lanes 0 and
+// 32. Words past the flock's last slot are not written (nothing reads them).
+__device__ __forceinline__ void write_mover_bits(const Geometry &g, const ResWork &w, uint32_t s, bool mover) {
+    const uint64_t bits = __ballot(mover);
+    const uint32_t lane = threadIdx.x & 63u;
+    if ((lane == 0u || lane == 32u) && s < g.n_slots)
+        w.mbits[s >> 5] = lane == 0u ? (uint32_t)bits : (uint32_t)(bits >> 32);
+}
+
+// Resident flight of slot s (every lane of a wave calls this, active or not,
+// with consecutive slots from a multiple of 64).
 __device__ __forceinline__ void resident_flight(const Geometry &g, const Records &in, const Records &out,
                                                 const ResWork &w, uint32_t s, bool active) {
     bool mover = false;
     if (active) {
         const Seg &sg = g.segs[slot_segment(g, in, s)];
-        float4 p4 = in.p4[s];
-        float4 v4 = in.v4[s];
-        float rate;
-        if (sg.left)
-            integrate_bird(p4, v4, in.s4[s], rate);
-        else
-            rate = in.spin[s];  // a frozen flock keeps its state
-        out.p4[s] = p4;
-        out.v4[s] = v4;
-        out.spin[s] = rate;
-        const uint32_t kn = bucket_of_cell(sg, cell_coord(p4.x, sg.inv_cell), cell_coord(p4.y, sg.inv_cell),
-                                           cell_coord(p4.z, sg.inv_cell));
-        const uint32_t ko = in.key[s];
-        if (kn != ko) {
-            mover = true;
-            w.arank[s] = atomicAdd(&w.arr[kn], 1u);
-            atomicAdd(&w.counts[kn], 1u);
-            atomicSub(&w.counts[ko], 1u);
-            if (kn / RES_TILE != ko / RES_TILE) {
-                atomicAdd(&w.tsum[kn / RES_TILE], 1u);
-                atomicSub(&w.tsum[ko / RES_TILE], 1u);
-            }
-        }
+        mover = fly_loaded(sg, in, out, w, s, in.p4[s], in.v4[s], in.s4[s], in.key[s]);
     }
-    const uint64_t bits = __ballot(mover);
-    const uint32_t lane = threadIdx.x & 63u;
-    if ((lane == 0u || lane == 32u) && s < g.n_slots)
-        w.mbits[s >> 5] = lane == 0u ? (uint32_t)bits : (uint32_t)(bits >> 32);
+    write_mover_bits(g, w, s, mover);
+}
+
+// Two slots per lane, s0 and s1 (each wave's lanes on 64 consecutive slots
+// from a multiple of 64 for both): the records of both are loaded before
+// either flies, so one memory latency covers two birds. Slots at or past
+// `end` do not fly (`valid` is any slot below it, loaded in their place),
+// and a wave's bits are written only for a run that starts below `end`.
+__device__ __forceinline__ void resident_flight2(const Geometry &g, const Records &in, const Records &out,
+                                                 const ResWork &w, uint32_t s0, uint32_t s1, uint32_t end,
+                                                 uint32_t valid) {
+    const bool a0 = s0 < end, a1 = s1 < end;
+    const uint32_t l0 = a0 ? s0 : valid, l1 = a1 ? s1 : valid;
+    const Seg &sg0 = g.segs[slot_segment(g, in, l0)];
+    const Seg &sg1 = g.segs[slot_segment(g, in, l1)];
+    const float4 p0 = in.p4[l0], v0 = in.v4[l0], q0 = in.s4[l0];
+    const float4 p1 = in.p4[l1], v1 = in.v4[l1], q1 = in.s4[l1];
+    const uint32_t k0 = in.key[l0], k1 = in.key[l1];
+    const bool m0 = a0 && fly_loaded(sg0, in, out, w, s0, p0, v0, q0, k0);
+    if ((s0 & ~63u) < end) write_mover_bits(g, w, s0, m0);  // uniform over the wave
+    const bool m1 = a1 && fly_loaded(sg1, in, out, w, s1, p1, v1, q1, k1);
+    if ((s1 & ~63u) < end) write_mover_bits(g, w, s1, m1);
 }
 
 }  // namespace fh
