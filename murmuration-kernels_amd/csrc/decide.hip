@@ -249,7 +249,7 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
                 s0[k] = r.x;
                 s1[k] = min(r.y, n_slots);
             }
-        constexpr int UNROLL = 4;
+        constexpr int UNROLL = 2;
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             for (uint32_t t0 = s0[k] + lane; t0 < s1[k]; t0 += UNROLL * G) {
@@ -417,8 +417,10 @@ constexpr uint32_t NO_BIRD = 0xffffffffu;
 // G = 2 is held to 128 VGPRs (4 waves): it fits without the fused flight, and
 // with it (136 VGPRs, 3 waves, left alone) one spilled register outside the
 // walk buys the fourth wave (fused step at 1M birds: 0.127 -> 0.099 ms).
-// G = 1 would spill inside the walk.
-constexpr int decide_min_waves(int g) { return g >= 4 ? 5 : g == 2 ? 4 : 1; }
+// The split walks (G >= 16) load 2 candidates ahead instead of 4 and run 6
+// waves in 80 VGPRs (dense clumps at 10M birds: -14% per step; reference
+// density at 1M: -3% decide). G = 1 would spill inside the walk.
+constexpr int decide_min_waves(int g) { return g >= 16 ? 6 : g >= 4 ? 5 : g == 2 ? 4 : 1; }
 
 template <int BLOCK, int G, bool RES, bool FLY>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_min_waves(G), 8))) void k_decide(

@@ -40,7 +40,7 @@ SWITCHES = {"occ": FLAG_OCC, "resident": FLAG_RESIDENT, "spatial": FLAG_SPATIAL,
 # Part of every cache key: raised whenever the searched space or the kernels'
 # costs change, so an entry found for other kernels or a smaller space is not
 # reused.
-SEARCH_VERSION = 10
+SEARCH_VERSION = 11
 FINALISTS = 6
 CONFIRM = 3
 
