@@ -274,6 +274,7 @@ static fh_status batch_alloc_resident(fh_batch *b) {
     const size_t o_arr = a.take<uint32_t>(nb), o_abase = a.take<uint32_t>(nb);
     const size_t o_off1 = a.take<uint32_t>(nb + 1), o_due = a.take<uint32_t>(4);
     const size_t o_tsum = a.take<uint32_t>(nb / RES_TILE);
+    const size_t o_tsum2 = a.take<uint32_t>(nb / RES_TILE / RES_SUPER + 1);
     const size_t o_fp4 = flight ? a.take<float4>(nn) : 0, o_fv4 = flight ? a.take<float4>(nn) : 0;
     const size_t o_fspin = flight ? a.take<float>(nn) : 0;
     const size_t o_shift = a.take<int32_t>(b->flock_cap ? b->flock_cap : 1);
@@ -297,6 +298,7 @@ static fh_status batch_alloc_resident(fh_batch *b) {
     b->W.counts = b->P.counts;
     b->W.abase = reinterpret_cast<uint32_t *>(base + o_abase);
     b->W.tsum = reinterpret_cast<uint32_t *>(base + o_tsum);
+    b->W.tsum2 = reinterpret_cast<uint32_t *>(base + o_tsum2);
     if (flight) {
         b->F.p4 = reinterpret_cast<float4 *>(base + o_fp4);
         b->F.v4 = reinterpret_cast<float4 *>(base + o_fv4);
@@ -435,7 +437,7 @@ static fh_status batch_resident_launches(fh_batch *b, hipEvent_t *ev) {
                                   b->res_due + 2 + (1u - q), b->W, b->F, b->stream,
                                   ev ? DecideTiming{ev[6], ev[7]} : DecideTiming{}));
     if (ev) FH_HIP(hipEventRecord(ev[1], b->stream));
-    if (!fused) FH_HIP(launch_resident_integrate(b->geo, p, b->R[q], b->W, b->d_segs, nseg, b->stream));
+    if (!fused) FH_HIP(launch_resident_integrate(b->geo, p, b->R[q], b->W, b->stream));
     if (ev) FH_HIP(hipEventRecord(ev[2], b->stream));
     Records src = b->R[q];
     if (fused) {
@@ -443,8 +445,8 @@ static fh_status batch_resident_launches(fh_batch *b, hipEvent_t *ev) {
         src.v4 = b->F.v4;
         src.spin = b->F.spin;
     }
-    FH_HIP(launch_resident_rebuild(b->geo, p, src, b->R[1u - q], b->W, b->off[q], b->off[1u - q],
-                                   fused ? b->d_segs : nullptr, nseg, b->stream));
+    FH_HIP(launch_resident_rebuild(b->geo, p, src, b->R[1u - q], b->W, b->off[q], b->off[1u - q], b->d_segs, nseg,
+                                   b->stream));
     if (ev)
         for (int k = 3; k < 6; ++k) FH_HIP(hipEventRecord(ev[k], b->stream));
     return FH_OK;
@@ -754,11 +756,14 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
     s.time = time;
     s.live = 1;
     s.mode = (b->prm.flags & FH_FLAG_SPATIAL) ? SEG_SPATIAL : 0u;
-    b->segs[f] = s;
-    b->times[f] = time;
-    for (uint32_t t = 0; t < ntile; ++t) b->tile_seg[t0 + t] = f;
-
-    if ((st = pinned(b, staged_bytes(n)))) return st;
+    // Until the tables are pushed, no launch sees the flock: its tiles map to
+    // no segment on the device, so a failed upload leaves the batch as it was.
+    auto unwind = [&](fh_status e) {
+        b->tiles.release(t0);
+        b->bucket_units.release(u0);
+        return e;
+    };
+    if ((st = pinned(b, staged_bytes(n)))) return unwind(st);
     const Staged h = staged(b->pin, n);
     convert(b, n, [&](size_t lo, size_t hi) {
         for (size_t i = lo; i < hi; ++i) {
@@ -772,13 +777,27 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
         }
     });
     const size_t o = s.bird_base;
-    FH_HIP(hipMemcpyAsync(b->P.pos4 + o, h.pos, n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
-    FH_HIP(hipMemcpyAsync(b->P.vel4 + o, h.vel, n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
-    FH_HIP(hipMemcpyAsync(b->P.steer4 + o, h.steer, n * sizeof(float4), hipMemcpyHostToDevice, b->stream));
-    FH_HIP(hipMemcpyAsync(b->P.spin + o, h.spin, n * 4, hipMemcpyHostToDevice, b->stream));
-    FH_HIP(hipMemcpyAsync(b->P.cen + o, h.cen, n * 4, hipMemcpyHostToDevice, b->stream));
-    FH_HIP(hipMemcpyAsync(b->P.nb + o, h.nb, n * 4, hipMemcpyHostToDevice, b->stream));
-    if ((st = push_tables(b, t0, t0 + ntile))) return st;  // synchronises: the staging is free again
+    hipError_t e = hipSuccess;
+    const struct {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    } up[6] = {{b->P.pos4 + o, h.pos, n * sizeof(float4)}, {b->P.vel4 + o, h.vel, n * sizeof(float4)},
+               {b->P.steer4 + o, h.steer, n * sizeof(float4)}, {b->P.spin + o, h.spin, n * 4u},
+               {b->P.cen + o, h.cen, n * 4u}, {b->P.nb + o, h.nb, n * 4u}};
+    for (const auto &u : up)
+        if (e == hipSuccess) e = hipMemcpyAsync(u.dst, u.src, u.bytes, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);  // the staging is free again
+    if (e != hipSuccess) return unwind(hip_fail(e, "fh_batch_add: upload"));
+    b->segs[f] = s;
+    b->times[f] = time;
+    for (uint32_t t = 0; t < ntile; ++t) b->tile_seg[t0 + t] = f;
+    if ((st = push_tables(b, t0, t0 + ntile))) {
+        b->segs[f].live = 0;
+        for (uint32_t t = 0; t < ntile; ++t) b->tile_seg[t0 + t] = NO_SEG;
+        (void)push_tables(b, t0, t0 + ntile);
+        return unwind(st);
+    }
     // Count the new flock's birds into the histogram if the others are already
     // counted; otherwise the next step hashes everything.
     refresh_geometry(b);

@@ -191,6 +191,7 @@ static fh_status alloc_resident(fh_flock *h) {
     const size_t o_fspin = flight ? a.take<float>(nn) : 0;
     const size_t o_off1 = a.take<uint32_t>(nb + 1), o_due = a.take<uint32_t>(4);
     const size_t o_tsum = a.take<uint32_t>(nb / RES_TILE);
+    const size_t o_tsum2 = a.take<uint32_t>(nb / RES_TILE / RES_SUPER + 1);
     hipError_t e = hipMalloc(&h->res_arena, a.used);
     if (e != hipSuccess) {
         h->res_arena = nullptr;
@@ -211,6 +212,7 @@ static fh_status alloc_resident(fh_flock *h) {
     h->W.counts = h->P.counts;
     h->W.abase = reinterpret_cast<uint32_t *>(base + o_abase);
     h->W.tsum = reinterpret_cast<uint32_t *>(base + o_tsum);
+    h->W.tsum2 = reinterpret_cast<uint32_t *>(base + o_tsum2);
     if (flight) {
         h->F.p4 = reinterpret_cast<float4 *>(base + o_fp4);
         h->F.v4 = reinterpret_cast<float4 *>(base + o_fv4);
@@ -273,7 +275,7 @@ static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
                                   h->res_due + 2 + (1u - q), h->W, h->F, h->stream,
                                   ev ? DecideTiming{ev[6], ev[7]} : DecideTiming{}));
     if (ev) FH_HIP(hipEventRecord(ev[1], h->stream));
-    if (!fused) FH_HIP(launch_resident_integrate(h->geo, p, h->R[q], h->W, h->d_seg, 1u, h->stream));
+    if (!fused) FH_HIP(launch_resident_integrate(h->geo, p, h->R[q], h->W, h->stream));
     if (ev) FH_HIP(hipEventRecord(ev[2], h->stream));
     // the move reads the flown positions, velocities and roll rates from F
     // when the flight was fused, everything else from the current records
@@ -283,8 +285,8 @@ static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
         src.v4 = h->F.v4;
         src.spin = h->F.spin;
     }
-    FH_HIP(launch_resident_rebuild(h->geo, p, src, h->R[1u - q], h->W, h->off[q], h->off[1u - q],
-                                   fused ? h->d_seg : nullptr, 1u, h->stream));
+    FH_HIP(launch_resident_rebuild(h->geo, p, src, h->R[1u - q], h->W, h->off[q], h->off[1u - q], h->d_seg, 1u,
+                                   h->stream));
     if (ev)
         for (int k = 3; k < 6; ++k) FH_HIP(hipEventRecord(ev[k], h->stream));
     return FH_OK;

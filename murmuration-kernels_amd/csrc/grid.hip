@@ -489,19 +489,38 @@ __global__ __launch_bounds__(256) void k_r_tile_sums(const uint32_t *__restrict_
     if (threadIdx.x == 0) tsum[blockIdx.x] = total;
 }
 
-// Rebuild scan (one launch; RES_TILE buckets per workgroup, 4 per lane): the
-// new offsets from the counts the flight kept current, and from the tiles'
-// totals before this tile; the first arrival slot of each bucket (its stayers
-// come first); the occupancy bits. Arrivals are zeroed for the next step.
-// With `tick`, workgroup 0 commits the step's clocks.
+// Totals of RES_SUPER consecutive resident tiles (one workgroup each).
+__global__ __launch_bounds__(256) void k_r_super_sums(const uint32_t *__restrict__ tsum, uint32_t ntiles,
+                                                      uint32_t *__restrict__ tsum2) {
+    static_assert(RES_SUPER == 256, "one tile total per lane");
+    const uint32_t t = blockIdx.x * RES_SUPER + threadIdx.x;
+    uint32_t total;
+    block_excl_scan<4>(t < ntiles ? tsum[t] : 0u, &total);
+    if (threadIdx.x == 0) tsum2[blockIdx.x] = total;
+}
+
+// Rebuild scan (RES_TILE buckets per workgroup, 4 per lane): the new offsets
+// from the counts the flight kept current, and from the totals of the tiles
+// before this tile (summed directly, or with tsum2 through the super tiles'
+// totals); the first arrival slot of each bucket (its stayers come first);
+// the occupancy bits. Arrivals are zeroed for the next step. Workgroup 0
+// commits the clocks and budgets of the `tick` segments.
 __global__ __launch_bounds__(256) void k_r_scan_apply(const uint32_t *__restrict__ counts,
                                                       uint32_t *__restrict__ arr,
-                                                      const uint32_t *__restrict__ tsum, uint32_t ntiles,
+                                                      const uint32_t *__restrict__ tsum,
+                                                      const uint32_t *__restrict__ tsum2, uint32_t ntiles,
                                                       uint32_t *__restrict__ noff, uint32_t *__restrict__ abase,
                                                       uint32_t *__restrict__ occ, Seg *tick, uint32_t ntick) {
     if (blockIdx.x == 0 && threadIdx.x == 0) commit_clocks(tick, ntick);
     uint32_t part = 0;
-    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += 256u) part += tsum[t];
+    if (tsum2) {
+        const uint32_t sup = blockIdx.x / RES_SUPER;
+        for (uint32_t u = threadIdx.x; u < sup; u += 256u) part += tsum2[u];
+        const uint32_t t = sup * RES_SUPER + threadIdx.x;  // RES_SUPER == 256: one tile per lane
+        if (t < blockIdx.x) part += tsum[t];
+    } else {
+        for (uint32_t t = threadIdx.x; t < blockIdx.x; t += 256u) part += tsum[t];
+    }
     uint32_t prefix;
     block_excl_scan<4>(part, &prefix);
     const size_t base = (size_t)blockIdx.x * RES_TILE + (size_t)threadIdx.x * 4u;
@@ -625,12 +644,26 @@ hipError_t launch_resident_flush_range(const Geometry &g, const Pools &p, const 
     return hipGetLastError();
 }
 
+// Commits the clocks of a layout with no buckets (every flock left a batch).
+__global__ void k_commit(Seg *tick, uint32_t ntick) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) commit_clocks(tick, ntick);
+}
+
 hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
                                    const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, Seg *tick,
                                    uint32_t ntick, hipStream_t s) {
     const uint32_t ntiles = (g.n_buckets + RES_TILE - 1) / RES_TILE;
-    if (ntiles == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_r_scan_apply, dim3(ntiles), dim3(256), 0, s, w.counts, w.arr, w.tsum, ntiles,
+    if (ntiles == 0) {
+        if (tick && ntick) hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, s, tick, ntick);
+        return hipGetLastError();
+    }
+    const uint32_t *tsum2 = nullptr;
+    if (ntiles > RES_DIRECT_TILES) {
+        hipLaunchKernelGGL(k_r_super_sums, dim3((ntiles + RES_SUPER - 1) / RES_SUPER), dim3(256), 0, s, w.tsum,
+                           ntiles, w.tsum2);
+        tsum2 = w.tsum2;
+    }
+    hipLaunchKernelGGL(k_r_scan_apply, dim3(ntiles), dim3(256), 0, s, w.counts, w.arr, w.tsum, tsum2, ntiles,
                        off_nxt, w.abase, p.occ, tick, tick ? ntick : 0u);
     if (g.n_slots)
         hipLaunchKernelGGL(k_r_move, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, cur, nxt, w,

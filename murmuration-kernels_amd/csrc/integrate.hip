@@ -58,18 +58,18 @@ __global__ __launch_bounds__(TILE) void k_integrate(Geometry g, Pools p) {
 }
 
 // Banked flight over resident records (slot order), in place; see
-// resident_flight. Workgroup 0 commits the step's clocks (each live segment's
-// time += DT, the time decide used).
-__global__ __launch_bounds__(TILE) void k_r_integrate(Geometry g, Records rec, ResWork w, Seg *segs, uint32_t nseg) {
+// resident_flight. Every workgroup reads its flocks' step budgets here, so
+// the clocks and budgets are committed by the rebuild scan that follows, never
+// inside this launch.
+__global__ __launch_bounds__(TILE) void k_r_integrate(Geometry g, Records rec, ResWork w) {
     const uint32_t s = blockIdx.x * TILE + threadIdx.x;
     resident_flight(g, rec, rec, w, s, s < g.n_slots);
-    if (blockIdx.x == 0 && threadIdx.x == 0) commit_clocks(segs, nseg);
 }
 
 hipError_t launch_resident_integrate(const Geometry &g, const Pools &, const Records &rec, const ResWork &w,
-                                     Seg *segs, uint32_t nseg, hipStream_t s) {
+                                     hipStream_t s) {
     if (g.n_slots == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_r_integrate, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, rec, w, segs, nseg);
+    hipLaunchKernelGGL(k_r_integrate, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, rec, w);
     return hipGetLastError();
 }
 

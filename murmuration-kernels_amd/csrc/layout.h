@@ -120,7 +120,17 @@ struct ResWork {
     uint32_t *counts;  // per bucket: birds in the layout (the flight moves each mover's count)
     uint32_t *abase;      // per bucket: first slot of its arrivals in the new layout
     uint32_t *tsum;       // per RES_TILE buckets: their count total, kept from step to step
+    uint32_t *tsum2;      // per RES_SUPER tiles: their total, summed by the rebuild (large layouts only)
 };
+
+// Up to RES_DIRECT_TILES tiles, each workgroup of the rebuild scan adds up the
+// totals of the tiles before it itself (at most 8 loads per lane, one launch).
+// Beyond, a first launch sums the totals of each RES_SUPER consecutive tiles,
+// and a workgroup adds up the super totals before its super tile and the
+// tile totals before it inside its super tile: a bounded number of loads per
+// workgroup, so the scan stays linear in the buckets.
+constexpr uint32_t RES_DIRECT_TILES = 2048;
+constexpr uint32_t RES_SUPER = 256;
 
 // Launch geometry of one build/step over the pools.
 struct Geometry {
@@ -211,13 +221,14 @@ hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Recor
                                   const DecideTiming &tm = DecideTiming{});
 // Resident passes (one flock): records from the bird-order state and a grid
 // built by hash + scan (counts kept); banked flight in slot order with the
-// movers counted (and the flock clock committed: += DT); the rebuild of the
-// grid and the move of every record to its slot in `nxt`; records back to the
-// bird-order state. The rebuild reads the records to move from `cur`; when
-// the flight was fused into decide, `tick` holds the ntick segments whose
-// clocks the rebuild commits (+= DT each live one) instead of the integrate
-// pass. In a batch (g.tile_seg set) every slot's flock is found through its
-// bird id; the enter pass skips the padding birds of each flock's range.
+// movers counted; the rebuild of the grid and the move of every record to its
+// slot in `nxt`; records back to the bird-order state. The rebuild reads the
+// records to move from `cur` and commits the clocks and step budgets of the
+// ntick segments in `tick` (+= DT for each live one with steps left): the
+// flight, fused into decide or not, reads the budgets, so the commit waits
+// for the launch after it. In a batch (g.tile_seg set) every slot's flock is
+// found through its bird id; the enter pass skips the padding birds of each
+// flock's range.
 hipError_t launch_resident_enter(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s);
 // The enter pass for the birds [bird_lo, bird_lo + count) only (a flock that
 // joins a resident batch).
@@ -232,7 +243,7 @@ hipError_t launch_resident_splice(const Geometry &g, const Pools &p, const Recor
 // Each resident scan tile's count total (after the entry scan, counts kept).
 hipError_t launch_resident_tile_sums(const Geometry &g, const Pools &p, const ResWork &w, hipStream_t s);
 hipError_t launch_resident_integrate(const Geometry &g, const Pools &p, const Records &rec, const ResWork &w,
-                                     Seg *segs, uint32_t nseg, hipStream_t s);
+                                     hipStream_t s);
 hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
                                    const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, Seg *tick,
                                    uint32_t ntick, hipStream_t s);

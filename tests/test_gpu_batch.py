@@ -103,3 +103,26 @@ def test_info_and_bytes(fh):
     assert info["live_flocks"] == 2 and info["live_birds"] == 71_000
     assert info["bucket_range"] >= (1 << 16) + (1 << 18)
     assert fh.flock_bytes(1000) == 1024 * 108 + (1 << 16) * 8 + (1 << 16) // 8
+
+
+@pytest.mark.parametrize("flags", [CANON, CANON | GRAPH])
+def test_scan_beyond_the_merged_tile_count(fh, fr, flags):
+    # More than 2048 scan tiles of 4096 buckets (a bucket range over 8.4M):
+    # the scan takes its separate tile-prefix pass. Three flocks of 2^22
+    # buckets each; every flock must equal stepping it alone, and the
+    # reference start the oracle.
+    ws = [fh.gen_reference(2000, 1), cube(fh, 7000, 81), cube(fh, 30_000, 82)]
+    with fh.Batch(100_000, 13 << 20, 4, flags=flags, buckets=1 << 22) as b:
+        ids = [b.add(w) for w in ws]
+        assert b.info()["bucket_range"] > 2048 * 4096
+        b.step(5)
+        got = [b.download(i) for i in ids]
+    for (g, t), w in zip(got, ws):
+        with fh.Flock.from_words(w, flags=CANON, buckets=1 << 22) as f:
+            f.step(5)
+            want, tw = f.download()
+        assert t == tw
+        assert same_bits(g, want)
+    cpu, tc = fr.step(ws[0], 0.0, 5)
+    d = fr.diff(cpu, got[0][0])
+    assert got[0][1] == tc and d["ok"], d

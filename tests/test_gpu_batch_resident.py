@@ -189,12 +189,15 @@ def test_resident_batch_rejects_canonical(fh):
     assert e.value.status == fh.ERR_INVALID
 
 
-def test_step_budgets_freeze_each_flock_on_its_last_step(fh):
+@pytest.mark.parametrize("flags", [RES, RES | GRAPH, RES | GRAPH | 2048])
+def test_step_budgets_freeze_each_flock_on_its_last_step(fh, flags):
     # Budgets of 3, 7 and unlimited steps, then 10 steps of the batch: each
     # flock matches that many steps alone, and a frozen flock continues once
-    # its budget is raised.
-    starts = [cube(fh, n, 50 + n) for n in (4000, 9000, 2500)]
-    with fh.Batch(50_000, 1 << 21, 4, flags=RES | GRAPH | 2048) as b:
+    # its budget is raised. Flocks of 100k+ birds give the flight hundreds of
+    # workgroups, so a budget committed while some of them still read it
+    # would freeze part of a flock a step early.
+    starts = [cube(fh, n, 50 + n) for n in (120_000, 150_000, 2500)]
+    with fh.Batch(400_000, 1 << 22, 4, flags=flags) as b:
         ids = [b.add(w) for w in starts]
         b.limit(ids[0], 3)
         b.limit(ids[1], 7)
@@ -219,3 +222,44 @@ def test_step_budgets_need_the_resident_layout(fh):
         with pytest.raises(fh.FlockHipError) as e:
             b.limit(fid, 5)
         assert e.value.status == fh.ERR_INVALID
+
+
+@pytest.mark.parametrize("flags", [RES, RES | GRAPH | 2048])
+def test_empty_resident_batch_steps_and_takes_new_flocks(fh, flags):
+    # Every flock leaves a resident batch, the empty batch steps, and a new
+    # flock joins it: that flock must follow stepping alone.
+    a, c = cube(fh, 3000, 61), cube(fh, 5000, 62)
+    with fh.Batch(50_000, 1 << 21, 4, flags=flags) as b:
+        ia = b.add(a)
+        b.step(3)
+        b.remove(ia)
+        assert b.info()["live_flocks"] == 0
+        b.step(2)
+        ic = b.add(c, 0.0)
+        b.step(5)
+        got, t = b.download(ic)
+    want, tw = alone(fh, c, 5, CANON)
+    assert t == tw
+    close(got, want)
+
+
+@pytest.mark.parametrize("flags", [RES | GRAPH, RES | GRAPH | 2048 | 128])
+def test_resident_scan_beyond_the_direct_tile_count(fh, fr, flags):
+    # A resident layout of more than 2048 scan tiles (over 2M buckets) sums
+    # the tile totals through super-tile totals; its flocks must follow the
+    # canonical batch, and a small one the oracle.
+    small = fh.gen_reference(2000, 1)
+    ws = [cube(fh, 300_000, 71), cube(fh, 400_000, 72), small]
+    out = {}
+    for name, fl in (("resident", flags), ("canonical", CANON)):
+        with fh.Batch(800_000, 5 << 20, 4, flags=fl) as b:
+            ids = [b.add(w) for w in ws]
+            assert b.info()["bucket_range"] > 2048 * 1024
+            b.step(6)
+            out[name] = [b.download(i) for i in ids]
+    for (g, tg), (w, tw) in zip(out["resident"], out["canonical"]):
+        assert tg == tw
+        close(g, w)
+    cpu, tc = fr.step(small, 0.0, 6)
+    d = fr.diff(cpu, out["resident"][2][0])
+    assert out["resident"][2][1] == tc and d["ok"], d

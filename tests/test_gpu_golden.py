@@ -17,6 +17,10 @@ def load(fh, name):
     return fh.read_flk(os.path.join(HERE, name))
 
 
+def abs_err(a, b):
+    return float(np.max(np.abs(a.astype(np.float64) - b.astype(np.float64))))
+
+
 def rel_err(a, b):
     a = a.astype(np.float64)
     b = b.astype(np.float64)
@@ -36,6 +40,9 @@ def test_one_step_matches_the_fixture(fh, flags):
     assert np.array_equal(got[:, 13].view(np.uint32), want[:, 13].view(np.uint32))
     assert np.array_equal(got[:, 12], want[:, 12])  # next decision times
     assert rel_err(got[:, 0:6], want[:, 0:6]) <= 1e-4
+    # the reference's own GPU backends differ from its oracle by 4.8e-7 m
+    # after one step at this start (logs/cuda-parity-1.log:1-5)
+    assert abs_err(got[:, 0:6], want[:, 0:6]) <= 1e-5
 
 
 @pytest.mark.parametrize("flags", [0, 64 | 4, 2048 | 64 | 4])
@@ -46,7 +53,9 @@ def test_twenty_steps_stay_near_the_fixture(fh, flags):
         f.step(20)
         got, t = f.download()
     assert np.float32(t) == np.float32(tw)
-    drift = rel_err(got[:, 0:6], want[:, 0:6])
-    print("20-step drift from the fixture", flags, drift)
-    assert drift <= 1e-3
+    drift = abs_err(got[:, 0:6], want[:, 0:6])
+    print("20-step drift from the fixture (max abs pos/vel)", flags, drift)
+    # the reference's 100-step parity at this start is 7.6e-6 m
+    # (logs/cuda-parity-100.log:1-5); its pointwise bar is 1e-3 (docs/SPEC.md:70)
+    assert drift <= 1e-4
     assert np.mean(got[:, 13].view(np.uint32) == want[:, 13].view(np.uint32)) >= 0.99

@@ -77,7 +77,7 @@ def test_spatial_step_matches_the_oracle(fh, fr, flags, group):
     cpu, tc = fr.step_grid(w0, 0.0, 1, threads=THREADS)
     d = fr.diff(cpu, gpu)
     print("spatial 1-step", flags, group, d)
-    assert t == tc and d["nb_mismatch"] == 0
+    assert t == tc and d["ok"], d  # absolute 1e-3 and equal neighbour counts (docs/SPEC.md:66-72)
     assert rel_err(gpu[:, 0:6], cpu[:, 0:6]) <= 1e-4
 
 

@@ -286,3 +286,34 @@ def test_flk_errors(fr, tmp_path):
     p.write_bytes(b"FLK1" + np.array([2, 15, 0], np.uint32).tobytes() + bytes(60))
     assert fr.read_flk(p)[2] == 4
     assert fr.read_flk(tmp_path / "missing.flk")[2] == 1
+
+
+@pytest.mark.parametrize("kind,n,cell", [("ref", 2000, 4.0), ("cube", 10_000, 4.0), ("cube", 10_000, 8.0)])
+def test_grid_step_follows_the_brute_force_step(fr, kind, n, cell):
+    # fr.step_grid (the grid walk of cuda/flock.cu:190-290 over the stable
+    # counting sort) is the checker of every large configuration; it must
+    # follow fr.step (brute force, model.rs:182-200) at the reference's own
+    # bar: 1, 5 and 20 steps pointwise within 1e-3 with equal neighbour
+    # counts and decision times (docs/SPEC.md:66-72). Sums differ only in
+    # order, so one step is also within 1e-5.
+    if kind == "ref":
+        w0 = fr.scatter(n, 1)
+    else:
+        side = 4.0 * (n / 0.64) ** (1.0 / 3.0)
+        rng = np.random.default_rng(3)
+        w0 = np.zeros((n, 15), np.float32)
+        w0[:, 0:3] = rng.uniform(-side / 2, side / 2, (n, 3))
+        w0[:, 2] += 100.0
+        d = rng.normal(size=(n, 3))
+        w0[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True) * rng.uniform(9, 11, (n, 1))
+        w0[:, 8] = 1.5
+        w0[:, 12] = rng.uniform(0, 0.05, n)
+    for steps in (1, 5, 20):
+        a, ta = fr.step(w0, 0.0, steps, threads=4)
+        g, tg = fr.step_grid(w0, 0.0, steps, cell=cell, threads=4)
+        assert ta == tg
+        assert np.array_equal(a[:, 12], g[:, 12])
+        d = fr.diff(a, g)
+        assert d["ok"], (steps, d)
+        if steps == 1:
+            assert d["dpos"] <= 1e-5 and d["dvel"] <= 1e-5, d
