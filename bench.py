@@ -88,6 +88,41 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def _cpu_baseline_costs(flocks, due_sample: int, bird_sample: int) -> dict:
+    """Single-thread costs of the oracle's brute-force step (model.rs:182-200)
+    on the given start states: seconds per candidate of one decision (a due
+    bird tests every other bird), timed on up to `due_sample` due birds of
+    each flock, and seconds per bird of step_bird, on a stride sample of up
+    to `bird_sample` birds of each."""
+    from oracle import flockref as fr
+
+    t = np.float32(0.0) + np.float32(0.005)
+    dec_s = cand = 0.0
+    bird_s = birds = 0.0
+    for w in flocks:
+        n = int(w.shape[0])
+        due = np.nonzero(t >= w[:, 12])[0][:due_sample]
+        t0 = time.perf_counter()
+        for i in due:
+            fr.decide(w, int(i))
+        dec_s += time.perf_counter() - t0
+        cand += len(due) * n
+        idx = range(0, n, max(1, n // bird_sample))
+        t0 = time.perf_counter()
+        for i in idx:
+            fr.step_bird(w[i])
+        bird_s += time.perf_counter() - t0
+        birds += len(idx)
+    return {"per_candidate_s": dec_s / max(1.0, cand), "per_bird_s": bird_s / max(1.0, birds),
+            "timed_s": dec_s + bird_s}
+
+
+def _cpu_baseline_step_s(costs: dict, n: int, due: float) -> float:
+    """Modelled single-thread seconds of one oracle step of n birds with
+    `due` of them deciding."""
+    return due * n * costs["per_candidate_s"] + n * costs["per_bird_s"]
+
+
 def cpu_baseline(words: np.ndarray, n: int, sample: int) -> dict:
     """The oracle's brute-force step (single thread) on a bounded sample:
     `sample` due birds' decisions plus step_bird for every bird, extrapolated
@@ -133,6 +168,33 @@ def cpu_baseline(words: np.ndarray, n: int, sample: int) -> dict:
     }
 
 
+def algo_step_bytes(n: int, due: float) -> float:
+    """Algorithmic HBM bytes of one step of one flock: 172 n + 24 B + 32 d
+    (histogram 16n + 8B, scan 8B, scatter 56n + 4B, decide 28n + 4B + 32d,
+    integrate 72n; B = buckets_for(n), the reference's grid)."""
+    import flock_hip as fh
+
+    return 172.0 * n + 24.0 * fh.buckets_for(int(n)) + 32.0 * due
+
+
+def hbm_roofline(nbytes: float, seconds: float, what: str) -> dict:
+    gbs = nbytes / seconds / 1e9
+    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "bytes": nbytes, "basis": what}
+
+
+def plain_point(args) -> tuple[dict, int]:
+    """Tunables and flags of the plain 1M-bird step (the autotuner's cached
+    point for the default plain workload)."""
+    import flock_hip as fh
+    from flock_hip import autotune, workloads
+
+    n = 1_000_000
+    tuned = autotune.tune([workloads.plain(n, 1)], "plain", flags=fh.FLAG_GRAPH, warmup=20, steps=20,
+                          progress=log, **tune_cache(args, f"plain-cube-{n}-seed1"))
+    return tuned["params"], tuned["flags"]
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +216,9 @@ def main() -> int:
                     help="plain leg at these tunables instead of autotuning, e.g. "
                          "'cell=4,decide_block=512,birds_per_thread=4,decide_group=4,bucket_shift=0,flags=4'")
     ap.add_argument("--no-clumps", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched leg (for a kernel trace of the batch and its per-flock comparators)")
     ap.add_argument("--clump-birds", type=int, default=10_000_000)
     ap.add_argument("--clump-steps", type=int, default=20)
     ap.add_argument("--clump-warmup", type=int, default=0)
@@ -176,6 +241,9 @@ def main() -> int:
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": x.numel() * 4, "launches": 3}))
+        return 0
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg(args, plain_point(args))}), flush=True)
         return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -218,9 +286,15 @@ def main() -> int:
             f.sync()
             return (time.perf_counter() - t0) * 1000.0 / args.steps, f.buckets
 
-    ms_per_step, buckets = timed_run(prm, flags)
+    # three runs of the same protocol (scripts/cloud-session.sh:62-63 reports
+    # three per size); the value is their median
+    runs = [timed_run(prm, flags) for _ in range(3)]
+    buckets = runs[0][1]
+    run_ms = [r[0] for r in runs]
+    ms_per_step = float(np.median(run_ms))
     default_ms = timed_run({}, args.flags)[0] if tuned else ms_per_step
-    log(f"plain leg: {ms_per_step:.4f} ms/step tuned, {default_ms:.4f} ms/step at the default tunables")
+    log(f"plain leg: {ms_per_step:.4f} ms/step tuned (runs {', '.join(f'{x:.4f}' for x in run_ms)}), "
+        f"{default_ms:.4f} ms/step at the default tunables")
 
     # Per-pass breakdown with hipEvents on the library's stream (same start).
     with fh.Flock.from_words(words, flags=flags & ~fh.FLAG_GRAPH, **prm) as f:
@@ -279,6 +353,8 @@ def main() -> int:
             "tunables_from_cache": bool(tuned and tuned.get("from_cache")),
             "device": dev["name"],
         },
+        "runs_ms_per_step": run_ms,
+        "runs_spread": (max(run_ms) - min(run_ms)) / ms_per_step,
         "ms_per_step_default_params": default_ms,
         "decide_ms": decide_alone_ms,
         "decide_kernel_ms": dec_ms,
@@ -300,16 +376,19 @@ def main() -> int:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(words, n, args.cpu_sample)
     del words
+    if not args.no_c1:
+        log("C1 leg: 10k birds, 20 + 20 steps")
+        out["c1"] = c1_leg(args)
     if not args.no_reference:
         log("reference-protocol leg")
         out["reference_protocol"] = reference_leg(args)
     if not args.no_batched:
         log("batched leg")
-        out["batched"] = batched_leg(args)
+        out["batched"] = batched_leg(args, (prm, flags))
         log(f"batched leg: speedup {out['batched']['speedup']:.2f}x over plain per-flock stepping")
         if not args.no_served:
             log("served leg")
-            out["served"] = served_leg(args, out["batched"])
+            out["served"] = served_leg(args, (prm, flags))
     if not args.no_clumps:
         log("clumps leg")
         out["clumps"] = clumps_leg(args)
@@ -350,6 +429,92 @@ def reference_leg(args) -> dict:
     return rows
 
 
+def c1_leg(args) -> dict:
+    """The plain step at 10,000 birds (uniform 100 m cube = 0.64 birds per 4 m
+    cell, isotropic headings 10 +- 1 m/s, seed 1): 20 warm-up + 20 timed
+    steps replayed from hipGraphs at autotuned tunables, three runs. Its CPU
+    baseline runs the oracle over the same 40 steps in full and checks the
+    GPU state after every one of them against it."""
+    import flock_hip as fh
+    from flock_hip import autotune, workloads
+
+    t0 = time.perf_counter()
+    n = 10_000
+    w = workloads.plain(n, args.seed)
+    tuned = autotune.tune([w], "plain", flags=fh.FLAG_GRAPH, warmup=20, steps=20, thorough=False, progress=log,
+                          **tune_cache(args, f"plain-cube-{n}-seed{args.seed}"))
+    prm, flags = tuned["params"], tuned["flags"]
+    runs = []
+    for _ in range(3):
+        with fh.Flock.from_words(w, flags=flags, **prm) as f:
+            f.step(20)
+            f.sync()
+            tt = time.perf_counter()
+            f.step(20)
+            f.sync()
+            runs.append((time.perf_counter() - tt) * 1e3 / 20)
+    ms = float(np.median(runs))
+    with fh.Flock.from_words(w, flags=flags & ~fh.FLAG_GRAPH, **prm) as f:
+        f.step(20)
+        passes = f.step(20, 0, timing=True)
+    nbytes = algo_step_bytes(n, passes["due_last"])
+    out = {
+        "workload": f"{n} birds, uniform 100 m cube (0.64 birds per 4 m cell), isotropic headings 10+-1 m/s, "
+                    f"seed {args.seed}; 20 warm-up + 20 timed steps, three runs",
+        "bird_steps_per_s": n / (ms * 1e-3),
+        "ms_per_step": ms,
+        "runs_ms_per_step": runs,
+        "passes_ms": {k: passes[k] for k in ("grid_histogram", "grid_scan", "grid_scatter", "decide", "integrate",
+                                            "step_total")},
+        "roofline": hbm_roofline(nbytes, ms * 1e-3, "172n + 24B + 32d per step (d measured)"),
+        "autotuned": prm,
+        "flags": flags,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = c1_cpu_baseline(w, prm, flags)
+    out["leg_seconds"] = time.perf_counter() - t0
+    return out
+
+
+def c1_cpu_baseline(w: np.ndarray, prm: dict, flags: int) -> dict:
+    """The oracle (brute force, single thread) over the C1 protocol's 40
+    steps in full, timed; the GPU steps the same start one step at a time at
+    the leg's tunables, and every step is checked against the oracle: the
+    reference's pointwise 1e-3 with equal neighbour counts
+    (docs/SPEC.md:66-72)."""
+    import flock_hip as fh
+    from oracle import flockref as fr
+
+    n = int(w.shape[0])
+    gpu = []
+    with fh.Flock.from_words(w, flags=flags & ~fh.FLAG_GRAPH, **prm) as f:
+        for _ in range(40):
+            f.step(1)
+            gpu.append(f.download())
+    cpu, tc = w, 0.0
+    spent = 0.0
+    dpos, dvel, nbm, ok = [], [], [], True
+    for g, tg in gpu:
+        t0 = time.perf_counter()
+        cpu, tc = fr.step(cpu, tc, 1)
+        spent += time.perf_counter() - t0
+        d = fr.diff(cpu, g)
+        dpos.append(d["dpos"])
+        dvel.append(d["dvel"])
+        nbm.append(d["nb_mismatch"])
+        ok = ok and d["ok"] and tg == tc
+    return {
+        "value": n * 40 / spent,
+        "unit": "bird-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"the whole protocol: 40 oracle steps of all {n} birds (brute force, single thread), {spent:.2f} s",
+        "parity_every_step": {"ok": ok, "max_dpos": dpos, "max_dvel": dvel, "nb_mismatch": nbm,
+                              "bar": "max |dpos|, |dvel| <= 1e-3 and equal neighbour counts after every step"},
+        "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model()},
+    }
+
+
 def clumps_leg(args) -> dict:
     """10,000,000 birds in 8 Gaussian clumps (sigma 5 m) with centres uniform
     in a 1 km cube, headings aligned per clump (10 degree cap), 10 m/s; cell,
@@ -375,26 +540,57 @@ def clumps_leg(args) -> dict:
     with fh.Flock.from_words(w, flags=flags & ~fh.FLAG_GRAPH, **prm) as f:
         f.step(args.clump_warmup)
         passes = f.step(args.clump_steps, 0, timing=True)
-    return {
+    ms = wall * 1e3 / args.clump_steps
+    out = {
         "workload": f"{n} birds, 8 Gaussian clumps sigma 5 m, centres uniform in a 1 km cube, aligned headings, "
                     f"10 m/s, seed {args.seed}; {args.clump_warmup} warm-up + {args.clump_steps} timed steps",
         "bird_steps_per_s": n * args.clump_steps / wall,
-        "ms_per_step": wall * 1e3 / args.clump_steps,
+        "ms_per_step": ms,
         "passes_ms": {k: passes[k] for k in ("grid_scan", "grid_scatter", "decide", "integrate", "step_total")},
         "due_birds_last_step": passes["due_last"],
+        "roofline": hbm_roofline(algo_step_bytes(n, n / 10.0), ms * 1e-3,
+                                 "172n + 24B + 32d per step, d = n/10 (the mean over a decision period)"),
         "autotuned": prm,
         "flags": flags,
-        "autotune_trials": len(tuned["trials"]),
         "tunables_from_cache": bool(tuned.get("from_cache")),
-        "leg_seconds": time.perf_counter() - t0,
     }
+    if not args.no_cpu:
+        costs = _cpu_baseline_costs([w], due_sample=12, bird_sample=20_000)
+        step_s = _cpu_baseline_step_s(costs, n, n / 10.0)
+        out["cpu_baseline"] = {
+            "value": n / step_s, "unit": "bird-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"the oracle's brute-force decide (single thread) timed on 12 due birds of the 10M-bird "
+                       f"flock and step_bird on 20,000 of its birds ({costs['timed_s']:.1f} s), extrapolated "
+                       f"to one step ({step_s:.0f} s)"),
+        }
+    out["leg_seconds"] = time.perf_counter() - t0
+    return out
 
 
-def batched_leg(args) -> dict:
+def _plain_per_flock(starts, prm: dict, flags: int, steps: int) -> float:
+    """Seconds of `steps` steps of every flock stepped alone, one flock after
+    another (each on its own handle, created, warmed for 3 steps, timed and
+    destroyed before the next: one flock's device state and graphs live at a
+    time)."""
+    import flock_hip as fh
+
+    total = 0.0
+    for w in starts:
+        with fh.Flock.from_words(w, flags=flags | fh.FLAG_GRAPH, **prm) as f:
+            f.step(3)
+            f.sync()
+            t0 = time.perf_counter()
+            f.step(steps)
+            f.sync()
+            total += time.perf_counter() - t0
+    return total
+
+
+def batched_leg(args, plain_pt) -> dict:
     """256 flocks of log-uniform sizes 1k-200k at 0.64 birds per cell: all of
     them in one batch (one launch per stage, hipGraph, autotuned) against the
     same flocks stepped one flock per launch sequence, each with its own
-    graph and the same tunables and flags."""
+    graph, at the batch's tunables and at the plain 1M-bird step's own."""
     import flock_hip as fh
     from flock_hip import autotune, workloads
 
@@ -413,32 +609,48 @@ def batched_leg(args) -> dict:
         tb = time.perf_counter()
         b.step(args.batch_steps)
         batched_s = time.perf_counter() - tb
-    flocks = [fh.Flock.from_words(w, flags=flags | fh.FLAG_GRAPH, **prm) for w in starts]
-    for f in flocks:
-        f.step(3)
-    tp = time.perf_counter()
-    for f in flocks:
-        f.step(args.batch_steps)
-    plain_s = time.perf_counter() - tp
-    for f in flocks:
-        f.close()
+        due = b.step(1, 0, timing=True)["due_last"]
+    log("batched leg: flocks alone at the batch's tunables")
+    plain_s = _plain_per_flock(starts, prm, flags, args.batch_steps)
+    log("batched leg: flocks alone at the plain step's tunables")
+    plain2_s = _plain_per_flock(starts, plain_pt[0], plain_pt[1], args.batch_steps)
     bs = total * args.batch_steps
-    return {
+    step_bytes_all = sum(172.0 * n + 24.0 * fh.buckets_for(n) for n in sizes) + 32.0 * due
+    out = {
         "workload": f"{len(starts)} flocks, sizes log-uniform 1e3..2e5 (seed {args.seed}), {total} birds, "
                     f"0.64 birds per 4 m cell, {args.batch_steps} steps",
         "birds": total,
         "steps": args.batch_steps,
         "batched_bird_steps_per_s": bs / batched_s,
-        "plain_bird_steps_per_s": bs / plain_s,
-        "speedup": plain_s / batched_s,
         "batched_ms_per_step": batched_s * 1e3 / args.batch_steps,
-        "plain_ms_per_step": plain_s * 1e3 / args.batch_steps,
+        "plain_bird_steps_per_s": bs / min(plain_s, plain2_s),
+        "plain_at_batch_tunables": {"bird_steps_per_s": bs / plain_s, "ms_per_step": plain_s * 1e3 / args.batch_steps},
+        "plain_at_plain_tunables": {"bird_steps_per_s": bs / plain2_s,
+                                    "ms_per_step": plain2_s * 1e3 / args.batch_steps,
+                                    "params": plain_pt[0], "flags": plain_pt[1]},
+        # against the faster of the two per-flock comparators
+        "speedup": min(plain_s, plain2_s) / batched_s,
+        "roofline": hbm_roofline(step_bytes_all, batched_s / args.batch_steps,
+                                 "sum over the flocks of 172n + 24B + 32d per step (d measured)"),
         "autotuned": prm,
         "flags": flags,
-        "autotune_trials": len(tuned["trials"]),
         "tunables_from_cache": bool(tuned.get("from_cache")),
-        "leg_seconds": time.perf_counter() - t0,
     }
+    if not args.no_cpu:
+        order = np.argsort(sizes)
+        pick = [starts[int(order[i])] for i in (0, 64, 128, 192, len(order) - 1)]
+        costs = _cpu_baseline_costs(pick, due_sample=200, bird_sample=20_000)
+        frac = due / total
+        step_s = sum(_cpu_baseline_step_s(costs, n, frac * n) for n in sizes)
+        out["cpu_baseline"] = {
+            "value": total / step_s, "unit": "bird-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"the oracle's brute-force decide (single thread) timed on up to 200 due birds of 5 of the "
+                       f"flocks (smallest, quartiles, largest) and step_bird on up to 20,000 birds of each "
+                       f"({costs['timed_s']:.1f} s), extrapolated to one step of all {len(sizes)} flocks "
+                       f"({step_s:.0f} s)"),
+        }
+    out["leg_seconds"] = time.perf_counter() - t0
+    return out
 
 
 def _replay(fh, req, args, prm, flags) -> dict:
@@ -449,6 +661,7 @@ def _replay(fh, req, args, prm, flags) -> dict:
     lat = np.array([r["done_s"] - r["arrival_s"] for r in ok])
     span = max(r["done_s"] for r in ok) - min(r["arrival_s"] for r in ok)
     done = np.array([r["birds"] * r["steps"] for r in ok], np.float64)
+    nbytes = sum(float(r["steps"]) * algo_step_bytes(int(r["birds"]), r["birds"] / 10.0) for r in ok)
     return {
         "requests": len(req["birds"]),
         "completed": len(ok),
@@ -464,6 +677,8 @@ def _replay(fh, req, args, prm, flags) -> dict:
         "busy_s": st["busy_s"],
         "admit_s": st["admit_s"],
         "retire_s": st["retire_s"],
+        "roofline": hbm_roofline(nbytes, span, "sum over the completed requests of steps x (172n + 24B + 3.2n), "
+                                               "over the span from the first arrival to the last completion"),
     }
 
 
@@ -480,10 +695,11 @@ def _plain_requests(fh, req, count: int, prm: dict, flags: int) -> dict:
             f.download()
         total_s += time.perf_counter() - t0
         bird_steps += float(b) * float(st)
-    return {"requests": count, "bird_steps_per_s": bird_steps / total_s, "seconds": total_s}
+    return {"requests": count, "bird_steps_per_s": bird_steps / total_s, "seconds": total_s, "params": prm,
+            "flags": flags | fh.FLAG_GRAPH}
 
 
-def served_leg(args, batched: dict) -> dict:
+def served_leg(args, plain_pt) -> dict:
     """Flock-simulation requests (birds log-uniform 2e3..5e5, steps uniform
     50..500, own seeds) through the continuous-batching server: admission by
     HBM bytes against a 200 GB budget and a slot cap, state resident on the
@@ -518,21 +734,41 @@ def served_leg(args, batched: dict) -> dict:
     stream = _replay(fh, req, args, prm, flags)
     log(f"served leg: stream p50 {stream['p50_latency_s']:.3f} s, p99 {stream['p99_latency_s']:.3f} s")
     plain = _plain_requests(fh, req, args.plain_sample, prm, flags)
-    return {
+    plain2 = _plain_requests(fh, req, args.plain_sample, plain_pt[0], plain_pt[1])
+    best_plain = max(plain["bird_steps_per_s"], plain2["bird_steps_per_s"])
+    out = {
         "workload": f"{args.requests} requests, Poisson {rate:.1f}/s ({args.load:.0%} of the measured capacity), "
                     f"birds log-uniform 2e3..5e5, steps uniform 50..500, slot cap {args.slots}, 200 GB HBM budget "
-                    f"({(fh.flock_bytes(512) - fh.flock_bytes(256)) // 256} B per bird, birds padded to 256, "
-                    f"+ 8 B per bucket)",
+                    f"({(fh.flock_bytes(512, flags=flags) - fh.flock_bytes(256, flags=flags)) // 256} B per bird "
+                    f"at these flags, birds padded to 256, + bytes per bucket)",
         "rate_per_s": rate,
         **stream,
         "capacity": capacity,
         "plain": plain,
-        "capacity_vs_plain": capacity["bird_steps_per_s"] / plain["bird_steps_per_s"],
+        "plain_at_plain_tunables": plain2,
+        # against the faster of the two per-request comparators
+        "capacity_vs_plain": capacity["bird_steps_per_s"] / best_plain,
         "autotuned": prm,
         "flags": flags,
         "tunables_from_cache": bool(tuned.get("from_cache")),
-        "leg_seconds": time.perf_counter() - t0,
     }
+    if not args.no_cpu:
+        order = np.argsort(req["birds"][: args.plain_sample])
+        pick = [fh.gen_request(int(req["birds"][int(i)]), int(req["seeds"][int(i)])) for i in
+                (order[0], order[len(order) // 2], order[-1])]
+        costs = _cpu_baseline_costs(pick, due_sample=200, bird_sample=20_000)
+        birds = req["birds"].astype(np.float64)
+        steps = req["steps"].astype(np.float64)
+        cpu_s = float(np.sum(steps * (birds * birds / 10.0 * costs["per_candidate_s"] + birds * costs["per_bird_s"])))
+        out["cpu_baseline"] = {
+            "value": float(np.sum(birds * steps)) / cpu_s, "unit": "bird-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"the oracle's brute-force decide (single thread) timed on up to 200 due birds of 3 request "
+                       f"start states (smallest, median, largest of the first {args.plain_sample}) and step_bird "
+                       f"on up to 20,000 birds of each ({costs['timed_s']:.1f} s), extrapolated to all "
+                       f"{args.requests} requests' steps ({cpu_s:.3g} s; one tenth of the birds decide per step)"),
+        }
+    out["leg_seconds"] = time.perf_counter() - t0
+    return out
 
 
 if __name__ == "__main__":

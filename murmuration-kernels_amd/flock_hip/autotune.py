@@ -92,6 +92,7 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
          cache_path: str | None = None, cells=CELLS, progress=None, thorough: bool = True, times=None,
          shifts=SHIFTS, spatial=(0, 1), label: str = "", refresh: bool = False) -> dict:
     """Returns {"params": {...}, "flags": f, "ms_per_step": x, "trials": [...]}
+    ("trials" only when searched; a cache entry holds the chosen point)
     for the batch of `starts` (kind "batch") or the single flock starts[0]
     ("plain"). `flags` is the caller's flags plus the switches the search
     turned on (FLAG_OCC, FLAG_RESIDENT, FLAG_SPATIAL, FLAG_PERSISTENT, FLAG_XCD); `spatial=(0,)` keeps
@@ -189,7 +190,9 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
     out = {"params": params, "flags": out_flags, "ms_per_step": best_t * 1e3,
            "trials": trials, "signature": key}
     if cache_path:
-        cache[key] = out
+        # the cache keeps the chosen point; the trial log is returned only
+        cache[key] = {"params": params, "flags": out_flags, "ms_per_step": best_t * 1e3, "signature": key,
+                      "trial_count": len(trials), "confirmed": [c[0] * 1e3 for c in sorted(confirmed, key=lambda c: c[0])]}
         os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)
         with open(cache_path, "w") as fp:
             json.dump(cache, fp, indent=1)
