@@ -191,7 +191,7 @@ def plain_point(args) -> tuple[dict, int]:
 
     n = 1_000_000
     tuned = autotune.tune([workloads.plain(n, 1)], "plain", flags=fh.FLAG_GRAPH, warmup=20, steps=20,
-                          progress=log, **tune_cache(args, f"plain-cube-{n}-seed1"))
+                          progress=log, **{**tune_cache(args, f"plain-cube-{n}-seed1"), "refresh": False})
     return tuned["params"], tuned["flags"]
 
 

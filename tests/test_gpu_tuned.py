@@ -260,3 +260,48 @@ def test_clumps_sample_20_step_drift(fh, fr):
           f"20 steps |dpos| {d['dpos']:.3e} |dvel| {d['dvel']:.3e}, neighbour mismatches {d['nb_mismatch']}, "
           f"birds beyond 1e-3: {d['beyond_1e-3']}", flush=True)
     assert fr.statdiff(c20, g20)["ok"]
+
+
+def test_c1_10k_tuned_point(fh, fr):
+    # 10,000 birds, 100 m cube, seed 1, at its cached tunables: the oracle
+    # after every one of 40 steps (the C1 protocol's warm-up and timed steps).
+    from flock_hip import workloads
+
+    prm, flags = tuned("plain-cube-10000-seed1")
+    w0 = workloads.plain(10_000, 1)
+    cpu, tc = w0, 0.0
+    worst = 0.0
+    with fh.Flock.from_words(w0, flags=flags, **prm) as f:
+        for s in range(40):
+            f.step(1)
+            gpu, t = f.download()
+            cpu, tc = fr.step(cpu, tc, 1, threads=THREADS)
+            d = fr.diff(cpu, gpu)
+            assert t == tc and d["ok"], (s, d)
+            worst = max(worst, d["dpos"], d["dvel"])
+            if s == 19:
+                print(f"C1 20-step drift at the tuned point: max |dpos|, |dvel| {worst:.3e}", flush=True)
+
+
+@pytest.mark.parametrize("n", [100_000, 1_000_000])
+def test_reference_protocol_tuned_points(fh, fr, n):
+    # Flock::scatter at density 0.5 (state.rs:79-101) at the cached tunables
+    # of the reference-protocol leg: one step against the oracle's grid walk.
+    prm, flags = tuned(f"reference-protocol-{n}-seed1")
+    w0 = fh.gen_reference(n, 1)
+    with fh.Flock.from_words(w0, flags=flags, **prm) as f:
+        f.step(1)
+        gpu, t = f.download()
+    cpu, tc = fr.step_grid(w0, 0.0, 1, threads=THREADS)
+    d = fr.diff(cpu, gpu)
+    print(f"reference protocol {n}: {d}", flush=True)
+    assert t == tc and d["ok"], d
+
+
+def test_every_cached_point_is_checked():
+    # each label of the tuning cache has a test above
+    with open(CACHE) as fp:
+        labels = {k.split("|")[2] for k in json.load(fp)}
+    assert labels <= {"plain-cube-1000000-seed1", "mix-256-seed1", "served-warm250-seed8",
+                      "clumps-8-sigma5-10000000-seed1", "plain-cube-10000-seed1", "reference-protocol-100000-seed1",
+                      "reference-protocol-1000000-seed1"}, labels
