@@ -219,6 +219,8 @@ def main() -> int:
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched leg (for a kernel trace of the batch and its per-flock comparators)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched leg without its per-flock comparators (a kernel trace of the batch alone)")
     ap.add_argument("--clump-birds", type=int, default=10_000_000)
     ap.add_argument("--clump-steps", type=int, default=20)
     ap.add_argument("--clump-warmup", type=int, default=0)
@@ -610,10 +612,13 @@ def batched_leg(args, plain_pt) -> dict:
         b.step(args.batch_steps)
         batched_s = time.perf_counter() - tb
         due = b.step(1, 0, timing=True)["due_last"]
-    log("batched leg: flocks alone at the batch's tunables")
-    plain_s = _plain_per_flock(starts, prm, flags, args.batch_steps)
-    log("batched leg: flocks alone at the plain step's tunables")
-    plain2_s = _plain_per_flock(starts, plain_pt[0], plain_pt[1], args.batch_steps)
+    if args.no_plain:
+        plain_s = plain2_s = float("nan")
+    else:
+        log("batched leg: flocks alone at the batch's tunables")
+        plain_s = _plain_per_flock(starts, prm, flags, args.batch_steps)
+        log("batched leg: flocks alone at the plain step's tunables")
+        plain2_s = _plain_per_flock(starts, plain_pt[0], plain_pt[1], args.batch_steps)
     bs = total * args.batch_steps
     step_bytes_all = sum(172.0 * n + 24.0 * fh.buckets_for(n) for n in sizes) + 32.0 * due
     out = {
