@@ -211,6 +211,27 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
 fh_status fh_sync(fh_flock *h);
 fh_status fh_get_time(const fh_flock *h, float *time);
 
+/* ---- one flock without allocation --------------------------------------- */
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): steps the n birds whose .flk records (FH_WORDS floats
+ * each) the caller holds in DEVICE memory at d_words, in place, `steps` times
+ * from clock `time`, with the launches of fh_step's fused standard path in
+ * canonical order; *time_out (host, may be NULL) gets the new clock. The
+ * result equals fh_create + fh_upload_words + fh_step + fh_download_words with
+ * the same tunables and FH_FLAG_CANONICAL, bit for bit. Launches go to
+ * `stream` (a hipStream_t; NULL = the null stream) and are not waited for.
+ * Every work array lives in one library-held device array per GPU (about
+ * 8 MiB, held until the process ends): no hipMalloc, hipFree or memset is
+ * made, and FH_ERR_INVALID is returned for n > FH_NOALLOC_MAX_BIRDS or more
+ * than 2^17 buckets. Calls that overlap on one GPU share that array and race.
+ * Flags other than FH_FLAG_CANONICAL, _WAVE_AGG, _NO_TILE, _OCC, _SPATIAL,
+ * _XCD and _WINDOW are refused (FH_ERR_INVALID). Kernel-side error bits are
+ * not read back. */
+#define FH_NOALLOC_MAX_BIRDS 65536u
+fh_status fh_step_device_noalloc(float *d_words, uint32_t n, float time, uint32_t steps, const fh_params *params,
+                                 void *stream, float *time_out);
+
 /* ---- debug read-back for parity ----------------------------------------- */
 /* Grid of the last build (any pointer may be NULL): keys[n], counts[B],
  * offsets[B+1], sorted[n] (bird indices in slot order). After fh_scan the

@@ -121,6 +121,7 @@ _SIGS = {
     "fh_batch_step": ([_P, _u32, _u32, _P], C.c_int32),
     "fh_batch_info": ([_P, _P, _P, _P, _P], C.c_int32),
     "fh_flock_bytes": ([_u32, _P], C.c_size_t),
+    "fh_step_device_noalloc": ([_P, _u32, C.c_float, _u32, _P, _P, _P], C.c_int32),
 }
 
 
@@ -342,6 +343,25 @@ class Flock:
         cnt = C.c_uint32(0)
         _check(lib().fh_neighbours(self._h, bird, walk, _ptr(ids), cap, C.addressof(cnt)))
         return ids[: min(cnt.value, cap)].copy() if cnt.value <= cap else ids.copy()
+
+
+NOALLOC_MAX_BIRDS = 65536
+
+
+def step_device_noalloc(d_words: int, n: int, time: float, steps: int, stream: int = 0, *, cell: float = 0.0,
+                        buckets: int = 0, decide_block: int = 0, birds_per_thread: int = 0, flags: int = 0,
+                        decide_group: int = 0, bucket_shift: int = 0) -> float:
+    """fh_step_device_noalloc: steps the n .flk records at device address
+    `d_words` in place (no allocation; launches on `stream`, not waited for);
+    returns the new clock."""
+    p = Params()
+    p.cell, p.buckets, p.decide_block = cell, buckets, decide_block
+    p.birds_per_thread, p.flags, p.decide_group = birds_per_thread, flags, decide_group
+    p.bucket_shift = bucket_shift
+    t = C.c_float(0)
+    _check(lib().fh_step_device_noalloc(d_words, n, np.float32(time), steps, C.addressof(p), stream or None,
+                                        C.addressof(t)))
+    return float(t.value)
 
 
 def flock_bytes(n: int, buckets: int = 0, bucket_shift: int = 0, flags: int = 0) -> int:
