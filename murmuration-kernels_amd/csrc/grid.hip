@@ -503,15 +503,16 @@ __global__ __launch_bounds__(256) void k_r_super_sums(const uint32_t *__restrict
 // from the counts the flight kept current, and from the totals of the tiles
 // before this tile (summed directly, or with tsum2 through the super tiles'
 // totals); the first arrival slot of each bucket (its stayers come first);
-// the occupancy bits. Arrivals are zeroed for the next step. Workgroup 0
-// commits the clocks and budgets of the `tick` segments.
+// the occupancy bits. Arrivals are zeroed for the next step. The grid also
+// commits the clocks and budgets of the `tick` segments (nothing in this
+// launch reads them).
 __global__ __launch_bounds__(256) void k_r_scan_apply(const uint32_t *__restrict__ counts,
                                                       uint32_t *__restrict__ arr,
                                                       const uint32_t *__restrict__ tsum,
                                                       const uint32_t *__restrict__ tsum2, uint32_t ntiles,
                                                       uint32_t *__restrict__ noff, uint32_t *__restrict__ abase,
                                                       uint32_t *__restrict__ occ, Seg *tick, uint32_t ntick) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) commit_clocks(tick, ntick);
+    commit_clocks(tick, ntick, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
     uint32_t part = 0;
     if (tsum2) {
         const uint32_t sup = blockIdx.x / RES_SUPER;
@@ -646,7 +647,7 @@ hipError_t launch_resident_flush_range(const Geometry &g, const Pools &p, const 
 
 // Commits the clocks of a layout with no buckets (every flock left a batch).
 __global__ void k_commit(Seg *tick, uint32_t ntick) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) commit_clocks(tick, ntick);
+    commit_clocks(tick, ntick, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,

@@ -150,14 +150,20 @@ __device__ __forceinline__ uint32_t slot_segment(const Geometry &g, const Record
 }
 
 // Advances the clock of every live segment with steps left by DT and spends
-// one step of its budget (one thread). A segment with no steps left is
-// frozen: its birds are neither due nor moved, and its clock stays.
-__device__ __forceinline__ void commit_clocks(Seg *segs, uint32_t nseg) {
-    for (uint32_t i = 0; i < nseg; ++i)
-        if (segs[i].live && segs[i].left) {
-            segs[i].time = segs[i].time + DT;
-            if (segs[i].left != SEG_UNLIMITED) segs[i].left -= 1u;
+// one step of its budget. A segment with no steps left is frozen: its birds
+// are neither due nor moved, and its clock stays. Segments are independent:
+// thread `lane` of `lanes` (a grid-stride split over the launch) commits
+// segments lane, lane + lanes, ..., so a batch's clocks take one round of
+// loads instead of one dependent round per flock.
+__device__ __forceinline__ void commit_clocks(Seg *segs, uint32_t nseg, uint32_t lane, uint32_t lanes) {
+    for (uint32_t i = lane; i < nseg; i += lanes) {
+        Seg &sg = segs[i];
+        const uint32_t left = sg.left;
+        if (sg.live && left) {
+            sg.time = sg.time + DT;
+            if (left != SEG_UNLIMITED) sg.left = left - 1u;
         }
+    }
 }
 
 // Launchers (stream-ordered, no host synchronisation). Each returns the
