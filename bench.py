@@ -219,6 +219,7 @@ def main() -> int:
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched leg (for a kernel trace of the batch and its per-flock comparators)")
+    ap.add_argument("--clumps-only", action="store_true", help="only the clumped 10M-bird leg (for its profiles)")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched leg without its per-flock comparators (a kernel trace of the batch alone)")
     ap.add_argument("--clump-birds", type=int, default=10_000_000)
@@ -243,6 +244,9 @@ def main() -> int:
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": x.numel() * 4, "launches": 3}))
+        return 0
+    if args.clumps_only:
+        print(json.dumps({"clumps": clumps_leg(args)}), flush=True)
         return 0
     if args.batched_only:
         print(json.dumps({"batched": batched_leg(args, plain_point(args))}), flush=True)
