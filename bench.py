@@ -217,9 +217,8 @@ def main() -> int:
                          "'cell=4,decide_block=512,birds_per_thread=4,decide_group=4,bucket_shift=0,flags=4'")
     ap.add_argument("--no-clumps", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
-    ap.add_argument("--batched-only", action="store_true",
-                    help="only the batched leg (for a kernel trace of the batch and its per-flock comparators)")
-    ap.add_argument("--clumps-only", action="store_true", help="only the clumped 10M-bird leg (for its profiles)")
+    ap.add_argument("--only", choices=("clumps", "batched", "served", "reference", "c1"),
+                    help="run this leg alone (for its profiles, or to retune it with --retune)")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched leg without its per-flock comparators (a kernel trace of the batch alone)")
     ap.add_argument("--clump-birds", type=int, default=10_000_000)
@@ -245,11 +244,11 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": x.numel() * 4, "launches": 3}))
         return 0
-    if args.clumps_only:
-        print(json.dumps({"clumps": clumps_leg(args)}), flush=True)
-        return 0
-    if args.batched_only:
-        print(json.dumps({"batched": batched_leg(args, plain_point(args))}), flush=True)
+    if args.only:
+        legs = {"clumps": lambda: clumps_leg(args), "batched": lambda: batched_leg(args, plain_point(args)),
+                "served": lambda: served_leg(args, plain_point(args)), "reference": lambda: reference_leg(args),
+                "c1": lambda: c1_leg(args)}
+        print(json.dumps({args.only: legs[args.only]()}), flush=True)
         return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
