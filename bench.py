@@ -685,6 +685,8 @@ def _replay(fh, req, args, prm, flags) -> dict:
         "busy_s": st["busy_s"],
         "admit_s": st["admit_s"],
         "retire_s": st["retire_s"],
+        "gap_s": st["gap_s"],
+        "gap_share": st["gap_s"] / max(1e-9, st["gap_s"] + st["busy_s"]),
         "roofline": hbm_roofline(nbytes, span, "sum over the completed requests of steps x (172n + 24B + 3.2n), "
                                                "over the span from the first arrival to the last completion"),
     }

@@ -317,8 +317,13 @@ typedef struct fh_server_stats {
     double wall_s;             /* time since the server started */
     double peak_bytes;         /* largest admitted HBM bytes */
     uint32_t peak_active, queued, active;
-    double admit_s;            /* time adding admitted flocks to the batch (GPU idle meanwhile) */
-    double retire_s;           /* time downloading and removing finished flocks (GPU idle meanwhile) */
+    double admit_s;            /* GPU-idle time spent adding admitted flocks to the batch */
+    double retire_s;           /* GPU-idle time spent downloading and removing finished flocks */
+    double gap_s;              /* resident batches: all GPU-idle time between step batches (admit_s +
+                                  retire_s + the loop's own work); with FH_FLAG_RESIDENT, start states
+                                  are staged in pinned memory ahead of admission and copies run on a
+                                  second stream while steps run, so admissions and downloads overlap
+                                  the steps and only one splice and scan per iteration is idle time */
 } fh_server_stats;
 
 typedef struct fh_server fh_server;

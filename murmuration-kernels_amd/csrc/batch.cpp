@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cstring>
 #include <map>
+#include <string>
 #include <vector>
 
 #include "batch.h"
@@ -63,6 +64,14 @@ static fh_status bcheck(const fh_batch *b, const char *fn) {
     return b ? FH_OK : set_error(FH_ERR_INVALID, "%s: NULL batch", fn);
 }
 
+// Queued membership changes of a segment (fh_batch::pending).
+constexpr uint8_t PENDING_JOIN = 1, PENDING_LEAVE = 2;
+
+// A live flock that takes part in the steps: not one still waiting to join.
+static bool in_layout(const fh_batch *b, uint32_t f) {
+    return b->segs[f].live && (f >= b->pending.size() || b->pending[f] != PENDING_JOIN);
+}
+
 // Geometry of the live flocks: bird range up to the highest live tile, bucket
 // range up to the highest live bucket.
 static void refresh_geometry(fh_batch *b) {
@@ -72,8 +81,8 @@ static void refresh_geometry(fh_batch *b) {
     b->geo.n_birds = b->tiles.high_water() * TILE;
     b->geo.n_buckets = b->bucket_units.high_water() * SCAN_TILE;
     uint32_t slots = 0;
-    for (const Seg &s : b->segs)
-        if (s.live) slots += s.n;
+    for (uint32_t f = 0; f < b->segs.size(); ++f)
+        if (in_layout(b, f)) slots += b->segs[f].n;
     b->geo.n_slots = slots;
     b->graph_valid = false;
 }
@@ -344,7 +353,7 @@ static fh_status batch_enter_resident(fh_batch *b) {
 static std::vector<uint32_t> resident_slot_starts(const fh_batch *b) {
     std::vector<uint32_t> order;
     for (uint32_t i = 0; i < b->segs.size(); ++i)
-        if (b->segs[i].live) order.push_back(i);
+        if (in_layout(b, i)) order.push_back(i);
     std::sort(order.begin(), order.end(),
               [&](uint32_t x, uint32_t y) { return b->segs[x].bucket_base < b->segs[y].bucket_base; });
     std::vector<uint32_t> start(b->segs.size(), 0u);
@@ -489,6 +498,37 @@ size_t resident_bytes_per_bird(bool flight) {
 }
 size_t resident_bytes_per_bucket() { return 4 /*arr*/ + 4 /*abase*/ + 4 /*offsets*/ + 1 /*tile totals*/; }
 
+// One step of every live flock (launched, not waited for), with per-pass
+// events when `ev` is given.
+static fh_status one_step(fh_batch *b, bool graph, hipEvent_t *ev) {
+    for (size_t f = 0; f < b->segs.size(); ++f)  // mirrors the device clock add (and budget)
+        if (in_layout(b, (uint32_t)f) && (!b->resident || b->segs[f].left)) {
+            b->times[f] += DT;
+            if (b->resident && b->segs[f].left != SEG_UNLIMITED) b->segs[f].left -= 1u;
+        }
+    fh_status st = FH_OK;
+    if (b->resident) {
+        if (graph) {
+            if ((st = batch_capture_resident(b))) return st;
+            hipError_t e = hipGraphLaunch(b->rgraph[b->parity], b->stream);
+            if (e != hipSuccess) st = hip_fail(e, "hipGraphLaunch");
+        } else {
+            st = batch_resident_launches(b, ev);
+        }
+        b->parity ^= 1u;
+    } else if (graph) {
+        if (!b->hashed && (st = batch_hash(b))) return st;
+        if ((st = batch_capture(b))) return st;
+        hipError_t e = hipGraphLaunch(b->graph, b->stream);
+        if (e != hipSuccess) st = hip_fail(e, "hipGraphLaunch");
+        b->hashed = true;
+        b->counts_dirty = true;
+    } else {
+        st = batch_step_launches(b, ev);
+    }
+    return st;
+}
+
 fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *out) {
     if (warmup > steps) warmup = steps;
     const bool graph = (b->prm.flags & FH_FLAG_GRAPH) != 0;
@@ -515,30 +555,7 @@ fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *o
             t0 = std::chrono::steady_clock::now();
             if (out && graph) (void)hipEventRecord(ev[0], b->stream);
         }
-        for (size_t f = 0; f < b->segs.size(); ++f)  // mirrors the device clock add (and budget)
-            if (b->segs[f].live && (!b->resident || b->segs[f].left)) {
-                b->times[f] += DT;
-                if (b->resident && b->segs[f].left != SEG_UNLIMITED) b->segs[f].left -= 1u;
-            }
-        if (b->resident) {
-            if (graph) {
-                if ((st = batch_capture_resident(b))) break;
-                hipError_t e = hipGraphLaunch(b->rgraph[b->parity], b->stream);
-                if (e != hipSuccess) st = hip_fail(e, "hipGraphLaunch");
-            } else {
-                st = batch_resident_launches(b, (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 8] : nullptr);
-            }
-            b->parity ^= 1u;
-        } else if (graph) {
-            if (!b->hashed && (st = batch_hash(b))) break;
-            if ((st = batch_capture(b))) break;
-            hipError_t e = hipGraphLaunch(b->graph, b->stream);
-            if (e != hipSuccess) st = hip_fail(e, "hipGraphLaunch");
-            b->hashed = true;
-            b->counts_dirty = true;
-        } else {
-            st = batch_step_launches(b, (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 8] : nullptr);
-        }
+        st = one_step(b, graph, (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 8] : nullptr);
     }
     if (st) {
         (void)hipStreamSynchronize(b->stream);
@@ -591,6 +608,284 @@ fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *o
         out->due_last = due;
     }
     destroy();
+    uint32_t err = 0;
+    FH_HIP(hipMemcpy(&err, b->P.err, 4, hipMemcpyDeviceToHost));
+    if (err & ERR_SLOT) return set_error(FH_ERR_HIP, "kernel reported a slot outside the sorted range (0x%x)", err);
+    return FH_OK;
+}
+
+size_t staged_size(uint32_t n) { return staged_bytes(n); }
+
+void stage_words(const float *words, uint32_t n, void *dst) {
+    const Staged h = staged(dst, n);
+    for (size_t i = 0; i < n; ++i) {
+        const float *w = words + i * FH_WORDS;
+        h.pos[i] = make_float4(w[PX], w[PY], w[PZ], w[NEXT]);
+        h.vel[i] = make_float4(w[VX], w[VY], w[VZ], w[ROLL]);
+        h.steer[i] = make_float4(w[SX], w[SY], w[SZ], w[RADIUS]);
+        h.spin[i] = w[SPIN];
+        std::memcpy(&h.nb[i], &w[NB], 4);
+        h.cen[i] = w[CEN];
+    }
+}
+
+void unstage_words(const void *src, uint32_t n, float *words) {
+    const Staged h = staged(const_cast<void *>(src), n);
+    for (size_t i = 0; i < n; ++i) {
+        float *w = words + i * FH_WORDS;
+        w[PX] = h.pos[i].x; w[PY] = h.pos[i].y; w[PZ] = h.pos[i].z;
+        w[VX] = h.vel[i].x; w[VY] = h.vel[i].y; w[VZ] = h.vel[i].z;
+        w[ROLL] = h.vel[i].w;
+        w[SPIN] = h.spin[i];
+        w[RADIUS] = h.steer[i].w;
+        w[SX] = h.steer[i].x; w[SY] = h.steer[i].y; w[SZ] = h.steer[i].z;
+        w[NEXT] = h.pos[i].w;
+        std::memcpy(&w[NB], &h.nb[i], 4);
+        w[CEN] = h.cen[i];
+    }
+}
+
+// The six arrays of the staged records of segment s and their pool addresses.
+struct Part {
+    void *pool;
+    void *host;
+    size_t bytes;
+};
+static void staged_parts(fh_batch *b, const Seg &s, void *host, Part out[6]) {
+    const Staged h = staged(host, s.n);
+    const size_t o = s.bird_base, n = s.n;
+    out[0] = {b->P.pos4 + o, h.pos, n * sizeof(float4)};
+    out[1] = {b->P.vel4 + o, h.vel, n * sizeof(float4)};
+    out[2] = {b->P.steer4 + o, h.steer, n * sizeof(float4)};
+    out[3] = {b->P.spin + o, h.spin, n * 4};
+    out[4] = {b->P.cen + o, h.cen, n * 4};
+    out[5] = {b->P.nb + o, h.nb, n * 4};
+}
+
+fh_status batch_join_async(fh_batch *b, const void *src, uint32_t n, float time, uint32_t steps, uint32_t *fid) {
+    if (!(b->prm.flags & FH_FLAG_RESIDENT)) return set_error(FH_ERR_INVALID, "batch_join_async: needs FH_FLAG_RESIDENT");
+    if (!src || !n || !fid) return set_error(FH_ERR_INVALID, "batch_join_async: need records, n > 0 and fid");
+    const uint32_t buckets = flock_buckets(n, b->prm);
+    const uint32_t ntile = (n + TILE - 1) / TILE;
+    const uint32_t nunit = (buckets + SCAN_TILE - 1) / SCAN_TILE;
+    uint32_t f = NO_SEG;
+    for (uint32_t i = 0; i < b->segs.size(); ++i)
+        if (!b->segs[i].live && !b->pending[i]) {
+            f = i;
+            break;
+        }
+    if (f == NO_SEG) {
+        if (b->segs.size() >= b->flock_cap) return set_error(FH_ERR_BUDGET, "batch_join_async: flock slots full");
+        f = (uint32_t)b->segs.size();
+        b->segs.push_back(Seg{});
+        b->times.push_back(0.0f);
+        b->pending.push_back(0);
+    }
+    const uint32_t t0 = b->tiles.alloc(ntile);
+    if (t0 == RangeAlloc::NONE) return set_error(FH_ERR_BUDGET, "batch_join_async: no room for %u birds", n);
+    const uint32_t u0 = b->bucket_units.alloc(nunit);
+    if (u0 == RangeAlloc::NONE) {
+        b->tiles.release(t0);
+        return set_error(FH_ERR_BUDGET, "batch_join_async: no room for %u buckets", buckets);
+    }
+    Seg sg{};
+    sg.bird_base = t0 * TILE;
+    sg.n = n;
+    sg.bucket_base = u0 * SCAN_TILE;
+    sg.mask = buckets - 1;
+    sg.cell = b->prm.cell;
+    sg.inv_cell = 1.0f / b->prm.cell;
+    sg.time = time;
+    sg.live = 1;
+    sg.mode = (b->prm.flags & FH_FLAG_SPATIAL) ? SEG_SPATIAL : 0u;
+    sg.left = steps;
+    Part parts[6];
+    staged_parts(b, sg, const_cast<void *>(src), parts);
+    hipError_t e = hipSuccess;
+    for (const Part &pt : parts)
+        if (e == hipSuccess) e = hipMemcpyAsync(pt.pool, pt.host, pt.bytes, hipMemcpyHostToDevice, b->copy);
+    if (e == hipSuccess) e = hipEventRecord(b->up_done, b->copy);
+    if (e != hipSuccess) {
+        // the ranges stay reserved: an upload may be in flight into them
+        return hip_fail(e, "batch_join_async: upload");
+    }
+    b->uploads = true;
+    b->segs[f] = sg;
+    b->times[f] = time;
+    b->pending[f] = PENDING_JOIN;
+    for (uint32_t t = 0; t < ntile; ++t) b->tile_seg[t0 + t] = f;
+    b->joins.push_back(f);
+    *fid = f;
+    return FH_OK;
+}
+
+fh_status batch_leave_async(fh_batch *b, uint32_t fid, void *dst, float *time, hipEvent_t done) {
+    if (fid >= b->segs.size() || !b->segs[fid].live || b->pending[fid])
+        return set_error(FH_ERR_INVALID, "batch_leave_async: no flock %u in the layout", fid);
+    if (!b->resident) return set_error(FH_ERR_STATE, "batch_leave_async: the batch has not stepped");
+    if (!dst) return set_error(FH_ERR_INVALID, "batch_leave_async: NULL destination");
+    const Seg sg = b->segs[fid];
+    FH_HIP(launch_resident_flush_range(b->geo, b->P, b->R[b->parity], resident_slot_starts(b)[fid], sg.n,
+                                       b->stream));
+    FH_HIP(hipEventRecord(b->flushed, b->stream));
+    FH_HIP(hipStreamWaitEvent(b->copy, b->flushed, 0));
+    Part parts[6];
+    staged_parts(b, sg, dst, parts);
+    for (const Part &pt : parts) FH_HIP(hipMemcpyAsync(pt.host, pt.pool, pt.bytes, hipMemcpyDeviceToHost, b->copy));
+    if (done) FH_HIP(hipEventRecord(done, b->copy));
+    if (time) *time = b->times[fid];
+    b->pending[fid] = PENDING_LEAVE;
+    b->leaves.push_back(fid);
+    return FH_OK;
+}
+
+// Pinned staging of the tables one apply uploads: segment tables A and B,
+// tile -> segment mirrors A and B, and the splice shifts.
+static fh_status table_staging(fh_batch *b, Seg **sa, Seg **sb, uint32_t **ta, uint32_t **tb, int32_t **sh) {
+    const size_t ns = b->flock_cap, nt = b->tile_seg.size();
+    const size_t segs_b = (ns * sizeof(Seg) + 255) & ~size_t(255), tiles_b = (nt * 4 + 255) & ~size_t(255);
+    const size_t need = 2 * segs_b + 2 * tiles_b + ns * 4;
+    if (b->tbl_bytes < need) {
+        if (b->tbl) (void)hipHostFree(b->tbl);
+        b->tbl = nullptr;
+        b->tbl_bytes = 0;
+        FH_HIP(hipHostMalloc(&b->tbl, need, hipHostMallocDefault));
+        b->tbl_bytes = need;
+    }
+    char *p = static_cast<char *>(b->tbl);
+    *sa = reinterpret_cast<Seg *>(p);
+    *sb = reinterpret_cast<Seg *>(p + segs_b);
+    *ta = reinterpret_cast<uint32_t *>(p + 2 * segs_b);
+    *tb = reinterpret_cast<uint32_t *>(p + 2 * segs_b + tiles_b);
+    *sh = reinterpret_cast<int32_t *>(p + 2 * segs_b + 2 * tiles_b);
+    return FH_OK;
+}
+
+// Uploads the segment table and the tile -> segment entries of the given
+// segments' ranges from pinned staging `seg_st` / `tile_st` (no host wait).
+static fh_status push_tables_async(fh_batch *b, Seg *seg_st, uint32_t *tile_st, const std::vector<uint32_t> &fs) {
+    for (size_t f = 0; f < b->segs.size(); ++f) {
+        seg_st[f] = b->segs[f];
+        seg_st[f].time = b->times[f];
+    }
+    if (!b->segs.empty())
+        FH_HIP(hipMemcpyAsync(b->d_segs, seg_st, b->segs.size() * sizeof(Seg), hipMemcpyHostToDevice, b->stream));
+    for (uint32_t f : fs) {
+        const uint32_t t0 = b->segs[f].bird_base / TILE, nt = (b->segs[f].n + TILE - 1) / TILE;
+        std::memcpy(tile_st + t0, b->tile_seg.data() + t0, (size_t)nt * 4);
+        FH_HIP(hipMemcpyAsync(b->d_tile_seg + t0, tile_st + t0, (size_t)nt * 4, hipMemcpyHostToDevice, b->stream));
+    }
+    return FH_OK;
+}
+
+fh_status batch_apply(fh_batch *b) {
+    if (b->joins.empty() && b->leaves.empty()) return FH_OK;
+    if (b->uploads) {
+        FH_HIP(hipStreamWaitEvent(b->stream, b->up_done, 0));
+        b->uploads = false;
+    }
+    Seg *sa, *sb;
+    uint32_t *ta, *tb;
+    int32_t *sh;
+    fh_status st = table_staging(b, &sa, &sb, &ta, &tb, &sh);
+    if (st) return st;
+    if (!b->resident) {
+        // Not stepped yet: the flocks join the bird-order state, and the next
+        // step hashes every flock and enters the resident layout.
+        if (!b->leaves.empty()) return set_error(FH_ERR_STATE, "batch_apply: leave before the first step");
+        for (uint32_t f : b->joins) b->pending[f] = 0;
+        if ((st = push_tables_async(b, sa, ta, b->joins))) return st;
+        refresh_geometry(b);
+        b->hashed = false;
+        b->joins.clear();
+        return FH_OK;
+    }
+    const uint32_t q = b->parity;
+    // Slot ranges before (every flock in the layout, the leaving ones too) and
+    // after (without the leaving flocks, with the joining ones), in bucket order.
+    std::vector<uint32_t> order;
+    for (uint32_t i = 0; i < b->segs.size(); ++i)
+        if (b->segs[i].live) order.push_back(i);
+    std::sort(order.begin(), order.end(),
+              [&](uint32_t x, uint32_t y) { return b->segs[x].bucket_base < b->segs[y].bucket_base; });
+    std::vector<uint32_t> old_start(b->segs.size(), 0u), new_start(b->segs.size(), 0u);
+    uint32_t n_cur = 0, n_new = 0;
+    for (uint32_t i : order) {
+        if (b->pending[i] != PENDING_JOIN) {
+            old_start[i] = n_cur;
+            n_cur += b->segs[i].n;
+        }
+        if (b->pending[i] != PENDING_LEAVE) {
+            new_start[i] = n_new;
+            n_new += b->segs[i].n;
+        }
+    }
+    for (uint32_t i = 0; i < b->segs.size(); ++i)
+        sh[i] = b->pending[i] == PENDING_LEAVE ? SPLICE_DROP
+                : (b->segs[i].live && !b->pending[i]) ? (int32_t)new_start[i] - (int32_t)old_start[i]
+                                                      : 0;
+    // Tables A: the joining flocks in, the leaving ones still there (the
+    // splice finds a record's flock through its bird id).
+    if ((st = push_tables_async(b, sa, ta, b->joins))) return st;
+    Geometry gu = b->geo;  // the ranges of the leaving and the joining flocks together
+    gu.n_birds = b->tiles.high_water() * TILE;
+    gu.n_buckets = b->bucket_units.high_water() * SCAN_TILE;
+    gu.n_slots = n_new;
+    for (uint32_t f : b->leaves) FH_HIP(launch_fill(b->P.counts + b->segs[f].bucket_base, seg_bucket_span(b->segs[f]), 0u, b->stream));
+    for (uint32_t f : b->joins) {
+        const Seg &sg = b->segs[f];
+        FH_HIP(launch_fill(b->P.counts + sg.bucket_base, seg_bucket_span(sg), 0u, b->stream));
+        Geometry g = gu;
+        g.tile_begin = sg.bird_base / TILE;
+        g.tile_count = (sg.n + TILE - 1) / TILE;
+        FH_HIP(launch_hash_count(g, b->P, (b->prm.flags & FH_FLAG_WAVE_AGG) != 0, b->stream));
+    }
+    FH_HIP(hipMemcpyAsync(b->d_shift, sh, b->segs.size() * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+    FH_HIP(launch_resident_splice(gu, b->P, b->R[q], b->R[1u - q], b->d_shift, n_cur, n_new, b->stream));
+    b->parity = 1u - q;
+    // Tables B: the leaving flocks out; their ranges free for later joins
+    // (their downloads run before any later upload on the copy stream).
+    std::vector<uint32_t> gone = b->leaves;
+    for (uint32_t f : gone) {
+        Seg &sg = b->segs[f];
+        const uint32_t t0 = sg.bird_base / TILE, nt = (sg.n + TILE - 1) / TILE;
+        for (uint32_t t = 0; t < nt; ++t) b->tile_seg[t0 + t] = NO_SEG;
+        b->tiles.release(t0);
+        b->bucket_units.release(sg.bucket_base / SCAN_TILE);
+        sg.live = 0;
+        b->pending[f] = 0;
+    }
+    for (uint32_t f : b->joins) b->pending[f] = 0;
+    if ((st = push_tables_async(b, sb, tb, gone))) return st;
+    refresh_geometry(b);
+    if ((st = resident_rescan(b))) return st;
+    Pools p = b->P;
+    p.offsets = b->off[b->parity];
+    FH_HIP(launch_resident_tile_sums(b->geo, p, b->W, b->stream));
+    for (uint32_t f : b->joins)
+        FH_HIP(launch_resident_enter_range(b->geo, p, b->R[b->parity], b->segs[f].bird_base, b->segs[f].n, b->stream));
+    b->joins.clear();
+    b->leaves.clear();
+    return FH_OK;
+}
+
+fh_status batch_step_async(fh_batch *b, uint32_t steps) {
+    if (!b->joins.empty() || !b->leaves.empty())
+        return set_error(FH_ERR_STATE, "batch_step_async: membership changes queued (batch_apply first)");
+    const bool graph = (b->prm.flags & FH_FLAG_GRAPH) != 0;
+    if ((b->prm.flags & FH_FLAG_RESIDENT) && steps && !b->resident && b->geo.n_slots) {
+        fh_status st = batch_enter_resident(b);
+        if (st) return st;
+    }
+    for (uint32_t k = 0; k < steps; ++k) {
+        fh_status st = one_step(b, graph, nullptr);
+        if (st) return st;
+    }
+    return FH_OK;
+}
+
+fh_status batch_wait(fh_batch *b) {
+    FH_HIP(hipStreamSynchronize(b->stream));
     uint32_t err = 0;
     FH_HIP(hipMemcpy(&err, b->P.err, 4, hipMemcpyDeviceToHost));
     if (err & ERR_SLOT) return set_error(FH_ERR_HIP, "kernel reported a slot outside the sorted range (0x%x)", err);
@@ -679,23 +974,27 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
     b->d_tile_seg = reinterpret_cast<uint32_t *>(base + o_tseg);
     b->tile_seg.assign(tile_cap, NO_SEG);
 
-    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) {
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&b->copy, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->up_done, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->flushed, hipEventDisableTiming)) != hipSuccess) {
+        if (b->flushed) (void)hipEventDestroy(b->flushed);
+        if (b->up_done) (void)hipEventDestroy(b->up_done);
+        if (b->copy) (void)hipStreamDestroy(b->copy);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
         (void)hipFree(b->arena);
         delete b;
-        return hip_fail(e, "hipStreamCreateWithFlags");
+        return hip_fail(e, "fh_batch_create: streams and events");
     }
     if ((e = hipMemsetAsync(b->arena, 0, a.used, b->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(b->stream)) != hipSuccess) {
-        (void)hipStreamDestroy(b->stream);
-        (void)hipFree(b->arena);
-        delete b;
+        fh_batch_destroy(b);
         return hip_fail(e, "hipMemsetAsync(batch arena)");
     }
     if ((st = push_tables(b, 0, tile_cap))) {
-        (void)hipStreamDestroy(b->stream);
-        (void)hipFree(b->arena);
-        delete b;
-        return st;
+        const std::string msg = fh_last_error();
+        fh_batch_destroy(b);
+        return set_error(st, "%s", msg.c_str());
     }
     refresh_geometry(b);
     *out = b;
@@ -709,9 +1008,14 @@ fh_status fh_batch_destroy(fh_batch *b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (auto &gx : b->rgraph)
         if (gx) (void)hipGraphExecDestroy(gx);
+    if (b->copy) (void)hipStreamSynchronize(b->copy);
     if (b->res_arena) (void)hipFree(b->res_arena);
     if (b->pin) (void)hipHostFree(b->pin);
+    if (b->tbl) (void)hipHostFree(b->tbl);
     (void)hipFree(b->arena);
+    if (b->up_done) (void)hipEventDestroy(b->up_done);
+    if (b->flushed) (void)hipEventDestroy(b->flushed);
+    if (b->copy) (void)hipStreamDestroy(b->copy);
     (void)hipStreamDestroy(b->stream);
     delete b;
     return FH_OK;
@@ -729,7 +1033,7 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
     // find a free segment slot
     uint32_t f = NO_SEG;
     for (uint32_t i = 0; i < b->segs.size(); ++i)
-        if (!b->segs[i].live) {
+        if (!b->segs[i].live && !b->pending[i]) {
             f = i;
             break;
         }
@@ -738,6 +1042,7 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
         f = (uint32_t)b->segs.size();
         b->segs.push_back(Seg{});
         b->times.push_back(0.0f);
+        b->pending.push_back(0);
     }
     const uint32_t t0 = b->tiles.alloc(ntile);
     if (t0 == RangeAlloc::NONE) return set_error(FH_ERR_BUDGET, "fh_batch_add: no room for %u birds", n);

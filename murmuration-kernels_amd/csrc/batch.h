@@ -101,8 +101,47 @@ struct fh_batch {
     // Helper threads for the record conversions of large flocks (made on
     // first use).
     std::unique_ptr<fh::ParFor> par;
+    // Joins and leaves queued for the next batch_apply (resident batches):
+    // per segment 0, PENDING_JOIN or PENDING_LEAVE. Uploads and downloads of
+    // the queued flocks run on `copy`, a second stream, while steps run on
+    // `stream`; `up_done` follows the last upload, `flushed` the last flush.
+    std::vector<uint8_t> pending;
+    std::vector<uint32_t> joins, leaves;
+    hipStream_t copy = nullptr;
+    hipEvent_t up_done = nullptr, flushed = nullptr;
+    bool uploads = false;  // an upload was queued since the last apply
+    // Pinned staging of the tables an apply uploads (two generations of the
+    // segment table and of tile -> segment, and the splice shifts), so its
+    // copies are asynchronous.
+    void *tbl = nullptr;
+    size_t tbl_bytes = 0;
 };
 
 namespace fh {
 fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *out);
+
+// The host form of a flock's records as the pools hold them (structure of
+// arrays: pos, vel, steer as float4, then spin, centrality, neighbours), the
+// unit of the asynchronous joins and leaves below.
+size_t staged_size(uint32_t n);
+void stage_words(const float *words, uint32_t n, void *dst);
+void unstage_words(const void *src, uint32_t n, float *words);
+
+// Asynchronous membership changes of a resident batch, for the served loop.
+// batch_join_async reserves the flock's ranges and queues the copy of its
+// staged records (`src`, pinned host memory the caller keeps until the next
+// batch_wait returns) on the copy stream; it may run while steps are in
+// flight. batch_leave_async (no steps in flight) flushes the flock's slots to
+// bird order and queues their copy into `dst` (pinned) on the copy stream,
+// then records `done`; *time gets the flock's clock. batch_apply applies
+// every queued join and leave with one splice and one scan (after the
+// uploads), without waiting on the host. The flock takes at most `steps`
+// steps (its budget).
+fh_status batch_join_async(fh_batch *b, const void *src, uint32_t n, float time, uint32_t steps, uint32_t *fid);
+fh_status batch_leave_async(fh_batch *b, uint32_t fid, void *dst, float *time, hipEvent_t done);
+fh_status batch_apply(fh_batch *b);
+// `steps` steps of every live flock, launched and not waited for; batch_wait
+// waits for them and reports kernel errors.
+fh_status batch_step_async(fh_batch *b, uint32_t steps);
+fh_status batch_wait(fh_batch *b);
 }

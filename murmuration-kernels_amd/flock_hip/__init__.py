@@ -485,6 +485,7 @@ class ServerStats(C.Structure):
         ("active", C.c_uint32),
         ("admit_s", C.c_double),
         ("retire_s", C.c_double),
+        ("gap_s", C.c_double),
     ]
 
     def as_dict(self) -> dict:
