@@ -295,45 +295,36 @@ __device__ __forceinline__ void acc_zero(Acc &a) {
     a.n_all = a.n_star = a.n_align = a.n_g = 0;
 }
 
-// Per-hit arithmetic. The lane walks (G 1-8) of sparse neighbourhoods round
-// every +, *, /, sqrt as the oracle does: there a walk is a chain of dependent
-// loads and the arithmetic is hidden under it (queueing the hits and summing
-// them after the walk, which executes the sums of 60% fewer hits, made decide
-// 3% slower). The candidate-split walks (G 16, 64) serve dense
-// neighbourhoods with hundreds to thousands of hits per bird, where the
-// arithmetic bounds decide: there (DENSE) the distance takes the hardware
-// square root and reciprocal (1 ulp each) and three products instead of a
-// correctly rounded square root and three divisions, the separation weight
-// the hardware exponential, and a neighbour's heading the hardware
-// reciprocal square root (10M clumped birds: 139.5 -> 97.3 ms per step).
-// Neighbour sets and counts are unchanged (they follow d^2 alone); forces
-// move by a few roundings.
+// Per-hit arithmetic. A walk's hits (candidates within 2r) take the hardware
+// square root, reciprocal, exponential and reciprocal square root (1 ulp
+// each) instead of a correctly rounded square root and divisions: the unit
+// offset is d times the reciprocal of the distance, the separation weight
+// the hardware exponential, a neighbour's heading its velocity times the
+// reciprocal square root of its length. Neighbour sets and counts are
+// unchanged (they follow d^2 alone, computed as the oracle does); forces move
+// by a few roundings, far inside the reference's bars. Measured on MI355X
+// against correctly rounded arithmetic: the 1M-bird step 6.8% faster, the
+// 256-flock batch 3.2%, the clumped 10M-bird step 30% (139.5 -> 97.3 ms),
+// where hundreds to thousands of hits per bird make the arithmetic the bound.
+// (Queueing the hits in LDS and summing them after the walk, which executes
+// the sums of 60% fewer hits with exact arithmetic, made decide 3-12%
+// slower: LDS traffic and occupancy cost more than the masked sums.)
 
 // walk 2 (model.rs:104-115) for a candidate with d^2 <= 4r^2: csum += d /
 // max(sqrt(d2), 1e-6); returns the unit offset and the distance.
-template <bool DENSE>
 __device__ __forceinline__ void accumulate_2r(Acc &a, f3 d, float d2, f3 &dhat, float &dist) {
-    if constexpr (DENSE) {
-        dist = __builtin_amdgcn_sqrtf(d2);
-        dhat = d * __builtin_amdgcn_rcpf(fmaxf(dist, 1e-6f));
-    } else {
-        dist = sqrtf(d2);
-        dhat = d / fmaxf(dist, 1e-6f);
-    }
+    dist = __builtin_amdgcn_sqrtf(d2);
+    dhat = d * __builtin_amdgcn_rcpf(fmaxf(dist, 1e-6f));
     a.csum = a.csum + dhat;
     ++a.n_g;
 }
 
 // walk 1 (model.rs:67-93) for a candidate within r; `velocity()` gives its
 // velocity (read only for an aligned neighbour).
-template <bool DENSE, class Velocity>
+template <class Velocity>
 __device__ __forceinline__ void accumulate_r(Acc &a, f3 f_i, f3 dhat, float dist, Velocity &&velocity) {
     ++a.n_all;
-    float g;
-    if constexpr (DENSE)
-        g = dist <= R_H ? 1.0f : __expf(-((dist - R_H) * (dist - R_H)) * (1.0f / SEP_DENOM));
-    else
-        g = dist <= R_H ? 1.0f : expf(-((dist - R_H) * (dist - R_H)) / SEP_DENOM);
+    const float g = dist <= R_H ? 1.0f : __expf(-((dist - R_H) * (dist - R_H)) * (1.0f / SEP_DENOM));
     a.sep = a.sep - dhat * g;
     if (dist > R_H) {
         a.coh = a.coh + dhat;
@@ -341,24 +332,20 @@ __device__ __forceinline__ void accumulate_r(Acc &a, f3 f_i, f3 dhat, float dist
     }
     if (dot3(f_i, dhat) >= BLIND_COS) {
         const float4 vj = velocity();
-        if constexpr (DENSE) {
-            const f3 v = mk3(vj.x, vj.y, vj.z);
-            const float l2 = dot3(v, v);
-            a.align = a.align + (l2 > 1e-12f ? v * __builtin_amdgcn_rsqf(l2) : mk3(1.f, 0.f, 0.f));
-        } else {
-            a.align = a.align + forward_of(mk3(vj.x, vj.y, vj.z));
-        }
+        const f3 v = mk3(vj.x, vj.y, vj.z);
+        const float l2 = dot3(v, v);
+        a.align = a.align + (l2 > 1e-12f ? v * __builtin_amdgcn_rsqf(l2) : mk3(1.f, 0.f, 0.f));  // forward_of
         ++a.n_align;
     }
 }
 
 // One candidate with offset d and d^2 <= 4r^2.
-template <bool DENSE = false, class Velocity>
+template <class Velocity>
 __device__ __forceinline__ void accumulate(Acc &a, f3 f_i, float r2, f3 d, float d2, Velocity &&velocity) {
     f3 dhat;
     float dist;
-    accumulate_2r<DENSE>(a, d, d2, dhat, dist);
-    if (d2 <= r2) accumulate_r<DENSE>(a, f_i, dhat, dist, velocity);
+    accumulate_2r(a, d, d2, dhat, dist);
+    if (d2 <= r2) accumulate_r(a, f_i, dhat, dist, velocity);
 }
 
 template <int G, int PF, int STRIDE>
@@ -373,7 +360,7 @@ __device__ __forceinline__ Acc walk_sums(const Pools &p, Cand cand, const Seg &s
     else
         walk_cells_split<G, PF, STRIDE>(w, sg, p, cand, n_slots, self, (uint32_t)first, use_occ,
                                         [&](f3 d, float d2, float4 vj) {
-                                            accumulate<true>(a, f_i, w.r2, d, d2, [&] { return vj; });
+                                            accumulate(a, f_i, w.r2, d, d2, [&] { return vj; });
                                         });
     return a;
 }
