@@ -1,662 +1,13 @@
-// decide for gfx950: each workgroup takes a run of sorted slots, compacts the
-// birds that are due into LDS, and gives every lane one due bird. Birds are
-// visited in slot order, so lanes of a wave that share a cell walk the same
-// buckets. The candidates come either from the sorted copies the scatter
-// writes (32-byte records: position, then velocity) or, in the resident
-// layout, from the state itself kept in slot order (positions and velocities
-// in their own arrays); a candidate is known by its slot.
-//
-// Mirrors crates/flock-ref/src/model.rs:53-145 over the grid of
-// cuda/flock.cu:190-290. The reference walks twice, once over the box of
-// radius r (sphere test r^2) and once over the box of 2r (sphere test 4r^2).
-// Here one walk covers the 2r box: every cell of the r walk is a cell of the
-// 2r walk (floor is monotone and r^2 <= 4r^2) and z-y-x order restricted to
-// the r box is the r walk's own order, so each accumulator receives the same
-// candidates in the same order as in the reference; walk-1 membership is the
-// oracle's own test d^2 <= r^2.
-#include <hip/hip_ext.h>
-
-#include <cstdio>
-#include <type_traits>
-
-#include "flight.h"
-#include "layout.h"
-#include "model.h"
+// decide's public launchers and the instances without a window (the kernels
+// are in decide_kernels.h; the instances with a window are built by
+// decide_win.hip). See decide_kernels.h for the walk.
+#include "decide_kernels.h"
 
 namespace fh {
 
-struct Decision {
-    f3 force;
-    float radius;
-    uint32_t n_all;
-    float centrality;
-};
-
-// Where a walk reads candidate t: position pos[STRIDE t], velocity
-// vel[STRIDE t] (STRIDE 2: interleaved sorted records; 1: resident arrays).
-// With a window, the positions and velocities of slots [wlo, wlo + wn) are
-// also held in LDS (winp, winv) and read from there.
-// The workgroup's dynamic LDS: the due-slot list, then (with a window) the
-// window's positions and velocities. Window reads index this array, so the
-// compiler sees LDS loads, never a generic pointer into LDS.
-extern __shared__ uint32_t decide_lds[];
-
-struct Cand {
-    const float4 *pos, *vel;
-    uint32_t wpos = 0, wvel = 0;  // word offsets of the window's positions and velocities in decide_lds
-    uint32_t wlo = 0, wn = 0;
-};
-
-template <int STRIDE>
-__device__ __forceinline__ float4 cand_pos(const Cand &c, uint32_t t) {
-    const uint32_t o = t - c.wlo;  // wraps for t < wlo
-    return o < c.wn ? reinterpret_cast<const float4 *>(decide_lds + c.wpos)[o] : c.pos[(size_t)STRIDE * t];
-}
-
-template <int STRIDE>
-__device__ __forceinline__ float4 cand_vel(const Cand &c, uint32_t t) {
-    const uint32_t o = t - c.wlo;
-    return o < c.wn ? reinterpret_cast<const float4 *>(decide_lds + c.wvel)[o] : c.vel[(size_t)STRIDE * t];
-}
-
-// Per-bird walk. `visit(slot)` sees each candidate slot that passes the
-// self and exact-cell checks and d^2 <= 4r^2, in walk order.
-struct Walk {
-    f3 p;
-    float r, r2, c2;
-    int32_t lo[3], hi[3];
-    bool ok;
-};
-
-__device__ __forceinline__ Walk make_walk(f3 p, float r, const Seg &sg) {
-    Walk w;
-    w.p = p;
-    w.r = r;
-    w.r2 = r * r;
-    const float rr = 2.0f * r;
-    w.c2 = rr * rr;
-    w.lo[0] = cell_coord(p.x - rr, sg.inv_cell);
-    w.hi[0] = cell_coord(p.x + rr, sg.inv_cell);
-    w.lo[1] = cell_coord(p.y - rr, sg.inv_cell);
-    w.hi[1] = cell_coord(p.y + rr, sg.inv_cell);
-    w.lo[2] = cell_coord(p.z - rr, sg.inv_cell);
-    w.hi[2] = cell_coord(p.z + rr, sg.inv_cell);
-    // A finite bird with r <= R_MAX spans at most 4 R_MAX / cell + 2 cells per
-    // axis; anything wider comes from a radius or position outside the model
-    // and is not walked (the bird then decides with no neighbours).
-    const int64_t max_extent = (int64_t)(4.0f * R_MAX * sg.inv_cell) + 3;
-    w.ok = true;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-        w.ok = w.ok && (int64_t)w.hi[a] - (int64_t)w.lo[a] <= max_extent && w.hi[a] >= w.lo[a] &&
-               w.hi[a] < 2147483647;
-    return w;
-}
-
-__device__ __forceinline__ bool bucket_occupied(const Pools &p, uint32_t b) {
-    return (p.occ[b >> 5] >> (b & 31u)) & 1u;
-}
-
-// Cells of the box are visited in z-y-x order (x fastest), as the nested
-// loops of cuda/flock.cu:209-211. A lane visits every G-th cell starting at
-// `first` (G = 1: every cell). Its cells are taken PF at a time: the bucket
-// ranges of all PF cells are loaded together, then the first candidate of
-// each non-empty bucket, so a lane has up to 2 * PF independent loads in
-// flight instead of a chain of dependent ones; within a bucket, candidates
-// are loaded AHEAD at a time. With `use_occ` the occupancy bits of the PF
-// buckets are read first (a small, cache-resident table) and empty buckets
-// never touch the offsets table. Within a lane the visit order is the walk
-// order.
-template <int G, int PF, int STRIDE, class Visit>
-__device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const Pools &p, Cand cand, uint32_t n_slots,
-                                           uint32_t self, int32_t first, bool use_occ, Visit &&visit) {
-    if (!w.ok) return;
-    const int32_t nx = w.hi[0] - w.lo[0] + 1, ny = w.hi[1] - w.lo[1] + 1, nz = w.hi[2] - w.lo[2] + 1;
-    const int32_t ncell = nx * ny * nz;
-    int32_t x = w.lo[0], y = w.lo[1], z = w.lo[2];
-    auto advance = [&](int32_t steps) {
-        for (int32_t s = 0; s < steps; ++s)
-            if (++x > w.hi[0]) {
-                x = w.lo[0];
-                if (++y > w.hi[1]) {
-                    y = w.lo[1];
-                    ++z;
-                }
-            }
-    };
-    advance(first);
-    for (int32_t base = first; base < ncell; base += G * PF) {
-        int32_t cx[PF], cy[PF], cz[PF];
-        uint32_t s0[PF], s1[PF], bk[PF];
-        bool want[PF];
-#pragma unroll
-        for (int k = 0; k < PF; ++k) {
-            cx[k] = x;
-            cy[k] = y;
-            cz[k] = z;
-            s0[k] = 0u;
-            s1[k] = 0u;
-            want[k] = base + k * G < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2);
-            bk[k] = bucket_of_cell(sg, x, y, z);
-#pragma unroll
-            for (int s = 0; s < G; ++s)
-                if (++x > w.hi[0]) {
-                    x = w.lo[0];
-                    if (++y > w.hi[1]) {
-                        y = w.lo[1];
-                        ++z;
-                    }
-                }
-        }
-        if (use_occ) {
-            uint32_t word[PF];
-#pragma unroll
-            for (int k = 0; k < PF; ++k) word[k] = want[k] ? p.occ[bk[k] >> 5] : 0u;
-#pragma unroll
-            for (int k = 0; k < PF; ++k) want[k] = (word[k] >> (bk[k] & 31u)) & 1u;
-        }
-#pragma unroll
-        for (int k = 0; k < PF; ++k)
-            if (want[k]) {
-                const uint2 r = bucket_range(p.offsets, bk[k]);
-                s0[k] = r.x;
-                s1[k] = r.y;
-            }
-        float4 head[PF];
-#pragma unroll
-        for (int k = 0; k < PF; ++k) {
-            s1[k] = min(s1[k], n_slots);
-            head[k] = s0[k] < s1[k] ? cand_pos<STRIDE>(cand, s0[k]) : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        // AHEAD candidates of a bucket are loaded before the first is tested, so
-        // one memory latency covers them (a walk is a chain of dependent
-        // loads; this is what bounds decide). Same candidates, same order.
-        constexpr int AHEAD = 4;
-#pragma unroll
-        for (int k = 0; k < PF; ++k) {
-            for (uint32_t t0 = s0[k]; t0 < s1[k]; t0 += AHEAD) {
-                float4 c[AHEAD];
-#pragma unroll
-                for (int u = 0; u < AHEAD; ++u) {
-                    const uint32_t t = t0 + (uint32_t)u;
-                    c[u] = u == 0 && t == s0[k] ? head[k]
-                           : (t < s1[k] ? cand_pos<STRIDE>(cand, t) : make_float4(0.f, 0.f, 0.f, 0.f));
-                }
-#pragma unroll
-                for (int u = 0; u < AHEAD; ++u) {
-                    const uint32_t t = t0 + (uint32_t)u;
-                    if (t >= s1[k] || t == self) continue;
-                    if (cell_coord(c[u].x, sg.inv_cell) != cx[k] || cell_coord(c[u].y, sg.inv_cell) != cy[k] ||
-                        cell_coord(c[u].z, sg.inv_cell) != cz[k])
-                        continue;
-                    const f3 d = mk3(c[u].x - w.p.x, c[u].y - w.p.y, c[u].z - w.p.z);
-                    const float d2 = dot3(d, d);
-                    if (d2 > w.c2) continue;
-                    visit(t, d, d2);
-                }
-            }
-        }
-    }
-}
-
-// Candidate-split walk for dense cells: the G lanes of a group walk the same
-// cells in walk order, and lane j takes candidates s0 + j, s0 + j + G, ... of
-// each bucket, so one bucket's candidates are read with coalesced loads and
-// spread over the group. PF cells' bucket ranges are loaded together (every
-// lane of the group loads the same words); a lane then loads UNROLL of its
-// candidates (position and velocity) before it tests the first, so one
-// memory latency covers UNROLL candidates. `visit(d, d2, velocity)`.
-template <int G, int PF, int STRIDE, class Visit>
-__device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, const Pools &p, Cand cand,
-                                                 uint32_t n_slots, uint32_t self, uint32_t lane, bool use_occ,
-                                                 Visit &&visit) {
-    if (!w.ok) return;
-    const int32_t nx = w.hi[0] - w.lo[0] + 1, ny = w.hi[1] - w.lo[1] + 1, nz = w.hi[2] - w.lo[2] + 1;
-    const int32_t ncell = nx * ny * nz;
-    int32_t x = w.lo[0], y = w.lo[1], z = w.lo[2];
-    for (int32_t base = 0; base < ncell; base += PF) {
-        int32_t cx[PF], cy[PF], cz[PF];
-        uint32_t s0[PF], s1[PF], bk[PF];
-        bool want[PF];
-#pragma unroll
-        for (int k = 0; k < PF; ++k) {
-            cx[k] = x;
-            cy[k] = y;
-            cz[k] = z;
-            s0[k] = 0u;
-            s1[k] = 0u;
-            want[k] = base + k < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2);
-            bk[k] = bucket_of_cell(sg, x, y, z);
-            if (++x > w.hi[0]) {
-                x = w.lo[0];
-                if (++y > w.hi[1]) {
-                    y = w.lo[1];
-                    ++z;
-                }
-            }
-        }
-        if (use_occ) {
-            uint32_t word[PF];
-#pragma unroll
-            for (int k = 0; k < PF; ++k) word[k] = want[k] ? p.occ[bk[k] >> 5] : 0u;
-#pragma unroll
-            for (int k = 0; k < PF; ++k) want[k] = (word[k] >> (bk[k] & 31u)) & 1u;
-        }
-#pragma unroll
-        for (int k = 0; k < PF; ++k)
-            if (want[k]) {
-                const uint2 r = bucket_range(p.offsets, bk[k]);
-                s0[k] = r.x;
-                s1[k] = min(r.y, n_slots);
-            }
-        constexpr int UNROLL = 2;
-#pragma unroll
-        for (int k = 0; k < PF; ++k) {
-            for (uint32_t t0 = s0[k] + lane; t0 < s1[k]; t0 += UNROLL * G) {
-                float4 c[UNROLL], v[UNROLL];
-#pragma unroll
-                for (int u = 0; u < UNROLL; ++u) {
-                    const uint32_t t = t0 + (uint32_t)u * G;
-                    c[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    v[u] = c[u];
-                    if (t < s1[k]) {
-                        c[u] = cand_pos<STRIDE>(cand, t);
-                        v[u] = cand_vel<STRIDE>(cand, t);
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < UNROLL; ++u) {
-                    const uint32_t t = t0 + (uint32_t)u * G;
-                    if (t >= s1[k] || t == self) continue;  // past the bucket's end, or the bird itself
-                    if (cell_coord(c[u].x, sg.inv_cell) != cx[k] || cell_coord(c[u].y, sg.inv_cell) != cy[k] ||
-                        cell_coord(c[u].z, sg.inv_cell) != cz[k])
-                        continue;
-                    const f3 d = mk3(c[u].x - w.p.x, c[u].y - w.p.y, c[u].z - w.p.z);
-                    const float d2 = dot3(d, d);
-                    if (d2 > w.c2) continue;
-                    visit(d, d2, v[u]);
-                }
-            }
-        }
-    }
-}
-
-// The running sums of one bird's walk (model.rs:62-115).
-struct Acc {
-    f3 sep, coh, align, csum;
-    uint32_t n_all, n_star, n_align, n_g;
-};
-
-__device__ __forceinline__ void acc_zero(Acc &a) {
-    a.sep = mk3(0.f, 0.f, 0.f);
-    a.coh = a.sep;
-    a.align = a.sep;
-    a.csum = a.sep;
-    a.n_all = a.n_star = a.n_align = a.n_g = 0;
-}
-
-// Per-hit arithmetic. A walk's hits (candidates within 2r) take the hardware
-// square root, reciprocal, exponential and reciprocal square root (1 ulp
-// each) instead of a correctly rounded square root and divisions: the unit
-// offset is d times the reciprocal of the distance, the separation weight
-// the hardware exponential, a neighbour's heading its velocity times the
-// reciprocal square root of its length. Neighbour sets and counts are
-// unchanged (they follow d^2 alone, computed as the oracle does); forces move
-// by a few roundings, far inside the reference's bars. Measured on MI355X
-// against correctly rounded arithmetic: the 1M-bird step 6.8% faster, the
-// 256-flock batch 3.2%, the clumped 10M-bird step 30% (139.5 -> 97.3 ms),
-// where hundreds to thousands of hits per bird make the arithmetic the bound.
-// (Queueing the hits in LDS and summing them after the walk, which executes
-// the sums of 60% fewer hits with exact arithmetic, made decide 3-12%
-// slower: LDS traffic and occupancy cost more than the masked sums.)
-
-// walk 2 (model.rs:104-115) for a candidate with d^2 <= 4r^2: csum += d /
-// max(sqrt(d2), 1e-6); returns the unit offset and the distance.
-__device__ __forceinline__ void accumulate_2r(Acc &a, f3 d, float d2, f3 &dhat, float &dist) {
-    dist = __builtin_amdgcn_sqrtf(d2);
-    dhat = d * __builtin_amdgcn_rcpf(fmaxf(dist, 1e-6f));
-    a.csum = a.csum + dhat;
-    ++a.n_g;
-}
-
-// walk 1 (model.rs:67-93) for a candidate within r; `velocity()` gives its
-// velocity (read only for an aligned neighbour).
-template <class Velocity>
-__device__ __forceinline__ void accumulate_r(Acc &a, f3 f_i, f3 dhat, float dist, Velocity &&velocity) {
-    ++a.n_all;
-    const float g = dist <= R_H ? 1.0f : __expf(-((dist - R_H) * (dist - R_H)) * (1.0f / SEP_DENOM));
-    a.sep = a.sep - dhat * g;
-    if (dist > R_H) {
-        a.coh = a.coh + dhat;
-        ++a.n_star;
-    }
-    if (dot3(f_i, dhat) >= BLIND_COS) {
-        const float4 vj = velocity();
-        const f3 v = mk3(vj.x, vj.y, vj.z);
-        const float l2 = dot3(v, v);
-        a.align = a.align + (l2 > 1e-12f ? v * __builtin_amdgcn_rsqf(l2) : mk3(1.f, 0.f, 0.f));  // forward_of
-        ++a.n_align;
-    }
-}
-
-// One candidate with offset d and d^2 <= 4r^2.
-template <class Velocity>
-__device__ __forceinline__ void accumulate(Acc &a, f3 f_i, float r2, f3 d, float d2, Velocity &&velocity) {
-    f3 dhat;
-    float dist;
-    accumulate_2r(a, d, d2, dhat, dist);
-    if (d2 <= r2) accumulate_r(a, f_i, dhat, dist, velocity);
-}
-
-template <int G, int PF, int STRIDE>
-__device__ __forceinline__ Acc walk_sums(const Pools &p, Cand cand, const Seg &sg, uint32_t n_slots, uint32_t self,
-                                         const Walk &w, f3 f_i, int32_t first, bool use_occ) {
-    Acc a;
-    acc_zero(a);
-    if constexpr (G <= 8)
-        walk_cells<G, PF, STRIDE>(w, sg, p, cand, n_slots, self, first, use_occ, [&](uint32_t t, f3 d, float d2) {
-            accumulate(a, f_i, w.r2, d, d2, [&] { return cand_vel<STRIDE>(cand, t); });
-        });
-    else
-        walk_cells_split<G, PF, STRIDE>(w, sg, p, cand, n_slots, self, (uint32_t)first, use_occ,
-                                        [&](f3 d, float d2, float4 vj) {
-                                            accumulate(a, f_i, w.r2, d, d2, [&] { return vj; });
-                                        });
-    return a;
-}
-
-__device__ __forceinline__ f3 shfl_xor3(f3 v, int m) {
-    return mk3(__shfl_xor(v.x, m), __shfl_xor(v.y, m), __shfl_xor(v.z, m));
-}
-
-// Sums the partial walks of the G lanes of a group (butterfly; every lane
-// ends with bit-identical totals because float addition is commutative).
-template <int G>
-__device__ __forceinline__ void reduce_group(Acc &a) {
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) {
-        a.sep = a.sep + shfl_xor3(a.sep, m);
-        a.coh = a.coh + shfl_xor3(a.coh, m);
-        a.align = a.align + shfl_xor3(a.align, m);
-        a.csum = a.csum + shfl_xor3(a.csum, m);
-        a.n_all += __shfl_xor(a.n_all, m);
-        a.n_star += __shfl_xor(a.n_star, m);
-        a.n_align += __shfl_xor(a.n_align, m);
-        a.n_g += __shfl_xor(a.n_g, m);
-    }
-}
-
-// combine (model.rs:95-137, cuda/flock.cu:242-283)
-__device__ __forceinline__ Decision finish(const Acc &a, f3 pos, f3 vel, f3 f_i, float r) {
-    f3 force = mk3(0.f, 0.f, 0.f);
-    if (a.n_all > 0) force = force + a.sep * (W_SEP / (float)a.n_all);
-    const float centrality = a.n_g > 0 ? sqrtf(dot3(a.csum, a.csum)) / (float)a.n_g : 0.0f;
-    if (a.n_star > 0) force = force + normalize_or_zero(a.coh) * (W_COH * centrality);
-    const f3 al = a.n_align > 0 ? a.align / (float)a.n_align - f_i : mk3(0.f, 0.f, 0.f);
-    force = force + al * W_ALIGN;
-    force = force + mk3(0.f, 0.f, 1.f) * (W_ROOST_V * clampf_((Z0 - pos.z) / H_ROOST_V, -1.0f, 1.0f));
-    const float speed = sqrtf(dot3(vel, vel));
-    force = force + f_i * ((MASS / TAU) * (V0 - speed));
-    Decision dcs;
-    dcs.force = force;
-    dcs.radius = radius_step(r, a.n_all);
-    dcs.n_all = a.n_all;
-    dcs.centrality = centrality;
-    return dcs;
-}
-
-// Cells of the union box the tiled path may walk, and how much larger than
-// the largest single box the union may be, before the per-lane walk is used.
-constexpr int64_t TILED_MAX_CELLS = 4096;
-constexpr int64_t TILED_MAX_RATIO = 8;
-constexpr uint32_t NO_BIRD = 0xffffffffu;
-
-// BLOCK lanes per workgroup, G lanes per due bird. With G = 1 each bird's sums
-// follow the walk order exactly (the order of cuda/flock.cu:209-238); with
-// G = 2..8 lane j sums cells j, j+G, ...; with G = 16 or 64 lane j sums
-// candidates j, j+G, ... of every bucket; the G partial sums are then added.
-//
-// Tiled path (G = 1): when the due birds a workgroup holds at once share
-// their neighbourhood (a dense cell: they sit together in slot order), the
-// workgroup walks the union of their boxes in z-y-x order. For each cell some
-// lane needs, the bucket's candidates are staged in LDS BLOCK at a time with
-// coalesced loads and every lane that needs the cell reads them from LDS
-// (broadcast). A lane sees the same (cell, slot) sequence as in its own walk,
-// so its sums are bit-identical to the per-lane path.
-//
-// Persistent form (WALK_PERSIST): a grid sized to the machine rather than to
-// the flock; each workgroup takes chunks from *p.work until they run out, so
-// a workgroup that drew a sparse chunk moves on at once and the tail of the
-// launch is the last chunk's walk, not the slowest workgroup's share.
-// Waves per SIMD the compiler keeps room for (1: no floor). With G >= 4 the
-// walk fits 95 VGPRs without spilling, so 5 waves per SIMD instead of the 4
-// that its natural 105-115 VGPRs allow (measured: -2% to -9% decide time).
-// G = 2 is held to 128 VGPRs (4 waves): it fits without the fused flight, and
-// with it (136 VGPRs, 3 waves, left alone) one spilled register outside the
-// walk buys the fourth wave (fused step at 1M birds: 0.127 -> 0.099 ms).
-// The split walks (G >= 16) load 2 candidates ahead instead of 4 and run 6
-// waves in 80 VGPRs (dense clumps at 10M birds: -14% per step; reference
-// density at 1M: -3% decide). G = 1 would spill inside the walk.
-constexpr int decide_min_waves(int g) { return g >= 16 ? 6 : g >= 4 ? 5 : g == 2 ? 4 : 1; }
-
-template <int BLOCK, int G, bool RES, bool FLY>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_min_waves(G), 8))) void k_decide(
-    Geometry g, Pools p, Records rec, float time, uint32_t chunk, int walk_flags, uint32_t *due_clear,
-    uint32_t *work_clear, ResWork w, Records flown) {
-    const bool allow_tiled = (walk_flags & WALK_TILED) != 0, use_occ = (walk_flags & WALK_OCC) != 0;
-    const bool persist = (walk_flags & WALK_PERSIST) != 0;
-    // Cells whose bucket ranges a lane loads at once. The split walks (G >= 16)
-    // take one cell at a time: their buckets are long, and the smaller code
-    // (the candidate loop is unrolled per cell) runs faster. A lane of G = 4-8
-    // walks one or two cells of a typical box and loads its candidates 4
-    // ahead, so one cell at a time keeps it within 95 VGPRs (5 waves per
-    // SIMD) without spilling. Fewer registers for the prefetch mean more
-    // waves (G = 2 with 4 cells instead of 8: -20% decide time at 1M birds).
-    constexpr int PF = G >= 4 ? 1 : 4;
-    constexpr int STRIDE = RES ? 1 : 2;
-    Cand cand = RES ? Cand{rec.p4, rec.v4} : Cand{p.sbird, p.sbird + 1};
-    const bool use_win = (walk_flags & WALK_WINDOW) != 0;
-    // decide_lds: [chunk] due slots, then with a window float4 positions and
-    // velocities [window_slots(chunk)] each
-    uint32_t *const due_slots = decide_lds;
-    float4 *const winp = reinterpret_cast<float4 *>(decide_lds + chunk);
-    float4 *const winv = winp + window_slots(chunk);
-    __shared__ uint32_t n_due;
-    __shared__ int32_t u_lo[3], u_hi[3];
-    __shared__ uint32_t u_seg_lo, u_seg_hi, u_vol, u_any;
-    __shared__ float4 tile[G == 1 ? BLOCK : 1];  // the tiled path (G = 1 only)
-    __shared__ uint32_t cur_chunk;
-    if (RES && blockIdx.x == 0 && threadIdx.x == 0) {
-        if (due_clear) *due_clear = 0;
-        if (work_clear) *work_clear = 0;
-    }
-    const uint32_t n_chunks = (g.n_slots + chunk - 1) / chunk;
-    for (uint32_t round = 0;; ++round) {  // one chunk per trip; every lane leaves together
-    if (threadIdx.x == 0) {
-        n_due = 0;
-        uint32_t c = round ? n_chunks : blockIdx.x;
-        if (persist) {
-            c = atomicAdd(p.work, 1u);
-        } else if (!round && (walk_flags & WALK_XCD)) {
-            // The workgroups of XCD x (blockIdx % XCDS == x) take the x-th
-            // contiguous run of chunks in order, so the halo cells two
-            // neighbouring chunks share are read through one L2.
-            const uint32_t per = n_chunks / XCDS, rem = n_chunks % XCDS;
-            const uint32_t x = blockIdx.x % XCDS, i = blockIdx.x / XCDS;
-            c = x * per + min(x, rem) + i;
-        }
-        cur_chunk = c;
-    }
-    __syncthreads();
-    const uint32_t c_idx = cur_chunk;
-    if (c_idx >= n_chunks) break;
-    const uint32_t start = c_idx * chunk;
-    const uint32_t end = min(start + chunk, g.n_slots);
-    if (use_win) {
-        // Stage the positions and velocities of the chunk and a halo on each
-        // side with coalesced loads; with the spatial hash most candidates
-        // of the chunk's due birds lie in this window of slots.
-        const uint32_t halo = window_halo(chunk);
-        const uint32_t wlo = start > halo ? start - halo : 0u;
-        const uint32_t whi = min(min(end + halo, g.n_slots), wlo + window_slots(chunk));
-        for (uint32_t t = wlo + threadIdx.x; t < whi; t += BLOCK) {
-            winp[t - wlo] = cand.pos[(size_t)STRIDE * t];
-            winv[t - wlo] = cand.vel[(size_t)STRIDE * t];
-        }
-        cand.wpos = chunk;
-        cand.wvel = chunk + 4u * window_slots(chunk);
-        cand.wlo = wlo;
-        cand.wn = whi - wlo;
-        __syncthreads();
-    }
-    // Sorted copies: time is relative (0) and snext holds next - time. Resident
-    // records: this step's clock is the bird's flock clock + DT (committed by
-    // the pass that follows) and p4.w holds next_decision.
-    for (uint32_t s = start + threadIdx.x; s < end; s += BLOCK) {
-        bool due;
-        if (RES) {
-            const Seg &sg = g.segs[slot_segment(g, rec, s)];
-            due = sg.left && sg.time + DT >= cand_pos<STRIDE>(cand, s).w;  // frozen flocks never decide
-        } else {
-            due = time >= p.snext[s];
-        }
-        if (due) due_slots[atomicAdd(&n_due, 1u)] = s;
-    }
-    __syncthreads();
-    const uint32_t nd = n_due;
-    if (threadIdx.x == 0 && nd) atomicAdd(p.due_count, nd);
-    constexpr uint32_t groups = BLOCK / G;
-    const uint32_t lane_g = threadIdx.x % G;
-    for (uint32_t base = 0; base < nd; base += groups) {  // same trip count for every lane
-        const uint32_t k = base + threadIdx.x / G;
-        bool active = k < nd;
-        uint32_t me = 0, si = 0, slot = 0;
-        f3 pos = mk3(0.f, 0.f, 0.f), vel = pos, f_i = pos;
-        float r = 0.0f;
-        Acc a;
-        acc_zero(a);
-        Walk w;
-        w.ok = false;
-        if (active) {
-            slot = due_slots[k];
-            const float4 q = cand_pos<STRIDE>(cand, slot);
-            const float4 v = cand_vel<STRIDE>(cand, slot);  // sorted copies: w is the bird's radius
-            me = RES ? rec.id[slot] : __float_as_uint(q.w);
-            if (me >= g.n_birds) {
-                if (lane_g == 0) atomicOr(p.err, ERR_SLOT);
-                active = false;
-            } else {
-                si = g.tile_seg ? g.tile_seg[me / TILE] : 0u;
-                pos = mk3(q.x, q.y, q.z);
-                vel = mk3(v.x, v.y, v.z);
-                f_i = forward_of(vel);
-                r = RES ? rec.s4[slot].w : v.w;
-                w = make_walk(pos, r, g.segs[si]);
-            }
-        }
-        bool tiled = false;
-        if (G == 1) {
-            if (threadIdx.x == 0) {
-                u_lo[0] = u_lo[1] = u_lo[2] = 2147483647;
-                u_hi[0] = u_hi[1] = u_hi[2] = -2147483647 - 1;
-                u_seg_lo = 0xffffffffu;
-                u_seg_hi = 0u;
-                u_vol = 0u;
-                u_any = 0u;
-            }
-            __syncthreads();
-            if (active && w.ok) {
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) {
-                    atomicMin(&u_lo[ax], w.lo[ax]);
-                    atomicMax(&u_hi[ax], w.hi[ax]);
-                }
-                atomicMin(&u_seg_lo, si);
-                atomicMax(&u_seg_hi, si);
-                const int64_t vol = (int64_t)(w.hi[0] - w.lo[0] + 1) * (w.hi[1] - w.lo[1] + 1) *
-                                    (w.hi[2] - w.lo[2] + 1);
-                atomicMax(&u_vol, (uint32_t)min<int64_t>(vol, 0x7fffffff));
-                u_any = 1u;
-            }
-            __syncthreads();
-            const int64_t ex = (int64_t)u_hi[0] - u_lo[0] + 1, ey = (int64_t)u_hi[1] - u_lo[1] + 1,
-                          ez = (int64_t)u_hi[2] - u_lo[2] + 1;
-            tiled = allow_tiled && u_any && u_seg_lo == u_seg_hi && ex <= TILED_MAX_CELLS && ey <= TILED_MAX_CELLS &&
-                    ez <= TILED_MAX_CELLS && ex * ey * ez <= TILED_MAX_CELLS &&
-                    ex * ey * ez <= TILED_MAX_RATIO * (int64_t)u_vol;
-        }
-        if (tiled) {
-            const Seg &sg = g.segs[u_seg_lo];
-            const int32_t lx = u_lo[0], ly = u_lo[1], lz = u_lo[2];
-            const int32_t hx = u_hi[0], hy = u_hi[1], hz = u_hi[2];
-            for (int32_t cz = lz; cz <= hz; ++cz)
-                for (int32_t cy = ly; cy <= hy; ++cy)
-                    for (int32_t cx = lx; cx <= hx; ++cx) {
-                        const bool need = active && w.ok && cx >= w.lo[0] && cx <= w.hi[0] && cy >= w.lo[1] &&
-                                          cy <= w.hi[1] && cz >= w.lo[2] && cz <= w.hi[2] &&
-                                          cell_meets_sphere(cx, cy, cz, sg.cell, w.p, w.c2);
-                        if (!__syncthreads_or(need)) continue;
-                        const uint32_t b = bucket_of_cell(sg, cx, cy, cz);
-                        if (use_occ && !bucket_occupied(p, b)) continue;  // same b for every lane
-                        const uint2 rg = bucket_range(p.offsets, b);
-                        const uint32_t s0 = rg.x;
-                        const uint32_t s1 = min(rg.y, g.n_slots);
-                        for (uint32_t t0 = s0; t0 < s1; t0 += BLOCK) {
-                            const uint32_t t = t0 + threadIdx.x;
-                            if (t < s1) {
-                                const float4 c = cand_pos<STRIDE>(cand, t);
-                                const bool in_cell = cell_coord(c.x, sg.inv_cell) == cx &&
-                                                     cell_coord(c.y, sg.inv_cell) == cy &&
-                                                     cell_coord(c.z, sg.inv_cell) == cz;
-                                tile[threadIdx.x] = make_float4(c.x, c.y, c.z, __uint_as_float(in_cell ? t : NO_BIRD));
-                            }
-                            __syncthreads();
-                            const uint32_t cnt = min((uint32_t)BLOCK, s1 - t0);
-                            if (need) {
-                                for (uint32_t j = 0; j < cnt; ++j) {
-                                    const float4 c = tile[j];
-                                    const uint32_t ts = __float_as_uint(c.w);
-                                    if (ts == NO_BIRD || ts == slot) continue;
-                                    const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
-                                    const float d2 = dot3(d, d);
-                                    if (d2 > w.c2) continue;
-                                    accumulate(a, f_i, w.r2, d, d2, [&] { return cand_vel<STRIDE>(cand, ts); });
-                                }
-                            }
-                            __syncthreads();
-                        }
-                    }
-        } else if (active) {
-            a = walk_sums<G, PF, STRIDE>(p, cand, g.segs[si], g.n_slots, slot, w, f_i, (int32_t)lane_g, use_occ);
-        }
-        if (G == 1) __syncthreads();  // the union box is reset by the next iteration
-        if (G > 1) reduce_group<G>(a);
-        if (active && lane_g == 0) {
-            const Decision d = finish(a, pos, vel, f_i, r);
-            if (RES) {
-                rec.s4[slot] = make_float4(d.force.x, d.force.y, d.force.z, d.radius);
-                rec.p4[slot].w = rec.p4[slot].w + DU;
-            } else {
-                p.steer4[me] = make_float4(d.force.x, d.force.y, d.force.z, d.radius);
-                p.pos4[me].w = p.pos4[me].w + DU;
-            }
-            p.nb[me] = d.n_all;
-            p.cen[me] = d.centrality;
-        }
-    }
-    if constexpr (RES && FLY) {
-        // Fused flight: the chunk's decisions are in rec (visible to the whole
-        // workgroup after the barrier); every bird of the chunk flies into
-        // `flown`, so the walks of other workgroups keep reading the positions
-        // of this step from rec. A separate instance: the flight's code would
-        // otherwise cost the G = 1-2 walks a wave per SIMD.
-        __syncthreads();
-        for (uint32_t b0 = start; b0 < end; b0 += 2u * BLOCK)
-            resident_flight2(g, rec, flown, w, b0 + threadIdx.x, b0 + BLOCK + threadIdx.x, end, start);
-    }
-    __syncthreads();  // due_slots and n_due are reused by the next chunk
-    }
-}
+template hipError_t decide_dispatch<false>(const Geometry &, const Pools &, const Records &, float, uint32_t, uint32_t,
+                                           uint32_t, int, uint32_t *, uint32_t *, const ResWork &, const Records &,
+                                           const DecideTiming &, hipStream_t, bool);
 
 // Debug walk for one bird (sorted copies): the ids the grid walk finds within
 // r (walk 1) or 2r (walk 2), in walk order. One lane; used by the parity
@@ -673,7 +24,7 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
     for (uint32_t t = own.x; t < min(own.y, g.n_slots); ++t)
         if (__float_as_uint(p.sbird[2u * t].w) == bird) self = t;
     uint32_t c = 0;
-    walk_cells<1, 8, 2>(w, sg, p, Cand{p.sbird, p.sbird + 1}, g.n_slots, self, 0, true,
+    walk_cells<1, 8, 2, false>(w, sg, p, Cand{p.sbird, p.sbird + 1}, g.n_slots, self, 0, true,
                         [&](uint32_t t, f3, float d2) {
                             if (walk == 1 && d2 > w.r2) return;
                             if (c < cap) ids[c] = __float_as_uint(p.sbird[2u * t].w);
@@ -682,112 +33,25 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
     *count = c;
 }
 
-// Workgroups of one persistent decide launch: as many as the chip holds at
-// once (occupancy of this instance at this LDS size, times the CUs), and no
-// more than there are chunks. The occupancy is a property of the compiled
-// kernel; it is looked up once per instance and chunk size.
-template <int BLOCK, int G, bool RES, bool FLY>
-static uint32_t persistent_blocks(uint32_t chunk, uint32_t n_chunks, size_t lds) {
-    // by log2(chunk / BLOCK) (1..16 birds per thread), without and with a window
-    static int per_cu[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
-    const uint32_t bpt = chunk / BLOCK;
-    const int idx = bpt <= 1 ? 0 : bpt <= 2 ? 1 : bpt <= 4 ? 2 : bpt <= 8 ? 3 : 4;
-    int *slot = &per_cu[lds > chunk * sizeof(uint32_t) ? 1 : 0][idx];
-    int occ = __atomic_load_n(slot, __ATOMIC_RELAXED);
-    if (occ <= 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_decide<BLOCK, G, RES, FLY>, BLOCK, lds) != hipSuccess ||
-            occ <= 0)
-            occ = 1;
-        __atomic_store_n(slot, occ, __ATOMIC_RELAXED);
-    }
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-        cus = 256;
-    return min(n_chunks, (uint32_t)occ * (uint32_t)cus);
-}
-
-// With timing events, the launch itself records the kernel's start and end
-// (hipExtLaunchKernelGGL), so the measured interval is the kernel alone.
-template <int BLOCK, int G, bool RES, bool FLY>
-static void launch_decide_block(const Geometry &g, const Pools &p, const Records &rec, float time, uint32_t chunk,
-                                int walk_flags, uint32_t *due_clear, uint32_t *work_clear, const ResWork &w,
-                                const Records &flown, const DecideTiming &tm, hipStream_t s) {
-    const uint32_t n_chunks = (g.n_slots + chunk - 1) / chunk;
-    const size_t lds = decide_lds_bytes(chunk, walk_flags);
-    const uint32_t blocks =
-        (walk_flags & WALK_PERSIST) ? persistent_blocks<BLOCK, G, RES, FLY>(chunk, n_chunks, lds) : n_chunks;
-    if (tm.start)
-        hipExtLaunchKernelGGL((k_decide<BLOCK, G, RES, FLY>), dim3(blocks), dim3(BLOCK), lds, s, tm.start, tm.stop, 0u, g,
-                              p, rec, time, chunk, walk_flags, due_clear, work_clear, w, flown);
-    else
-        hipLaunchKernelGGL((k_decide<BLOCK, G, RES, FLY>), dim3(blocks), dim3(BLOCK), lds, s, g, p, rec, time, chunk,
-                           walk_flags, due_clear, work_clear, w, flown);
-}
-
-template <int BLOCK, bool RES, bool FLY>
-static hipError_t launch_decide_group(const Geometry &g, const Pools &p, const Records &rec, float time,
-                                      uint32_t chunk, uint32_t group, int walk_flags, uint32_t *due_clear,
-                                      uint32_t *work_clear, const ResWork &w, const Records &flown,
-                                      const DecideTiming &tm, hipStream_t s) {
-#define FH_DECIDE_G(GV)                                                                                    \
-    case GV:                                                                                               \
-        launch_decide_block<BLOCK, GV, RES, FLY>(g, p, rec, time, chunk, walk_flags, due_clear, work_clear, w, flown, tm, \
-                                            s);                                                                    \
-        break;
-    switch (group) {
-        FH_DECIDE_G(1)
-        FH_DECIDE_G(2)
-        FH_DECIDE_G(4)
-        FH_DECIDE_G(8)
-        FH_DECIDE_G(16)
-        FH_DECIDE_G(64)
-        default: return hipErrorInvalidValue;
-    }
-#undef FH_DECIDE_G
-    return hipSuccess;
-}
-
-template <bool RES, bool FLY>
-static hipError_t launch_decide_any(const Geometry &g, const Pools &p, const Records &rec, float time,
-                                    uint32_t block, uint32_t birds_per_thread, uint32_t group, int walk_flags,
-                                    uint32_t *due_clear, uint32_t *work_clear, const ResWork &w,
-                                    const Records &flown, const DecideTiming &tm, hipStream_t s) {
-    if (g.n_slots == 0) return hipSuccess;
-    const uint32_t chunk = block * birds_per_thread;
-    auto by_block = [&](auto block_c) {
-        constexpr int B = decltype(block_c)::value;
-        return launch_decide_group<B, RES, FLY>(g, p, rec, time, chunk, group, walk_flags, due_clear, work_clear, w, flown,
-                                           tm, s);
-    };
-    hipError_t e;
-    switch (block) {
-        case 64: e = by_block(std::integral_constant<int, 64>{}); break;
-        case 128: e = by_block(std::integral_constant<int, 128>{}); break;
-        case 256: e = by_block(std::integral_constant<int, 256>{}); break;
-        case 512: e = by_block(std::integral_constant<int, 512>{}); break;
-        default: return hipErrorInvalidValue;
-    }
-    if (e != hipSuccess) return e;
-    return hipGetLastError();
-}
-
 hipError_t launch_decide(const Geometry &g, const Pools &p, float time, uint32_t block,
                          uint32_t birds_per_thread, uint32_t group, int walk_flags, hipStream_t s,
                          const DecideTiming &tm) {
-    return launch_decide_any<false, false>(g, p, Records{}, time, block, birds_per_thread, group,
-                                    walk_flags & ~WALK_FLIGHT, nullptr, nullptr, ResWork{}, Records{}, tm, s);
+    if (walk_flags & WALK_WINDOW)
+        return decide_dispatch<true>(g, p, Records{}, time, block, birds_per_thread, group, walk_flags, nullptr,
+                                     nullptr, ResWork{}, Records{}, tm, s, false);
+    return decide_dispatch<false>(g, p, Records{}, time, block, birds_per_thread, group, walk_flags, nullptr, nullptr,
+                                  ResWork{}, Records{}, tm, s, false);
 }
 
 hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Records &rec, uint32_t block,
                                   uint32_t birds_per_thread, uint32_t group, int walk_flags, uint32_t *due_clear,
                                   uint32_t *work_clear, const ResWork &w, const Records &flown, hipStream_t s,
                                   const DecideTiming &tm) {
-    if (walk_flags & WALK_FLIGHT)
-        return launch_decide_any<true, true>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear,
-                                             work_clear, w, flown, tm, s);
-    return launch_decide_any<true, false>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear,
-                                          work_clear, w, flown, tm, s);
+    if (walk_flags & WALK_WINDOW)
+        return decide_dispatch<true>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear, work_clear,
+                                     w, flown, tm, s, true);
+    return decide_dispatch<false>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear, work_clear, w,
+                                  flown, tm, s, true);
 }
 
 hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, int walk, uint32_t *ids,
@@ -799,12 +63,12 @@ hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, i
 template <int BLOCK, bool RES, bool FLY>
 static const void *decide_fn_group(uint32_t group) {
     switch (group) {
-        case 1: return reinterpret_cast<const void *>(k_decide<BLOCK, 1, RES, FLY>);
-        case 2: return reinterpret_cast<const void *>(k_decide<BLOCK, 2, RES, FLY>);
-        case 4: return reinterpret_cast<const void *>(k_decide<BLOCK, 4, RES, FLY>);
-        case 8: return reinterpret_cast<const void *>(k_decide<BLOCK, 8, RES, FLY>);
-        case 16: return reinterpret_cast<const void *>(k_decide<BLOCK, 16, RES, FLY>);
-        case 64: return reinterpret_cast<const void *>(k_decide<BLOCK, 64, RES, FLY>);
+        case 1: return reinterpret_cast<const void *>(k_decide<BLOCK, 1, RES, FLY, false>);
+        case 2: return reinterpret_cast<const void *>(k_decide<BLOCK, 2, RES, FLY, false>);
+        case 4: return reinterpret_cast<const void *>(k_decide<BLOCK, 4, RES, FLY, false>);
+        case 8: return reinterpret_cast<const void *>(k_decide<BLOCK, 8, RES, FLY, false>);
+        case 16: return reinterpret_cast<const void *>(k_decide<BLOCK, 16, RES, FLY, false>);
+        case 64: return reinterpret_cast<const void *>(k_decide<BLOCK, 64, RES, FLY, false>);
         default: return nullptr;
     }
 }
@@ -826,7 +90,7 @@ static const void *decide_fn(uint32_t block, uint32_t group) {
 // with the fused flight (f).
 hipError_t decide_kernel_attributes(const char *name, hipFuncAttributes *out) {
     if (!__builtin_strcmp(name, "decide"))
-        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_decide<256, 1, false, false>));
+        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_decide<256, 1, false, false, false>));
     unsigned block = 0, group = 0;
     char kind = 0;
     if (std::sscanf(name, "decide:%u:%u:%c", &block, &group, &kind) == 3) {
