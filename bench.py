@@ -195,6 +195,129 @@ def plain_point(args) -> tuple[dict, int]:
     return tuned["params"], tuned["flags"]
 
 
+def plain_tunables(args, words, cached_only: bool = False) -> tuple[dict, dict | None, int] | None:
+    """Tunables and flags of the plain step on `words`: those of --params, the
+    library defaults (--no-tune), or the autotuner's (its cached point when
+    the cache holds this workload). Returns (params, autotune result or None,
+    flags); None when `cached_only` and the cache lacks the workload."""
+    prm, tuned, flags = {}, None, args.flags
+    if args.params:
+        for item in args.params.split(","):
+            k, v = item.split("=")
+            if k == "flags":
+                flags = int(v)
+            else:
+                prm[k] = float(v) if k == "cell" else int(v)
+    elif not args.no_tune:
+        from flock_hip import autotune
+
+        # trials time the same window as the timed run (radii grow each decision)
+        log(f"plain leg: autotune at {words.shape[0]} birds")
+        tuned = autotune.tune([words], "plain", flags=args.flags, warmup=args.warmup, steps=args.steps, progress=log,
+                              cached_only=cached_only,
+                              **tune_cache(args, f"plain-cube-{words.shape[0]}-seed{args.seed}"))
+        if tuned is None:
+            return None
+        prm, flags = tuned["params"], tuned["flags"]
+    return prm, tuned, flags
+
+
+def traffic_probe(args) -> int:
+    """The program that live_traffic runs under rocprofv3 --pmc: a 256 MiB
+    device-to-device copy (the counters' calibration), then the plain step's
+    decide as the timed run has it (same workload and tunables, pass by pass
+    so each launch is its own dispatch) for warm-up + timed steps. Prints the
+    tunables it ran as one JSON line."""
+    import torch
+
+    import flock_hip as fh
+
+    x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+    words = fh.gen_cube(args.birds, fh.cube_side_for(args.birds), args.seed)
+    point = plain_tunables(args, words, cached_only=True)
+    if point is None:  # no search under the profiler: the parent falls back
+        print("traffic probe: the autotuner cache lacks this workload", file=sys.stderr)
+        return 3
+    prm, _, flags = point
+    with fh.Flock.from_words(words, flags=flags & ~fh.FLAG_GRAPH, **prm) as f:
+        f.step(args.warmup)
+        f.step(args.steps)
+        f.sync()
+    print(json.dumps({"params": prm, "flags": flags, "copy_bytes": 256 << 20}), flush=True)
+    return 0
+
+
+def live_traffic(args) -> dict | None:
+    """HBM bytes per decide launch of the plain step, measured in this bench
+    run: two rocprofv3 --pmc runs of `bench.py --traffic-probe` (FETCH_SIZE,
+    then WRITE_SIZE, in KiB per dispatch), averaged over the timed steps'
+    decide dispatches and each scaled by what the counter reads for the
+    probe's 256 MiB copy (FETCH_SIZE reads half the bytes a copy fetches,
+    WRITE_SIZE all it writes). Runs before this process touches the GPU (the
+    probe is a child process). None when rocprofv3 is missing or a run fails."""
+    import csv
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    probe = [sys.executable, os.path.abspath(__file__), "--traffic-probe", "--birds", str(args.birds),
+             "--seed", str(args.seed), "--warmup", str(args.warmup), "--steps", str(args.steps),
+             "--flags", str(args.flags), "--tune-cache", args.tune_cache]
+    if args.params:
+        probe += ["--params", args.params]
+    if args.no_tune:
+        probe.append("--no-tune")
+    out = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t", "--", *probe]
+            log(f"plain leg: live traffic, rocprofv3 --pmc {counter}")
+            p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                                 start_new_session=True, env={**os.environ, "TMPDIR": "/tmp"})
+            try:
+                so, se = p.communicate(timeout=240)
+            except subprocess.TimeoutExpired:
+                os.killpg(p.pid, signal.SIGKILL)
+                p.communicate()
+                log(f"plain leg: live traffic run ({counter}) timed out")
+                return None
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            ran = [ln for ln in so.splitlines() if ln.startswith("{")]
+            if p.returncode != 0 or not files or not ran:
+                log(f"plain leg: live traffic run ({counter}) failed: rc {p.returncode}, {se.strip()[-300:]}")
+                return None
+            ran = json.loads(ran[-1])
+            decide, copy = {}, []
+            for path in files:
+                with open(path) as fp:
+                    for r in csv.DictReader(fp):
+                        if r["Counter_Name"] != counter:
+                            continue
+                        if "k_decide<" in r["Kernel_Name"]:
+                            decide[int(r["Dispatch_Id"])] = float(r["Counter_Value"])
+                        elif "copyBuffer" in r["Kernel_Name"] and float(r["Counter_Value"]) > 65536:
+                            copy.append(float(r["Counter_Value"]))
+            timed = [decide[k] for k in sorted(decide)][-args.steps:]
+            if len(timed) < args.steps or not copy:
+                log(f"plain leg: live traffic run ({counter}): {len(decide)} decide and {len(copy)} copy dispatches")
+                return None
+            out[counter] = {"kib": float(np.mean(timed)), "scale": (ran["copy_bytes"] / 1024) / float(np.mean(copy))}
+            out["params"], out["flags"] = ran["params"], ran["flags"]
+    f, w = out["FETCH_SIZE"], out["WRITE_SIZE"]
+    return {"bytes": (f["kib"] * f["scale"] + w["kib"] * w["scale"]) * 1024.0, "fetch_kib": f["kib"],
+            "write_kib": w["kib"], "fetch_scale": f["scale"], "write_scale": w["scale"], "params": out["params"],
+            "flags": out["flags"], "dispatches": args.steps}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,7 +356,12 @@ def main() -> int:
     ap.add_argument("--plain-sample", type=int, default=32, help="served requests also stepped alone")
     ap.add_argument("--copy-cal", action="store_true",
                     help="only run a 256 MiB device-to-device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--no-live-traffic", action="store_true",
+                    help="take the decide launch's HBM traffic from a committed profile instead of measuring it")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.traffic_probe:
+        return traffic_probe(args)
     if args.copy_cal:
         import torch
 
@@ -255,30 +383,16 @@ def main() -> int:
               "(flocks would shard by HBM bytes across GPUs, see the design notes)", file=sys.stderr)
         return 2
 
+    # the traffic runs start before this process touches the GPU
+    live = None if (args.no_live_traffic or args.retune) else live_traffic(args)
+
     import flock_hip as fh
 
     n = args.birds
     side = fh.cube_side_for(n)
     words = fh.gen_cube(n, side, args.seed)
     dev = fh.device_info()
-
-    # Tunables of the plain step, picked by the autotuner on this workload.
-    prm, tuned, flags = {}, None, args.flags
-    if args.params:
-        for item in args.params.split(","):
-            k, v = item.split("=")
-            if k == "flags":
-                flags = int(v)
-            else:
-                prm[k] = float(v) if k == "cell" else int(v)
-    elif not args.no_tune:
-        from flock_hip import autotune
-
-        # trials time the same window as the timed run (radii grow each decision)
-        log(f"plain leg: autotune at {n} birds")
-        tuned = autotune.tune([words], "plain", flags=args.flags, warmup=args.warmup, steps=args.steps, progress=log,
-                              **tune_cache(args, f"plain-cube-{n}-seed{args.seed}"))
-        prm, flags = tuned["params"], tuned["flags"]
+    prm, tuned, flags = plain_tunables(args, words)
 
     def timed_run(params: dict, flags: int) -> tuple[float, int]:
         """Fused step replayed from a hipGraph; wall clock over K steps,
@@ -332,7 +446,14 @@ def main() -> int:
     # waves per SIMD the register file allows (512 VGPRs per lane, granule 8)
     waves = min(8, 512 // (((vgpr + 7) // 8) * 8)) if vgpr else None
     full_prm = {"cell": 4.0, "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0, **prm}
-    traffic = profiled_traffic(full_prm, flags)
+    if live and live["flags"] == flags and live["params"] == prm:
+        traffic = {**live, "source": "measured in this run: rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, of "
+                                     "bench.py --traffic-probe (same workload and tunables, timed steps' decide "
+                                     "dispatches), each scaled by the probe's 256 MiB copy"}
+    else:
+        traffic = profiled_traffic(full_prm, flags)
+        if traffic:
+            traffic["source"] = f"profiles/{traffic['profile']} (counter runs at these tunables)"
 
     out = {
         "metric": "bird-steps/s served+batched vs plain; decide ms @1M birds; p50/p99 latency",
@@ -375,7 +496,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
                      "kernel": "decide with fused flight" if fused else "decide", "bytes_per_launch": dec_bytes,
-                     "traffic_profile": traffic["profile"] if traffic else None},
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "traffic_detail": ({k: traffic[k] for k in ("fetch_kib", "write_kib", "fetch_scale",
+                                                                  "write_scale", "dispatches")}
+                                        if traffic and "fetch_kib" in traffic else None)},
     }
     if not args.no_cpu:
         log("cpu baseline")

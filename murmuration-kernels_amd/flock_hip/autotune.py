@@ -90,7 +90,7 @@ def _buckets(n: int) -> int:
 
 def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, steps: int = 8,
          cache_path: str | None = None, cells=CELLS, progress=None, thorough: bool = True, times=None,
-         shifts=SHIFTS, spatial=(0, 1), label: str = "", refresh: bool = False) -> dict:
+         shifts=SHIFTS, spatial=(0, 1), label: str = "", refresh: bool = False, cached_only: bool = False):
     """Returns {"params": {...}, "flags": f, "ms_per_step": x, "trials": [...]}
     ("trials" only when searched; a cache entry holds the chosen point)
     for the batch of `starts` (kind "batch") or the single flock starts[0]
@@ -104,7 +104,8 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
     per trial. `label` names the workload in the cache key (flocks of the
     same sizes but another distribution want their own entry); `refresh`
     searches again and replaces a cached entry. A result read from the cache
-    carries "from_cache": True."""
+    carries "from_cache": True. With `cached_only` nothing is searched: a
+    workload the cache lacks returns None."""
     times = [0.0] * len(starts) if times is None else [float(t) for t in times]
     sizes = [int(w.shape[0]) for w in starts]
     key = (f"v{SEARCH_VERSION}|{device_info()['name']}|{label}|" + _signature(kind, sizes, steps, flags) +
@@ -112,6 +113,8 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
     cache = _load(cache_path) if cache_path else {}
     if key in cache and not refresh:
         return {**cache[key], "from_cache": True}
+    if cached_only:
+        return None
 
     def timer(prm):
         prm = dict(prm)
