@@ -32,14 +32,16 @@ def log(msg: str) -> None:
     print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
-def step_bytes(n: int, buckets: int, due: int, resident: bool) -> dict:
+def step_bytes(n: int, buckets: int, due: int, resident: bool, rebuild_every: int = 1) -> dict:
     """Algorithmic HBM bytes per pass (n birds, B buckets, d due), keyed like
     the per-pass times: the standard step (histogram, scan, scatter of the
     sorted copies, decide, integrate) or the resident one (decide, integrate
-    counting the movers, then scan and move as one 'grid_scatter' pass)."""
+    counting the movers, then scan and move as one 'grid_scatter' pass, which
+    lazy rebuilds run once every `rebuild_every` steps: its bytes per step are
+    spread over them, like its mean time)."""
     if resident:
         return {"decide": 28 * n + 4 * buckets + 32 * due, "integrate": 72 * n,
-                "grid_scatter": 20 * buckets + 120 * n}
+                "grid_scatter": (20 * buckets + 120 * n) / rebuild_every}
     return {
         "grid_histogram": 16 * n + 8 * buckets,
         "grid_scan": 8 * buckets,
@@ -421,7 +423,8 @@ def main() -> int:
         passes = f.step(args.steps, 0, timing=True)
     due_last = int(passes["due_last"])  # birds that decided in the last timed step
 
-    nbytes = step_bytes(n, buckets, due_last, bool(flags & fh.FLAG_RESIDENT))
+    lazy = bool(flags & fh.FLAG_LAZY)
+    nbytes = step_bytes(n, buckets, due_last, bool(flags & fh.FLAG_RESIDENT), fh.LAZY_PERIOD if lazy else 1)
     # With the fused flight, the decide launch also does the integrate pass's
     # work (its event interval is then empty): the dominant kernel's bytes are
     # both passes'. decide alone is timed in a run without the fusion.
@@ -433,14 +436,14 @@ def main() -> int:
     if fused:
         pass_gbs = {"decide+flight": achieved,
                     "grid_scatter": nbytes["grid_scatter"] / (passes["grid_scatter"] * 1e-3) / 1e9}
-        with fh.Flock.from_words(words, flags=flags & ~(fh.FLAG_GRAPH | fh.FLAG_FLIGHT), **prm) as f:
+        with fh.Flock.from_words(words, flags=flags & ~(fh.FLAG_GRAPH | fh.FLAG_FLIGHT | fh.FLAG_LAZY), **prm) as f:
             f.step(args.warmup)
             decide_alone_ms = f.step(args.steps, 0, timing=True)["decide"]
     else:
         pass_gbs = {k: b / (passes[k] * 1e-3) / 1e9 for k, b in nbytes.items() if passes[k] > 0}
         decide_alone_ms = dec_ms
     dec_inst = (f"decide:{full_prm_block(prm)}:{prm.get('decide_group', 1)}:"
-                f"{('f' if flags & fh.FLAG_FLIGHT else 'r') if flags & fh.FLAG_RESIDENT else 's'}")
+                f"{('m' if lazy else 'f' if flags & fh.FLAG_FLIGHT else 'r') if flags & fh.FLAG_RESIDENT else 's'}")
     kinfo = fh.kernel_info(dec_inst)
     vgpr = kinfo["vgprs"]
     # waves per SIMD the register file allows (512 VGPRs per lane, granule 8)

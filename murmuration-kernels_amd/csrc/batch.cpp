@@ -915,6 +915,7 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
     fh_params prm;
     fh_status st = resolve_params(params, &prm, "fh_batch_create");
     if (st) return st;
+    if (prm.flags & FH_FLAG_LAZY) return set_error(FH_ERR_INVALID, "fh_batch_create: FH_FLAG_LAZY is for single flocks");
     st = require_device();
     if (st) return st;
 
@@ -1221,6 +1222,11 @@ size_t fh_flock_bytes(uint32_t n, const fh_params *params) {
     size_t bytes = birds * batch_bytes_per_bird() + bk * batch_bytes_per_bucket() + bk / 8 /*occupancy bits*/;
     if (prm.flags & FH_FLAG_RESIDENT)
         bytes += birds * resident_bytes_per_bird((prm.flags & FH_FLAG_FLIGHT) != 0) + bk * resident_bytes_per_bucket();
+    if (prm.flags & FH_FLAG_LAZY) {  // two sets of mover list entries, and 4 arrays of list heads
+        size_t mb = 4096;
+        while (mb < birds / 8 && mb < (1u << 20)) mb <<= 1;
+        bytes += birds * 2 * 36 + mb * 4 * 4 + 64;
+    }
     return bytes;
 }
 

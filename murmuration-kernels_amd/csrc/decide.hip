@@ -24,12 +24,14 @@ __global__ void k_neighbours(Geometry g, Pools p, uint32_t bird, int walk, uint3
     for (uint32_t t = own.x; t < min(own.y, g.n_slots); ++t)
         if (__float_as_uint(p.sbird[2u * t].w) == bird) self = t;
     uint32_t c = 0;
-    walk_cells<1, 8, 2, false>(w, sg, p, Cand{p.sbird, p.sbird + 1}, g.n_slots, self, 0, true,
-                        [&](uint32_t t, f3, float d2) {
-                            if (walk == 1 && d2 > w.r2) return;
-                            if (c < cap) ids[c] = __float_as_uint(p.sbird[2u * t].w);
-                            ++c;
-                        });
+    walk_cells<1, 8, 2, false, false>(
+        w, sg, p, Cand{p.sbird, p.sbird + 1}, g.n_slots, self, 0, true,
+        [&](uint32_t t, f3, float d2) {
+            if (walk == 1 && d2 > w.r2) return;
+            if (c < cap) ids[c] = __float_as_uint(p.sbird[2u * t].w);
+            ++c;
+        },
+        ResWork{}, [](uint32_t, f3, float) {});
     *count = c;
 }
 
@@ -47,6 +49,11 @@ hipError_t launch_decide_resident(const Geometry &g, const Pools &p, const Recor
                                   uint32_t birds_per_thread, uint32_t group, int walk_flags, uint32_t *due_clear,
                                   uint32_t *work_clear, const ResWork &w, const Records &flown, hipStream_t s,
                                   const DecideTiming &tm) {
+    if (walk_flags & WALK_MOVERS) {
+        if (!(walk_flags & WALK_FLIGHT) || (walk_flags & WALK_WINDOW)) return hipErrorInvalidValue;
+        return decide_dispatch_movers(g, p, rec, block, birds_per_thread, group, walk_flags, due_clear, work_clear, w,
+                                      flown, tm, s);
+    }
     if (walk_flags & WALK_WINDOW)
         return decide_dispatch<true>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear, work_clear,
                                      w, flown, tm, s, true);
@@ -63,12 +70,12 @@ hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, i
 template <int BLOCK, bool RES, bool FLY>
 static const void *decide_fn_group(uint32_t group) {
     switch (group) {
-        case 1: return reinterpret_cast<const void *>(k_decide<BLOCK, 1, RES, FLY, false>);
-        case 2: return reinterpret_cast<const void *>(k_decide<BLOCK, 2, RES, FLY, false>);
-        case 4: return reinterpret_cast<const void *>(k_decide<BLOCK, 4, RES, FLY, false>);
-        case 8: return reinterpret_cast<const void *>(k_decide<BLOCK, 8, RES, FLY, false>);
-        case 16: return reinterpret_cast<const void *>(k_decide<BLOCK, 16, RES, FLY, false>);
-        case 64: return reinterpret_cast<const void *>(k_decide<BLOCK, 64, RES, FLY, false>);
+        case 1: return reinterpret_cast<const void *>(k_decide<BLOCK, 1, RES, FLY, false, false>);
+        case 2: return reinterpret_cast<const void *>(k_decide<BLOCK, 2, RES, FLY, false, false>);
+        case 4: return reinterpret_cast<const void *>(k_decide<BLOCK, 4, RES, FLY, false, false>);
+        case 8: return reinterpret_cast<const void *>(k_decide<BLOCK, 8, RES, FLY, false, false>);
+        case 16: return reinterpret_cast<const void *>(k_decide<BLOCK, 16, RES, FLY, false, false>);
+        case 64: return reinterpret_cast<const void *>(k_decide<BLOCK, 64, RES, FLY, false, false>);
         default: return nullptr;
     }
 }
@@ -84,19 +91,22 @@ static const void *decide_fn(uint32_t block, uint32_t group) {
     }
 }
 
+const void *decide_movers_fn(uint32_t block, uint32_t group);
+
 // "decide" is the default instance (block 256, G 1, sorted copies);
 // "decide:BLOCK:G:K" names the instance for that workgroup size and lane
-// group over sorted copies (K = s), resident records (r), or resident records
-// with the fused flight (f).
+// group over sorted copies (K = s), resident records (r), resident records
+// with the fused flight (f), or those with the mover grid too (m).
 hipError_t decide_kernel_attributes(const char *name, hipFuncAttributes *out) {
     if (!__builtin_strcmp(name, "decide"))
-        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_decide<256, 1, false, false, false>));
+        return hipFuncGetAttributes(out, reinterpret_cast<const void *>(k_decide<256, 1, false, false, false, false>));
     unsigned block = 0, group = 0;
     char kind = 0;
     if (std::sscanf(name, "decide:%u:%u:%c", &block, &group, &kind) == 3) {
         const void *f = kind == 's'   ? decide_fn<false, false>(block, group)
                         : kind == 'r' ? decide_fn<true, false>(block, group)
                         : kind == 'f' ? decide_fn<true, true>(block, group)
+                        : kind == 'm' ? decide_movers_fn(block, group)
                                       : nullptr;
         if (f) return hipFuncGetAttributes(out, f);
     }

@@ -135,9 +135,15 @@ __device__ __forceinline__ bool bucket_occupied(const Pools &p, uint32_t b) {
 // buckets are read first (a small, cache-resident table) and empty buckets
 // never touch the offsets table. Within a lane the visit order is the walk
 // order.
-template <int G, int PF, int STRIDE, bool WIN, class Visit>
+//
+// With MG (FH_FLAG_LAZY) each cell's mover list is visited too, after the
+// cell's own bucket: `visit_m(t, d, d2)` sees list entry t. A mover's record
+// in its old bucket is never taken (its position lies in another cell), so
+// each bird is seen once.
+template <int G, int PF, int STRIDE, bool WIN, bool MG, class Visit, class VisitM>
 __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const Pools &p, Cand cand, uint32_t n_slots,
-                                           uint32_t self, int32_t first, bool use_occ, Visit &&visit) {
+                                           uint32_t self, int32_t first, bool use_occ, Visit &&visit,
+                                           const ResWork &mw, VisitM &&visit_m) {
     if (!w.ok) return;
     const int32_t nx = w.hi[0] - w.lo[0] + 1, ny = w.hi[1] - w.lo[1] + 1, nz = w.hi[2] - w.lo[2] + 1;
     const int32_t ncell = nx * ny * nz;
@@ -157,6 +163,7 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
         int32_t cx[PF], cy[PF], cz[PF];
         uint32_t s0[PF], s1[PF], bk[PF];
         bool want[PF];
+        [[maybe_unused]] uint32_t mh[PF];
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             cx[k] = x;
@@ -166,6 +173,7 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
             s1[k] = 0u;
             want[k] = base + k * G < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2);
             bk[k] = bucket_of_cell(sg, x, y, z);
+            if constexpr (MG) mh[k] = want[k] && mw.mr_head ? mw.mr_head[bk[k] & mw.mmask] : MOVER_NIL;
 #pragma unroll
             for (int s = 0; s < G; ++s)
                 if (++x > w.hi[0]) {
@@ -222,6 +230,18 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
                         visit(t, d, d2);
                 }
             }
+            if constexpr (MG) {
+                const float fx = (float)cx[k], fy = (float)cy[k], fz = (float)cz[k];
+                // a list holds each slot once at most: n_slots entries bound any walk of it
+                uint32_t t = mh[k];
+                for (uint32_t it = 0; t < n_slots && it < n_slots; ++it, t = mw.mr_next[t]) {
+                    const float4 c = mw.mr_p4[t];  // w: the mover's slot
+                    const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
+                    const float d2 = dot3(d, d);
+                    if ((int)(__float_as_uint(c.w) != self) & (int)in_cell(c, sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2))
+                        visit_m(t, d, d2);
+                }
+            }
         }
     }
 }
@@ -233,10 +253,12 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
 // lane of the group loads the same words); a lane then loads UNROLL of its
 // candidates (position and velocity) before it tests the first, so one
 // memory latency covers UNROLL candidates. `visit(d, d2, velocity)`.
-template <int G, int PF, int STRIDE, bool WIN, class Visit>
+// With MG, each cell's mover list follows its own bucket, taken by the
+// group's first lane (lists hold a few birds at most).
+template <int G, int PF, int STRIDE, bool WIN, bool MG, class Visit>
 __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, const Pools &p, Cand cand,
                                                  uint32_t n_slots, uint32_t self, uint32_t lane, bool use_occ,
-                                                 Visit &&visit) {
+                                                 Visit &&visit, const ResWork &mw) {
     if (!w.ok) return;
     const int32_t nx = w.hi[0] - w.lo[0] + 1, ny = w.hi[1] - w.lo[1] + 1, nz = w.hi[2] - w.lo[2] + 1;
     const int32_t ncell = nx * ny * nz;
@@ -245,6 +267,7 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
         int32_t cx[PF], cy[PF], cz[PF];
         uint32_t s0[PF], s1[PF], bk[PF];
         bool want[PF];
+        [[maybe_unused]] uint32_t mh[PF];
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             cx[k] = x;
@@ -254,6 +277,7 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
             s1[k] = 0u;
             want[k] = base + k < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2);
             bk[k] = bucket_of_cell(sg, x, y, z);
+            if constexpr (MG) mh[k] = want[k] && lane == 0u && mw.mr_head ? mw.mr_head[bk[k] & mw.mmask] : MOVER_NIL;
             if (++x > w.hi[0]) {
                 x = w.lo[0];
                 if (++y > w.hi[1]) {
@@ -300,6 +324,17 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
                     // inside the bucket, not the bird itself, in this cell, within 2r
                     if ((int)(t < s1[k]) & (int)(t != self) & (int)in_cell(c[u], sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2))
                         visit(d, d2, v[u]);
+                }
+            }
+            if constexpr (MG) {
+                const float fx = (float)cx[k], fy = (float)cy[k], fz = (float)cz[k];
+                uint32_t t = mh[k];
+                for (uint32_t it = 0; t < n_slots && it < n_slots; ++it, t = mw.mr_next[t]) {
+                    const float4 c = mw.mr_p4[t];  // w: the mover's slot
+                    const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
+                    const float d2 = dot3(d, d);
+                    if ((int)(__float_as_uint(c.w) != self) & (int)in_cell(c, sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2))
+                        visit(d, d2, mw.mr_v4[t]);
                 }
             }
         }
@@ -373,20 +408,24 @@ __device__ __forceinline__ void accumulate(Acc &a, f3 f_i, float r2, f3 d, float
     if (d2 <= r2) accumulate_r(a, f_i, dhat, dist, velocity);
 }
 
-template <int G, int PF, int STRIDE, bool WIN>
+template <int G, int PF, int STRIDE, bool WIN, bool MG>
 __device__ __forceinline__ Acc walk_sums(const Pools &p, Cand cand, const Seg &sg, uint32_t n_slots, uint32_t self,
-                                         const Walk &w, f3 f_i, int32_t first, bool use_occ) {
+                                         const Walk &w, f3 f_i, int32_t first, bool use_occ, const ResWork &mw) {
     Acc a;
     acc_zero(a);
     if constexpr (G <= 8)
-        walk_cells<G, PF, STRIDE, WIN>(w, sg, p, cand, n_slots, self, first, use_occ, [&](uint32_t t, f3 d, float d2) {
-            accumulate(a, f_i, w.r2, d, d2, [&] { return cand_vel<STRIDE, WIN>(cand, t); });
-        });
+        walk_cells<G, PF, STRIDE, WIN, MG>(
+            w, sg, p, cand, n_slots, self, first, use_occ,
+            [&](uint32_t t, f3 d, float d2) {
+                accumulate(a, f_i, w.r2, d, d2, [&] { return cand_vel<STRIDE, WIN>(cand, t); });
+            },
+            mw, [&](uint32_t t, f3 d, float d2) { accumulate(a, f_i, w.r2, d, d2, [&] { return mw.mr_v4[t]; }); });
     else
-        walk_cells_split<G, PF, STRIDE, WIN>(w, sg, p, cand, n_slots, self, (uint32_t)first, use_occ,
-                                        [&](f3 d, float d2, float4 vj) {
-                                            accumulate(a, f_i, w.r2, d, d2, [&] { return vj; });
-                                        });
+        walk_cells_split<G, PF, STRIDE, WIN, MG>(w, sg, p, cand, n_slots, self, (uint32_t)first, use_occ,
+                                                 [&](f3 d, float d2, float4 vj) {
+                                                     accumulate(a, f_i, w.r2, d, d2, [&] { return vj; });
+                                                 },
+                                                 mw);
     return a;
 }
 
@@ -464,11 +503,16 @@ constexpr uint32_t NO_BIRD = 0xffffffffu;
 // density at 1M: -3% decide). G = 1 would spill inside the walk.
 constexpr int decide_min_waves(int g) { return g >= 16 ? 6 : g >= 4 ? 5 : g == 2 ? 4 : 1; }
 
-template <int BLOCK, int G, bool RES, bool FLY, bool WIN>
+// MG: the walks also visit the mover grid (FH_FLAG_LAZY; resident records with
+// the fused flight only), and the flight counts movers into it unless the step
+// rebuilds (WALK_LAZY).
+template <int BLOCK, int G, bool RES, bool FLY, bool WIN, bool MG>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_min_waves(G), 8))) void k_decide(
     Geometry g, Pools p, Records rec, float time, uint32_t chunk, int walk_flags, uint32_t *due_clear,
     uint32_t *work_clear, ResWork w, Records flown) {
-    const bool allow_tiled = (walk_flags & WALK_TILED) != 0, use_occ = (walk_flags & WALK_OCC) != 0;
+    static_assert(!MG || (RES && FLY && !WIN), "the mover grid goes with resident records and the fused flight");
+    const ResWork &movers = w;  // inside the walk loop, `w` is the bird's Walk
+    const bool allow_tiled = !MG && (walk_flags & WALK_TILED) != 0, use_occ = (walk_flags & WALK_OCC) != 0;
     const bool persist = (walk_flags & WALK_PERSIST) != 0;
     // Cells whose bucket ranges a lane loads at once. The split walks (G >= 16)
     // take one cell at a time: their buckets are long, and the smaller code
@@ -493,6 +537,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
     if (RES && blockIdx.x == 0 && threadIdx.x == 0) {
         if (due_clear) *due_clear = 0;
         if (work_clear) *work_clear = 0;
+    }
+    [[maybe_unused]] float now = 0.0f;  // the step's clock (MG: from the lazy clock)
+    if constexpr (MG) {
+        now = *w.mclock_now;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            // The next step's clock; between rebuilds also the segment's (its
+            // clock field is read by no launch of this step).
+            *w.mclock_next = now + DT;
+            if ((walk_flags & WALK_LAZY) && w.mtick->live && w.mtick->left) w.mtick->time = now + DT;
+        }
+        // mover lists a later step of the period writes start empty
+        for (int j = 0; j < 2; ++j)
+            if (w.mc_head[j])
+                for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i <= w.mmask; i += gridDim.x * BLOCK)
+                    w.mc_head[j][i] = MOVER_NIL;
     }
     const uint32_t n_chunks = (g.n_slots + chunk - 1) / chunk;
     for (uint32_t round = 0;; ++round) {  // one chunk per trip; every lane leaves together
@@ -540,7 +599,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
         bool due;
         if (RES) {
             const Seg &sg = g.segs[slot_segment(g, rec, s)];
-            due = sg.left && sg.time + DT >= cand_pos<STRIDE, WIN>(cand, s).w;  // frozen flocks never decide
+            due = sg.left && (MG ? now : sg.time) + DT >= cand_pos<STRIDE, WIN>(cand, s).w;  // frozen flocks never decide
         } else {
             due = time >= p.snext[s];
         }
@@ -651,7 +710,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
                         }
                     }
         } else if (active) {
-            a = walk_sums<G, PF, STRIDE, WIN>(p, cand, g.segs[si], g.n_slots, slot, w, f_i, (int32_t)lane_g, use_occ);
+            a = walk_sums<G, PF, STRIDE, WIN, MG>(p, cand, g.segs[si], g.n_slots, slot, w, f_i, (int32_t)lane_g,
+                                                  use_occ, movers);
         }
         if (G == 1) __syncthreads();  // the union box is reset by the next iteration
         if (G > 1) reduce_group<G>(a);
@@ -676,7 +736,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
         // otherwise cost the G = 1-2 walks a wave per SIMD.
         __syncthreads();
         for (uint32_t b0 = start; b0 < end; b0 += 2u * BLOCK)
-            resident_flight2(g, rec, flown, w, b0 + threadIdx.x, b0 + BLOCK + threadIdx.x, end, start);
+            resident_flight2<MG>(g, rec, flown, w, b0 + threadIdx.x, b0 + BLOCK + threadIdx.x, end, start,
+                                 (walk_flags & WALK_LAZY) != 0);
     }
     __syncthreads();  // due_slots and n_due are reused by the next chunk
     }
@@ -686,7 +747,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
 // once (occupancy of this instance at this LDS size, times the CUs), and no
 // more than there are chunks. The occupancy is a property of the compiled
 // kernel; it is looked up once per instance and chunk size.
-template <int BLOCK, int G, bool RES, bool FLY, bool WIN>
+template <int BLOCK, int G, bool RES, bool FLY, bool WIN, bool MG>
 static uint32_t persistent_blocks(uint32_t chunk, uint32_t n_chunks, size_t lds) {
     // by log2(chunk / BLOCK) (1..16 birds per thread), without and with a window
     static int per_cu[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
@@ -695,7 +756,7 @@ static uint32_t persistent_blocks(uint32_t chunk, uint32_t n_chunks, size_t lds)
     int *slot = &per_cu[lds > chunk * sizeof(uint32_t) ? 1 : 0][idx];
     int occ = __atomic_load_n(slot, __ATOMIC_RELAXED);
     if (occ <= 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_decide<BLOCK, G, RES, FLY, WIN>, BLOCK, lds) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_decide<BLOCK, G, RES, FLY, WIN, MG>, BLOCK, lds) != hipSuccess ||
             occ <= 0)
             occ = 1;
         __atomic_store_n(slot, occ, __ATOMIC_RELAXED);
@@ -709,30 +770,30 @@ static uint32_t persistent_blocks(uint32_t chunk, uint32_t n_chunks, size_t lds)
 
 // With timing events, the launch itself records the kernel's start and end
 // (hipExtLaunchKernelGGL), so the measured interval is the kernel alone.
-template <int BLOCK, int G, bool RES, bool FLY, bool WIN>
+template <int BLOCK, int G, bool RES, bool FLY, bool WIN, bool MG>
 static void launch_decide_block(const Geometry &g, const Pools &p, const Records &rec, float time, uint32_t chunk,
                                 int walk_flags, uint32_t *due_clear, uint32_t *work_clear, const ResWork &w,
                                 const Records &flown, const DecideTiming &tm, hipStream_t s) {
     const uint32_t n_chunks = (g.n_slots + chunk - 1) / chunk;
     const size_t lds = decide_lds_bytes(chunk, walk_flags);
     const uint32_t blocks =
-        (walk_flags & WALK_PERSIST) ? persistent_blocks<BLOCK, G, RES, FLY, WIN>(chunk, n_chunks, lds) : n_chunks;
+        (walk_flags & WALK_PERSIST) ? persistent_blocks<BLOCK, G, RES, FLY, WIN, MG>(chunk, n_chunks, lds) : n_chunks;
     if (tm.start)
-        hipExtLaunchKernelGGL((k_decide<BLOCK, G, RES, FLY, WIN>), dim3(blocks), dim3(BLOCK), lds, s, tm.start, tm.stop, 0u, g,
+        hipExtLaunchKernelGGL((k_decide<BLOCK, G, RES, FLY, WIN, MG>), dim3(blocks), dim3(BLOCK), lds, s, tm.start, tm.stop, 0u, g,
                               p, rec, time, chunk, walk_flags, due_clear, work_clear, w, flown);
     else
-        hipLaunchKernelGGL((k_decide<BLOCK, G, RES, FLY, WIN>), dim3(blocks), dim3(BLOCK), lds, s, g, p, rec, time, chunk,
+        hipLaunchKernelGGL((k_decide<BLOCK, G, RES, FLY, WIN, MG>), dim3(blocks), dim3(BLOCK), lds, s, g, p, rec, time, chunk,
                            walk_flags, due_clear, work_clear, w, flown);
 }
 
-template <int BLOCK, bool RES, bool FLY, bool WIN>
+template <int BLOCK, bool RES, bool FLY, bool WIN, bool MG>
 static hipError_t launch_decide_group(const Geometry &g, const Pools &p, const Records &rec, float time,
                                       uint32_t chunk, uint32_t group, int walk_flags, uint32_t *due_clear,
                                       uint32_t *work_clear, const ResWork &w, const Records &flown,
                                       const DecideTiming &tm, hipStream_t s) {
 #define FH_DECIDE_G(GV)                                                                                    \
     case GV:                                                                                               \
-        launch_decide_block<BLOCK, GV, RES, FLY, WIN>(g, p, rec, time, chunk, walk_flags, due_clear, work_clear, w, flown, tm, \
+        launch_decide_block<BLOCK, GV, RES, FLY, WIN, MG>(g, p, rec, time, chunk, walk_flags, due_clear, work_clear, w, flown, tm, \
                                             s);                                                                    \
         break;
     switch (group) {
@@ -748,7 +809,7 @@ static hipError_t launch_decide_group(const Geometry &g, const Pools &p, const R
     return hipSuccess;
 }
 
-template <bool RES, bool FLY, bool WIN>
+template <bool RES, bool FLY, bool WIN, bool MG = false>
 static hipError_t launch_decide_any(const Geometry &g, const Pools &p, const Records &rec, float time,
                                     uint32_t block, uint32_t birds_per_thread, uint32_t group, int walk_flags,
                                     uint32_t *due_clear, uint32_t *work_clear, const ResWork &w,
@@ -757,7 +818,7 @@ static hipError_t launch_decide_any(const Geometry &g, const Pools &p, const Rec
     const uint32_t chunk = block * birds_per_thread;
     auto by_block = [&](auto block_c) {
         constexpr int B = decltype(block_c)::value;
-        return launch_decide_group<B, RES, FLY, WIN>(g, p, rec, time, chunk, group, walk_flags, due_clear, work_clear, w, flown,
+        return launch_decide_group<B, RES, FLY, WIN, MG>(g, p, rec, time, chunk, group, walk_flags, due_clear, work_clear, w, flown,
                                            tm, s);
     };
     hipError_t e;
@@ -789,6 +850,13 @@ hipError_t decide_dispatch(const Geometry &g, const Pools &p, const Records &rec
     return launch_decide_any<true, false, WIN>(g, p, rec, 0.0f, block, birds_per_thread, group, walk_flags, due_clear,
                                                work_clear, w, flown, tm, s);
 }
+
+// decide over resident records with the fused flight whose walks also visit
+// the mover grid (FH_FLAG_LAZY; instances without a window, decide_lazy.hip).
+hipError_t decide_dispatch_movers(const Geometry &g, const Pools &p, const Records &rec, uint32_t block,
+                                  uint32_t birds_per_thread, uint32_t group, int walk_flags, uint32_t *due_clear,
+                                  uint32_t *work_clear, const ResWork &w, const Records &flown,
+                                  const DecideTiming &tm, hipStream_t s);
 
 extern template hipError_t decide_dispatch<false>(const Geometry &, const Pools &, const Records &, float, uint32_t,
                                                   uint32_t, uint32_t, int, uint32_t *, uint32_t *, const ResWork &,

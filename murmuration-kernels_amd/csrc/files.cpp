@@ -68,6 +68,12 @@ fh_status resolve_params(const fh_params *in, fh_params *out, const char *who) {
     if (prm.bucket_shift > 8) return set_error(FH_ERR_INVALID, "%s: bucket_shift %u > 8", who, prm.bucket_shift);
     if ((prm.flags & FH_FLAG_RESIDENT) && (prm.flags & FH_FLAG_CANONICAL))
         return set_error(FH_ERR_INVALID, "%s: FH_FLAG_RESIDENT does not combine with FH_FLAG_CANONICAL", who);
+    if ((prm.flags & FH_FLAG_LAZY) &&
+        (!(prm.flags & FH_FLAG_RESIDENT) || !(prm.flags & FH_FLAG_FLIGHT) || (prm.flags & FH_FLAG_WINDOW) ||
+         (prm.flags & FH_FLAG_UNFUSED)))
+        return set_error(FH_ERR_INVALID,
+                         "%s: FH_FLAG_LAZY needs FH_FLAG_RESIDENT and FH_FLAG_FLIGHT, without FH_FLAG_WINDOW or "
+                         "FH_FLAG_UNFUSED", who);
     if (!(prm.cell >= 0.5f) || !(prm.cell <= 1000.0f))
         return set_error(FH_ERR_INVALID, "%s: cell %g outside [0.5, 1000] m", who, prm.cell);
     if (prm.buckets && (prm.buckets & (prm.buckets - 1)))

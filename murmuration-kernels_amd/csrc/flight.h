@@ -47,6 +47,19 @@ __device__ __forceinline__ void integrate_bird(float4 &p4, float4 &v4, const flo
     rate_out = rate;
 }
 
+// A mover from bucket ko to bucket kn at slot s: its count moves between the
+// buckets, it takes an arrival rank in kn, and its count moves between the
+// scan tiles' totals when the tile changes.
+__device__ __forceinline__ void count_mover(const ResWork &w, uint32_t s, uint32_t kn, uint32_t ko) {
+    w.arank[s] = atomicAdd(&w.arr[kn], 1u);
+    atomicAdd(&w.counts[kn], 1u);
+    atomicSub(&w.counts[ko], 1u);
+    if (kn / RES_TILE != ko / RES_TILE) {
+        atomicAdd(&w.tsum[kn / RES_TILE], 1u);
+        atomicSub(&w.tsum[ko / RES_TILE], 1u);
+    }
+}
+
 // Resident flight of slot s from its loaded records (position and decision
 // time p4, velocity and roll v4, steering and radius s4, old bucket ko):
 // banked flight into out[s] (out may be in), its new bucket by its flock's
@@ -67,14 +80,36 @@ __device__ __forceinline__ bool fly_loaded(const Seg &sg, const Records &in, con
     const uint32_t kn = bucket_of_cell(sg, cell_coord(p4.x, sg.inv_cell), cell_coord(p4.y, sg.inv_cell),
                                        cell_coord(p4.z, sg.inv_cell));
     if (kn == ko) return false;
-    w.arank[s] = atomicAdd(&w.arr[kn], 1u);
-    atomicAdd(&w.counts[kn], 1u);
-    atomicSub(&w.counts[ko], 1u);
-    if (kn / RES_TILE != ko / RES_TILE) {
-        atomicAdd(&w.tsum[kn / RES_TILE], 1u);
-        atomicSub(&w.tsum[ko / RES_TILE], 1u);
-    }
+    count_mover(w, s, kn, ko);
     return true;
+}
+
+// The same flight between rebuilds (FH_FLAG_LAZY), up to the bucket test:
+// banked flight into out[s]; returns the bird's new bucket through kn and
+// whether it is a mover (its bucket now differs from its slot's, ko: the
+// grid keeps the buckets of the last rebuild).
+__device__ __forceinline__ bool fly_loaded_lazy(const Seg &sg, const Records &in, const Records &out, uint32_t s,
+                                                float4 &p4, float4 &v4, const float4 s4, uint32_t ko, uint32_t &kn) {
+    float rate;
+    if (sg.left)
+        integrate_bird(p4, v4, s4, rate);
+    else
+        rate = in.spin[s];
+    out.p4[s] = p4;
+    out.v4[s] = v4;
+    out.spin[s] = rate;
+    kn = bucket_of_cell(sg, cell_coord(p4.x, sg.inv_cell), cell_coord(p4.y, sg.inv_cell),
+                        cell_coord(p4.z, sg.inv_cell));
+    return kn != ko;
+}
+
+// Appends the mover in slot s to the next step's list of its bucket
+// kn & mmask, as entry s (entries are indexed by slot, so no counter is
+// shared): its new position (its slot in w) and velocity.
+__device__ __forceinline__ void append_mover(const ResWork &w, uint32_t s, uint32_t kn, float4 p4, float4 v4) {
+    w.mw_p4[s] = make_float4(p4.x, p4.y, p4.z, __uint_as_float(s));
+    w.mw_v4[s] = v4;
+    w.mw_next[s] = atomicExch(&w.mw_head[kn & w.mmask], s);
 }
 
 // The mover bits of a wave whose lanes hold 64 consecutive slots from a
@@ -104,9 +139,10 @@ __device__ __forceinline__ void resident_flight(const Geometry &g, const Records
 // either flies, so one memory latency covers two birds. Slots at or past
 // `end` do not fly (`valid` is any slot below it, loaded in their place),
 // and a wave's bits are written only for a run that starts below `end`.
+template <bool MG>
 __device__ __forceinline__ void resident_flight2(const Geometry &g, const Records &in, const Records &out,
                                                  const ResWork &w, uint32_t s0, uint32_t s1, uint32_t end,
-                                                 uint32_t valid) {
+                                                 uint32_t valid, bool lazy) {
     const bool a0 = s0 < end, a1 = s1 < end;
     const uint32_t l0 = a0 ? s0 : valid, l1 = a1 ? s1 : valid;
     const Seg &sg0 = g.segs[slot_segment(g, in, l0)];
@@ -114,6 +150,13 @@ __device__ __forceinline__ void resident_flight2(const Geometry &g, const Record
     const float4 p0 = in.p4[l0], v0 = in.v4[l0], q0 = in.s4[l0];
     const float4 p1 = in.p4[l1], v1 = in.v4[l1], q1 = in.s4[l1];
     const uint32_t k0 = in.key[l0], k1 = in.key[l1];
+    if (MG && lazy) {
+        float4 f0 = p0, g0 = v0, f1 = p1, g1 = v1;
+        uint32_t n0 = 0, n1 = 0;
+        if (a0 && fly_loaded_lazy(sg0, in, out, s0, f0, g0, q0, k0, n0)) append_mover(w, s0, n0, f0, g0);
+        if (a1 && fly_loaded_lazy(sg1, in, out, s1, f1, g1, q1, k1, n1)) append_mover(w, s1, n1, f1, g1);
+        return;
+    }
     const bool m0 = a0 && fly_loaded(sg0, in, out, w, s0, p0, v0, q0, k0);
     if ((s0 & ~63u) < end) write_mover_bits(g, w, s0, m0);  // uniform over the wave
     const bool m1 = a1 && fly_loaded(sg1, in, out, w, s1, p1, v1, q1, k1);

@@ -51,7 +51,25 @@ struct fh_flock {
     hipGraphExec_t rgraph[2]{};
     hipGraphExec_t graph_run[2]{};   // GRAPH_RUNS[i] fused steps from the Hashed stage
     hipGraphExec_t rgraph_run[2]{};  // GRAPH_RUNS[i] resident steps from parity 0
+    // FH_FLAG_LAZY: the layout is rebuilt every LAZY_PERIOD steps. In between,
+    // each step flies the records in place (alternating between R[parity]'s
+    // positions, velocities and roll rates and F's), and appends its movers
+    // to the mover lists, which the next step's walks visit.
+    bool lazy = false;
+    uint32_t phase = 0;  // steps since the last rebuild
+    uint32_t fq = 0;     // 1: the current positions, velocities and roll rates are F's
+    uint32_t cpar = 0;   // due-counter parity of the next step (the step count's parity)
+    uint32_t mheads = 0;                   // list heads per phase (a power of two)
+    uint32_t *mhead = nullptr;             // LAZY_PERIOD arrays of heads; phase 0's stays empty
+    float *mclock = nullptr;               // the clock by step parity (cpar)
+    float4 *mp4[2]{}, *mv4[2]{};           // two sets of list entries (by slot): steps alternate
+    uint32_t *mnext[2]{};
+    hipGraphExec_t lgraph_run[2][2][2]{};  // GRAPH_RUNS[i] lazy steps from phase 0, layout parity, counter parity
 };
+
+// Steps per rebuild with FH_FLAG_LAZY. A divisor of every GRAPH_RUNS entry,
+// so that a graph run starts and ends on a rebuild.
+constexpr uint32_t LAZY_PERIOD = 4;
 
 // Events per timed step with per-pass timing: six on the stream between the
 // passes, two recorded by the decide launch at its kernel's start and end.
@@ -192,6 +210,18 @@ static fh_status alloc_resident(fh_flock *h) {
     const size_t o_off1 = a.take<uint32_t>(nb + 1), o_due = a.take<uint32_t>(4);
     const size_t o_tsum = a.take<uint32_t>(nb / RES_TILE);
     const size_t o_tsum2 = a.take<uint32_t>(nb / RES_TILE / RES_SUPER + 1);
+    // mover lists: about one list head per 8 birds, per phase
+    const bool lazy = (h->prm.flags & FH_FLAG_LAZY) != 0;
+    uint32_t mb = 4096;
+    while (lazy && mb < nn / 8 && mb < (1u << 20)) mb <<= 1;
+    const size_t o_mhead = lazy ? a.take<uint32_t>((size_t)LAZY_PERIOD * mb) : 0;
+    const size_t o_mclock = lazy ? a.take<float>(2) : 0;
+    size_t o_mp4[2]{}, o_mv4[2]{}, o_mnext[2]{};
+    for (int k = 0; lazy && k < 2; ++k) {
+        o_mp4[k] = a.take<float4>(nn);
+        o_mv4[k] = a.take<float4>(nn);
+        o_mnext[k] = a.take<uint32_t>(nn);
+    }
     hipError_t e = hipMalloc(&h->res_arena, a.used);
     if (e != hipSuccess) {
         h->res_arena = nullptr;
@@ -221,6 +251,17 @@ static fh_status alloc_resident(fh_flock *h) {
     h->off[0] = h->P.offsets;
     h->off[1] = reinterpret_cast<uint32_t *>(base + o_off1);
     h->res_due = reinterpret_cast<uint32_t *>(base + o_due);
+    if (lazy) {
+        h->lazy = true;
+        h->mheads = mb;
+        h->mhead = reinterpret_cast<uint32_t *>(base + o_mhead);
+        h->mclock = reinterpret_cast<float *>(base + o_mclock);
+        for (int k = 0; k < 2; ++k) {
+            h->mp4[k] = reinterpret_cast<float4 *>(base + o_mp4[k]);
+            h->mv4[k] = reinterpret_cast<float4 *>(base + o_mv4[k]);
+            h->mnext[k] = reinterpret_cast<uint32_t *>(base + o_mnext[k]);
+        }
+    }
     // arrivals start at zero and every rebuild zeroes them again
     FH_HIP(hipMemsetAsync(h->res_arena, 0, a.used, h->stream));
     FH_HIP(hipStreamSynchronize(h->stream));
@@ -241,6 +282,12 @@ static fh_status enter_resident(fh_flock *h) {
     FH_HIP(launch_resident_tile_sums(h->geo, p, h->W, h->stream));
     // parity 0's counters may hold a step taken before the last leave
     FH_HIP(launch_fill(h->res_due, 4, 0u, h->stream));
+    if (h->lazy) {
+        FH_HIP(launch_fill(h->mhead, LAZY_PERIOD * h->mheads, MOVER_NIL, h->stream));
+    }
+    h->phase = 0;
+    h->fq = 0;
+    h->cpar = 0;
     h->parity = 0;
     h->resident = true;
     h->counts_dirty = false;  // the counts now describe the current layout
@@ -248,9 +295,42 @@ static fh_status enter_resident(fh_flock *h) {
     return FH_OK;
 }
 
+// The records of the current step with FH_FLAG_LAZY: R[parity]'s, with the
+// positions, velocities and roll rates F's when `f`.
+static Records lazy_view(const fh_flock *h, uint32_t f) {
+    Records r = h->R[h->parity];
+    if (f) {
+        r.p4 = h->F.p4;
+        r.v4 = h->F.v4;
+        r.spin = h->F.spin;
+    }
+    return r;
+}
+
+// With FH_FLAG_LAZY between rebuilds: the layout of the current positions
+// (mover accounting against the slots' buckets, then the rebuild without a
+// clock tick), so the records and the grid are those of a rebuilt step.
+static fh_status settle_lazy(fh_flock *h) {
+    if (!h->lazy || !h->resident || h->phase == 0) return FH_OK;
+    const uint32_t q = h->parity;
+    Pools p = h->P;
+    p.offsets = h->off[q];
+    const Records cur = lazy_view(h, h->fq);
+    FH_HIP(launch_resident_account(h->geo, p, cur, h->W, h->stream));
+    FH_HIP(launch_resident_rebuild(h->geo, p, cur, h->R[1u - q], h->W, h->off[q], h->off[1u - q], nullptr, 0u,
+                                   h->stream));
+    FH_HIP(launch_fill(h->mhead, LAZY_PERIOD * h->mheads, MOVER_NIL, h->stream));
+    h->parity = 1u - q;
+    h->fq = 0;
+    h->phase = 0;
+    return FH_OK;
+}
+
 // Resident records -> bird-order state (before any call that reads it).
 static fh_status leave_resident(fh_flock *h) {
     if (!h->resident) return FH_OK;
+    fh_status st = settle_lazy(h);
+    if (st) return st;
     FH_HIP(launch_resident_flush(h->geo, h->P, h->R[h->parity], h->stream));
     h->resident = false;
     h->counts_dirty = true;  // the counts hold the last layout, not a fresh histogram
@@ -287,6 +367,79 @@ static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
     }
     FH_HIP(launch_resident_rebuild(h->geo, p, src, h->R[1u - q], h->W, h->off[q], h->off[1u - q], h->d_seg, 1u,
                                    h->stream));
+    if (ev)
+        for (int k = 3; k < 6; ++k) FH_HIP(hipEventRecord(ev[k], h->stream));
+    return FH_OK;
+}
+
+// The mover lists of a step at `phase` of the period: it reads the lists the
+// step before wrote (heads of its phase; phase 0's stay empty), writes the
+// next phase's (not at the rebuild, the period's last step), and clears the
+// heads a later phase writes: the first step clears the last writer's, the
+// rebuild the others. List entries alternate between two sets (written at
+// phases 0 and 2, and 1).
+static ResWork lazy_work(const fh_flock *h, uint32_t phase) {
+    ResWork w = h->W;
+    const uint32_t nh = h->mheads;
+    auto head = [&](uint32_t k) { return h->mhead + (size_t)k * nh; };
+    const uint32_t rd = (phase + 1u) % 2u, wr = phase % 2u;
+    w.mr_head = phase ? head(phase) : nullptr;  // the first step after a rebuild has no movers to visit
+    w.mr_next = h->mnext[rd];
+    w.mr_p4 = h->mp4[rd];
+    w.mr_v4 = h->mv4[rd];
+    const bool writes = phase + 1u < LAZY_PERIOD;
+    w.mw_head = writes ? head(phase + 1u) : nullptr;
+    w.mw_next = h->mnext[wr];
+    w.mw_p4 = h->mp4[wr];
+    w.mw_v4 = h->mv4[wr];
+    w.mc_head[0] = w.mc_head[1] = nullptr;
+    if (phase == 0) {
+        w.mc_head[0] = head(LAZY_PERIOD - 1u);
+    } else if (phase + 1u == LAZY_PERIOD) {
+        for (uint32_t k = 1; k + 1u < LAZY_PERIOD; ++k) w.mc_head[k - 1u] = head(k);  // phases 1 and 2
+    }
+    w.mmask = nh - 1u;
+    w.mclock_now = h->mclock + h->cpar;
+    w.mclock_next = h->mclock + (1u - h->cpar);
+    w.mtick = h->d_seg;
+    return w;
+}
+static_assert(LAZY_PERIOD == 4, "lazy_work clears two head arrays at the rebuild: phases 1 and 2");
+
+// One step with FH_FLAG_LAZY: decide (the walks visit the mover lists too)
+// with the fused flight from the current records into the other set. Between
+// rebuilds the flight appends its movers to the next step's lists, and the
+// decide launch commits the clock itself; every LAZY_PERIOD-th step the
+// flight counts the movers against the layout instead, and the rebuild
+// follows (it commits the clock). Events as resident_step_launches.
+static fh_status lazy_step_launches(fh_flock *h, hipEvent_t *ev) {
+    const uint32_t q = h->parity, c = h->cpar;
+    const bool rebuild = h->phase + 1u == LAZY_PERIOD;
+    Pools p = h->P;
+    p.offsets = h->off[q];
+    p.due_count = h->res_due + c;
+    p.work = h->res_due + 2 + c;
+    const Records cur = lazy_view(h, h->fq), flown = lazy_view(h, h->fq ^ 1u);
+    const int wf = walk_flags(h->prm.flags) | WALK_MOVERS | (rebuild ? 0 : WALK_LAZY);
+    if (ev) FH_HIP(hipEventRecord(ev[0], h->stream));
+    FH_HIP(launch_decide_resident(h->geo, p, cur, h->prm.decide_block, h->prm.birds_per_thread, h->prm.decide_group,
+                                  wf, h->res_due + (1u - c), h->res_due + 2 + (1u - c), lazy_work(h, h->phase), flown,
+                                  h->stream, ev ? DecideTiming{ev[6], ev[7]} : DecideTiming{}));
+    if (ev) {
+        FH_HIP(hipEventRecord(ev[1], h->stream));
+        FH_HIP(hipEventRecord(ev[2], h->stream));
+    }
+    if (rebuild) {
+        FH_HIP(launch_resident_rebuild(h->geo, p, flown, h->R[1u - q], h->W, h->off[q], h->off[1u - q], h->d_seg,
+                                       1u, h->stream));
+        h->parity = 1u - q;
+        h->fq = 0;
+        h->phase = 0;
+    } else {
+        h->fq ^= 1u;
+        h->phase += 1u;
+    }
+    h->cpar = 1u - c;
     if (ev)
         for (int k = 3; k < 6; ++k) FH_HIP(hipEventRecord(ev[k], h->stream));
     return FH_OK;
@@ -332,6 +485,30 @@ static fh_status capture_resident_run(fh_flock *h, int i) {
         h->parity = q0;
         return st;
     });
+}
+
+// GRAPH_RUNS[i] lazy steps from phase 0 at the current counter parity, for
+// both layout parities at once (a run whose rebuild count is odd ends at the
+// other one, so the next run needs that graph too: capturing it here keeps
+// the capture out of later timed steps). The host state is restored; the
+// run's own effect is applied by the caller after each replay.
+static fh_status capture_lazy_run(fh_flock *h, int i) {
+    const uint32_t q0 = h->parity, c0 = h->cpar;
+    fh_status st = FH_OK;
+    for (uint32_t q = 0; q < 2 && !st; ++q) {
+        h->parity = q;
+        st = capture_once(h, &h->lgraph_run[i][q][c0], [&]() -> fh_status {
+            fh_status s2 = FH_OK;
+            for (uint32_t k = 0; k < GRAPH_RUNS[i] && !s2; ++k) s2 = lazy_step_launches(h, nullptr);
+            h->parity = q;
+            h->cpar = c0;
+            h->phase = 0;
+            h->fq = 0;
+            return s2;
+        });
+    }
+    h->parity = q0;
+    return st;
 }
 
 // GRAPH_RUNS[i] fused steps from the Hashed stage (stage and counts state restored).
@@ -475,6 +652,10 @@ fh_status fh_destroy(fh_flock *h) {
     }
     for (auto &gx : h->rgraph)
         if (gx) (void)hipGraphExecDestroy(gx);
+    for (auto &by_q : h->lgraph_run)
+        for (auto &by_c : by_q)
+            for (auto &gx : by_c)
+                if (gx) (void)hipGraphExecDestroy(gx);
     if (h->res_arena) (void)hipFree(h->res_arena);
     if (h->d_nbr) (void)hipFree(h->d_nbr);
     (void)hipFree(h->arena);
@@ -644,6 +825,7 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
     const bool res = fused && (h->prm.flags & FH_FLAG_RESIDENT);
     if ((st = sync_clock(h))) return st;
     if (res && !h->resident && (st = enter_resident(h))) return st;
+    if (res && h->lazy) FH_HIP(launch_set_clock(h->mclock + h->cpar, h->time, h->stream));
     const uint32_t timed = steps - warmup;
     std::vector<hipEvent_t> ev;
     const bool per_pass = out && !graph;
@@ -664,19 +846,27 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
         // A run of GRAPH_RUNS[i] steps as one graph, never across the warm-up
         // boundary (the timed steps start on a step of their own).
         const uint32_t seg_end = s < warmup ? warmup : steps;
+        const bool lazy = res && h->lazy;
         int run = -1;
-        if (graph && (!res || h->parity == 0))
+        if (graph && (lazy ? h->phase == 0 : (!res || h->parity == 0)))
             for (int i = 0; i < 2 && run < 0; ++i)
                 if (seg_end - s >= GRAPH_RUNS[i]) run = i;
         if (run >= 0) {
             if (!res && h->stage != Stage::Hashed && (st = do_hash(h))) break;
-            if ((st = res ? capture_resident_run(h, run) : capture_graph_run(h, run))) break;
-            hipError_t e = hipGraphLaunch(res ? h->rgraph_run[run] : h->graph_run[run], h->stream);
+            if ((st = lazy  ? capture_lazy_run(h, run)
+                      : res ? capture_resident_run(h, run)
+                            : capture_graph_run(h, run)))
+                break;
+            hipError_t e = hipGraphLaunch(lazy  ? h->lgraph_run[run][h->parity][h->cpar]
+                                          : res ? h->rgraph_run[run]
+                                                : h->graph_run[run],
+                                          h->stream);
             if (e != hipSuccess) {
                 st = hip_fail(e, "hipGraphLaunch");
                 break;
             }
             for (uint32_t k = 0; k < GRAPH_RUNS[run]; ++k) h->time += DT;  // as one step at a time
+            if (lazy) h->parity ^= (GRAPH_RUNS[run] / LAZY_PERIOD) & 1u;  // one rebuild per period
             if (!res) {
                 h->stage = Stage::Hashed;
                 h->counts_dirty = true;
@@ -686,7 +876,10 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
             continue;
         }
         h->time += DT;  // host f32 clock (cuda/flock.cu:455); the device adds DT the same way
-        if (res) {
+        if (lazy) {
+            hipEvent_t *e6 = (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * EV_PER_STEP] : nullptr;
+            if ((st = lazy_step_launches(h, e6))) break;
+        } else if (res) {
             if (graph) {
                 if ((st = capture_resident_graph(h))) break;
                 hipError_t e = hipGraphLaunch(h->rgraph[h->parity], h->stream);
@@ -768,7 +961,8 @@ fh_status fh_step(fh_flock *h, uint32_t steps, uint32_t warmup, fh_pass_ms *out)
     destroy_events();
     if (out) {
         uint32_t due = 0;
-        FH_HIP(hipMemcpy(&due, res ? h->res_due + (h->parity ^ 1u) : h->P.due_count, 4, hipMemcpyDeviceToHost));
+        const uint32_t last = h->lazy ? h->cpar ^ 1u : h->parity ^ 1u;  // the last step's counter parity
+        FH_HIP(hipMemcpy(&due, res ? h->res_due + last : h->P.due_count, 4, hipMemcpyDeviceToHost));
         out->due_last = due;
     }
     return read_error_word(h);
@@ -782,7 +976,10 @@ fh_status fh_read_grid(fh_flock *h, uint32_t *keys, uint32_t *counts, uint32_t *
     FH_HIP(hipStreamSynchronize(h->stream));
     if (h->resident) {
         // The resident layout is the grid of the current positions: keys by
-        // bird, offsets, counts and the bird of each slot, read in place.
+        // bird, offsets, counts and the bird of each slot, read in place
+        // (between lazy rebuilds, after the layout is rebuilt).
+        if ((st = settle_lazy(h))) return st;
+        FH_HIP(hipStreamSynchronize(h->stream));
         const Records &r = h->R[h->parity];
         std::vector<uint32_t> id(n), key(n), off((size_t)B + 1);
         if (n) {

@@ -6,6 +6,7 @@
 // totals / add (:144-172), grid_scatter (:175-179). Keys, counts and offsets
 // are the reference's exactly; the scatter keeps the reference's arrival order
 // within a bucket unless the canonical pass reorders each bucket by bird index.
+#include "flight.h"
 #include "layout.h"
 #include "model.h"
 
@@ -361,6 +362,15 @@ __global__ void k_set_time(Seg *segs, uint32_t nseg, float time, float dt) {
     }
 }
 
+__global__ void k_set_clock(float *clock, float time) {
+    if (threadIdx.x == 0) *clock = time;
+}
+
+hipError_t launch_set_clock(float *clock, float time, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(64), 0, s, clock, time);
+    return hipGetLastError();
+}
+
 static inline uint32_t launch_tiles(const Geometry &g) {
     return g.tile_count ? g.tile_count : (g.n_birds + TILE - 1) / TILE;
 }
@@ -601,6 +611,34 @@ __global__ __launch_bounds__(TILE) void k_r_move(Geometry g, Pools p, Records cu
     nxt.spin[ns] = cur.spin[s];
     nxt.id[ns] = cur.id[s];
     nxt.key[ns] = kn;
+}
+
+// The rebuild's mover accounting for the current records without a flight:
+// each bird against the bucket of its slot (as fly_loaded counts after
+// flying). One lane per slot, whole waves (the mover bits are ballots).
+__global__ __launch_bounds__(TILE) void k_r_account(Geometry g, Records cur, ResWork w) {
+    const uint32_t s = blockIdx.x * TILE + threadIdx.x;
+    bool mover = false;
+    if (s < g.n_slots) {
+        const float4 p4 = cur.p4[s];
+        const Seg &sg = g.segs[slot_segment(g, cur, s)];
+        const uint32_t ko = cur.key[s];
+        const uint32_t kn = bucket_of_cell(sg, cell_coord(p4.x, sg.inv_cell), cell_coord(p4.y, sg.inv_cell),
+                                           cell_coord(p4.z, sg.inv_cell));
+        if (kn != ko) {
+            count_mover(w, s, kn, ko);
+            mover = true;
+        }
+    }
+    write_mover_bits(g, w, s, mover);
+}
+
+hipError_t launch_resident_account(const Geometry &g, const Pools &p, const Records &cur, const ResWork &w,
+                                   hipStream_t s) {
+    (void)p;
+    if (g.n_slots == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_r_account, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, cur, w);
+    return hipGetLastError();
 }
 
 hipError_t launch_resident_enter(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s) {

@@ -121,7 +121,31 @@ struct ResWork {
     uint32_t *abase;      // per bucket: first slot of its arrivals in the new layout
     uint32_t *tsum;       // per RES_TILE buckets: their count total, kept from step to step
     uint32_t *tsum2;      // per RES_SUPER tiles: their total, summed by the rebuild (large layouts only)
+    // Mover lists (FH_FLAG_LAZY): between rebuilds the records stay in the
+    // buckets of the last rebuild, flown in place, and every bird whose cell
+    // now hashes to another bucket than its slot's (`key`) is also in a list
+    // of its current bucket & mmask: head[m] is its last mover (MOVER_NIL:
+    // none), next[] links the rest. Entries are indexed by slot; an entry's
+    // p4 holds the position with the slot in w (the bird's identity in a
+    // walk), v4 the velocity. A step reads the lists its previous step wrote
+    // (mr_*; no lists when mr_head is null), writes the next step's (mw_*),
+    // and clears the head arrays that a later step of the period writes
+    // (mc_head).
+    const uint32_t *mr_head, *mr_next;
+    const float4 *mr_p4, *mr_v4;
+    uint32_t *mw_head, *mw_next;
+    float4 *mw_p4, *mw_v4;
+    uint32_t *mc_head[2];
+    uint32_t mmask;     // list heads - 1 (a power of two)
+    // The clock of a lazy flock: the step reads it from mclock_now (its due
+    // test) and writes the next step's to mclock_next; a step without a
+    // rebuild also commits it to the flock's segment, mtick (a rebuild's
+    // scan does that otherwise).
+    const float *mclock_now;
+    float *mclock_next;
+    Seg *mtick;
 };
+constexpr uint32_t MOVER_NIL = 0xffffffffu;
 
 // Up to RES_DIRECT_TILES tiles, each workgroup of the rebuild scan adds up the
 // totals of the tiles before it itself (at most 8 loads per lane, one launch).
@@ -184,6 +208,8 @@ constexpr int WALK_PERSIST = 4;  // a fixed grid of workgroups takes chunks from
 constexpr int WALK_XCD = 8;      // each XCD's workgroups take one contiguous run of chunks
 constexpr int WALK_WINDOW = 16;  // stage the positions of the chunk and a halo in LDS
 constexpr int WALK_FLIGHT = 32;  // resident: fly each chunk's birds right after deciding them
+constexpr int WALK_LAZY = 64;    // fused flight of a step without a rebuild: movers go to the mover lists
+constexpr int WALK_MOVERS = 128;  // the walks also visit the mover lists (FH_FLAG_LAZY; resident, fused flight)
 // decide's LDS window: the chunk and a halo of half a chunk on each side, at
 // most WINDOW_MAX slots and within DECIDE_LDS_MAX bytes of dynamic LDS together
 // with the chunk's due-slot list (32 bytes per slot: position and velocity).
@@ -254,6 +280,12 @@ hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Reco
                                    const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, Seg *tick,
                                    uint32_t ntick, hipStream_t s);
 hipError_t launch_resident_flush(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s);
+// The movers of the current records `cur` against the buckets of their slots,
+// counted as a fused flight counts them at a rebuild (arrival ranks, counts,
+// tile totals, mover bits), without flying: with launch_resident_rebuild
+// after it (no tick), the layout of the current positions.
+hipError_t launch_resident_account(const Geometry &g, const Pools &p, const Records &cur, const ResWork &w,
+                                   hipStream_t s);
 // The flush of the slots [slot_lo, slot_lo + count) only (one flock of a batch).
 hipError_t launch_resident_flush_range(const Geometry &g, const Pools &p, const Records &rec, uint32_t slot_lo,
                                        uint32_t count, hipStream_t s);
@@ -262,6 +294,8 @@ hipError_t launch_integrate(const Geometry &g, const Pools &p, bool hash, bool w
 hipError_t launch_neighbours(const Geometry &g, const Pools &p, uint32_t bird, int walk,
                              uint32_t *ids, uint32_t cap, uint32_t *count, hipStream_t s);
 hipError_t launch_set_time(Seg *segs, uint32_t nseg, float time, float dt, hipStream_t s);
+// *clock = time (a lazy flock's clock, before a run of steps).
+hipError_t launch_set_clock(float *clock, float time, hipStream_t s);
 
 // Resource usage of the kernels, for reports (name: "decide", "integrate",
 // "scatter", "hash_count", "scan").

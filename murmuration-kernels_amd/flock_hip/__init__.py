@@ -32,6 +32,8 @@ FLAG_PERSISTENT = 256
 FLAG_XCD = 512
 FLAG_WINDOW = 1024
 FLAG_FLIGHT = 2048
+FLAG_LAZY = 4096
+LAZY_PERIOD = 4  # steps per rebuild with FLAG_LAZY
 
 OK = 0
 ERR_INVALID, ERR_FLK, ERR_NOMEM, ERR_HIP, ERR_BUDGET, ERR_STATE, ERR_IO, ERR_NODEVICE = range(1, 9)

@@ -26,7 +26,7 @@ import time
 
 import numpy as np
 
-from . import (FLAG_CANONICAL, FLAG_FLIGHT, FLAG_GRAPH, FLAG_OCC, FLAG_PERSISTENT, FLAG_RESIDENT, FLAG_SPATIAL,
+from . import (FLAG_CANONICAL, FLAG_FLIGHT, FLAG_GRAPH, FLAG_LAZY, FLAG_OCC, FLAG_PERSISTENT, FLAG_RESIDENT, FLAG_SPATIAL,
                FLAG_WINDOW, FLAG_XCD, Batch, Flock, device_info)
 
 BPT = (1, 2, 4, 8, 16)
@@ -36,11 +36,11 @@ CELLS = (4.0, 3.0, 5.0, 6.0, 8.0, 10.0, 12.0)
 SHIFTS = (0, 1, 2, 3, 4)
 # on/off tunables and the flag each one sets
 SWITCHES = {"occ": FLAG_OCC, "resident": FLAG_RESIDENT, "spatial": FLAG_SPATIAL, "persist": FLAG_PERSISTENT,
-            "xcd": FLAG_XCD, "window": FLAG_WINDOW, "flight": FLAG_FLIGHT}
+            "xcd": FLAG_XCD, "window": FLAG_WINDOW, "flight": FLAG_FLIGHT, "lazy": FLAG_LAZY}
 # Part of every cache key: raised whenever the searched space or the kernels'
 # costs change, so an entry found for other kernels or a smaller space is not
 # reused.
-SEARCH_VERSION = 11
+SEARCH_VERSION = 12
 FINALISTS = 6
 CONFIRM = 3
 
@@ -121,12 +121,14 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
         f = flags
         for name, bit in SWITCHES.items():
             f |= bit if prm.pop(name) else 0
+        if f & FLAG_LAZY and (kind == "batch" or not (f & FLAG_RESIDENT and f & FLAG_FLIGHT) or f & FLAG_WINDOW):
+            return float("inf")  # lazy rebuilds: single flocks, resident layout, fused flight, no window
         if kind == "batch":
             return _time_batch(starts, times, prm, f, warmup, steps)
         return _time_plain(starts[0], times[0], prm, f, warmup, steps)
 
     best = {"cell": float(cells[0]), "decide_block": 256, "birds_per_thread": 8, "decide_group": 1, "bucket_shift": 0,
-            "occ": 0, "resident": 0, "spatial": 0, "persist": 0, "xcd": 0, "window": 0, "flight": 0}
+            "occ": 0, "resident": 0, "spatial": 0, "persist": 0, "xcd": 0, "window": 0, "flight": 0, "lazy": 0}
     trials = []
     seen = {}
 
@@ -163,10 +165,12 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
             sweep([nv])
     # the resident layout keeps no canonical order
     resident = (0, 1) if not (flags & FLAG_CANONICAL) else (0,)
-    # the fused flight exists only in the resident layout
+    # the fused flight exists only in the resident layout, lazy rebuilds only
+    # with the fused flight in a single flock
+    lazy = resident if kind == "plain" else (0,)
     for nv in (("resident", resident), ("spatial", spatial), ("cell", cells), ("bucket_shift", shifts),
                ("occ", (0, 1)), ("persist", (0, 1)), ("xcd", (0, 1)), ("window", (0, 1)), ("flight", resident),
-               ("cell", cells)):
+               ("lazy", lazy), ("cell", cells)):
         sweep([nv])
     if thorough:
         sweep(shape[:2])
@@ -194,6 +198,9 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
            "trials": trials, "signature": key}
     if cache_path:
         # the cache keeps the chosen point; the trial log is returned only
+        # entries of an older search space are dropped: their points were
+        # found among fewer choices or for other kernels
+        cache = {k: v for k, v in cache.items() if k.startswith(f"v{SEARCH_VERSION}|")}
         cache[key] = {"params": params, "flags": out_flags, "ms_per_step": best_t * 1e3, "signature": key,
                       "trial_count": len(trials), "confirmed": [c[0] * 1e3 for c in sorted(confirmed, key=lambda c: c[0])]}
         os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)

@@ -132,7 +132,9 @@ def test_device_calls_fail_loudly_without_a_gpu(fh):
 
 
 @pytest.mark.parametrize("kw", [dict(decide_group=32), dict(decide_group=3), dict(decide_block=100),
-                                dict(birds_per_thread=65), dict(bucket_shift=9), dict(buckets=1000)])
+                                dict(birds_per_thread=65), dict(bucket_shift=9), dict(buckets=1000),
+                                dict(flags=4096 | 64), dict(flags=4096 | 2048), dict(flags=4096 | 2048 | 64 | 1024),
+                                dict(flags=4096 | 2048 | 64 | 2)])
 def test_bad_params_are_rejected_before_the_device(fh, kw):
     # Parameter validation runs before any device call, so it reports
     # FH_ERR_INVALID on any machine.

@@ -91,13 +91,13 @@ def test_resident_c1_20_steps(fh, fr):
     assert drift <= 1e-3
 
 
-@pytest.mark.parametrize("group,extra", [(2, 32), (4, 0)])
+@pytest.mark.parametrize("group,extra", [(2, 32), (4, 0), (2, 4096)])
 def test_resident_parity_ladder_2000(fh, fr, group, extra):
     # The ladder of scripts/parity.sh:23-34 (docs/SPEC.md:66-72) on the path of
     # the tuned 1M-bird step: resident layout, fused flight, spatial hash,
-    # lane groups of 2 with the occupancy bits (and of 4). Same bars as the
-    # canonical ladder: pointwise 1e-3 up to 30 steps, at most 2 outliers
-    # under 1e-2 at 100, aggregates at 1000.
+    # lane groups of 2 with the occupancy bits (and of 4, and of 2 with lazy
+    # rebuilds). Same bars as the canonical ladder: pointwise 1e-3 up to 30
+    # steps, at most 2 outliers under 1e-2 at 100, aggregates at 1000.
     w0 = fh.gen_reference(2000, 1)
     flags = fh.FLAG_RESIDENT | fh.FLAG_FLIGHT | fh.FLAG_SPATIAL | fh.FLAG_GRAPH | extra
     for k in (1, 10, 30, 100, 1000):
