@@ -16,7 +16,7 @@ static int usage() {
     std::fprintf(stderr,
                  "usage: flock_hip --in FILE [--steps K] [--warmup W] [--out FILE]\n"
                  "       [--canonical] [--unfused] [--graph] [--resident] [--occ] [--spatial]\n"
-                 "       [--persistent] [--xcd] [--window] [--flight]\n"
+                 "       [--persistent] [--xcd] [--window] [--flight] [--lazy]\n"
                  "       [--cell M] [--block B] [--bpt K] [--group G] [--shift S]\n"
                  "(the first W steps are not timed)\n");
     return 2;
@@ -60,6 +60,7 @@ int main(int argc, char **argv) {
         else if (a == "--xcd") prm.flags |= FH_FLAG_XCD;
         else if (a == "--window") prm.flags |= FH_FLAG_WINDOW;
         else if (a == "--flight") prm.flags |= FH_FLAG_FLIGHT;
+        else if (a == "--lazy") prm.flags |= FH_FLAG_LAZY;
         else return usage();
     }
     if (!in) {
