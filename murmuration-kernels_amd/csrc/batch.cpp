@@ -1225,7 +1225,7 @@ size_t fh_flock_bytes(uint32_t n, const fh_params *params) {
     if (prm.flags & FH_FLAG_LAZY) {  // two sets of mover list entries, and 4 arrays of list heads
         size_t mb = 4096;
         while (mb < birds / 8 && mb < (1u << 20)) mb <<= 1;
-        bytes += birds * 2 * 36 + mb * 4 * 4 + 64;
+        bytes += birds * 2 * 36 + mb * 4 * 20 + 64;
     }
     return bytes;
 }

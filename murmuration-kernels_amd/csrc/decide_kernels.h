@@ -163,7 +163,7 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
         int32_t cx[PF], cy[PF], cz[PF];
         uint32_t s0[PF], s1[PF], bk[PF];
         bool want[PF];
-        [[maybe_unused]] uint32_t mh[PF];
+        [[maybe_unused]] float4 mh[PF];
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             cx[k] = x;
@@ -173,7 +173,9 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
             s1[k] = 0u;
             want[k] = base + k * G < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2);
             bk[k] = bucket_of_cell(sg, x, y, z);
-            if constexpr (MG) mh[k] = want[k] && mw.mr_head ? mw.mr_head[bk[k] & mw.mmask] : MOVER_NIL;
+            if constexpr (MG)
+                mh[k] = want[k] && mw.mr_head4 ? mw.mr_head4[bk[k] & mw.mmask]
+                                               : make_float4(0.f, 0.f, 0.f, __uint_as_float(MOVER_NIL));
 #pragma unroll
             for (int s = 0; s < G; ++s)
                 if (++x > w.hi[0]) {
@@ -231,15 +233,23 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
                 }
             }
             if constexpr (MG) {
-                const float fx = (float)cx[k], fy = (float)cy[k], fz = (float)cz[k];
-                // a list holds each slot once at most: n_slots entries bound any walk of it
-                uint32_t t = mh[k];
-                for (uint32_t it = 0; t < n_slots && it < n_slots; ++it, t = mw.mr_next[t]) {
-                    const float4 c = mw.mr_p4[t];  // w: the mover's slot
-                    const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
-                    const float d2 = dot3(d, d);
-                    if ((int)(__float_as_uint(c.w) != self) & (int)in_cell(c, sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2))
-                        visit_m(t, d, d2);
+                const uint32_t hw = __float_as_uint(mh[k].w);
+                if (hw != MOVER_NIL) {
+                    const float fx = (float)cx[k], fy = (float)cy[k], fz = (float)cz[k];
+                    uint32_t t = hw & ~MOVER_OVF;  // the head's bird, then the overflow chain
+                    float4 c = mh[k];
+                    uint32_t next = (hw & MOVER_OVF) ? mw.mr_ovf[bk[k] & mw.mmask] : MOVER_NIL;
+                    // a list holds each slot once at most: n_slots entries bound any walk of it
+                    for (uint32_t it = 0; t < n_slots && it < n_slots; ++it) {
+                        const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
+                        const float d2 = dot3(d, d);
+                        if ((int)(t != self) & (int)in_cell(c, sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2))
+                            visit_m(t, d, d2);
+                        t = next;
+                        if (t >= n_slots) break;
+                        c = mw.mr_p4[t];
+                        next = mw.mr_next[t];
+                    }
                 }
             }
         }
@@ -267,7 +277,7 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
         int32_t cx[PF], cy[PF], cz[PF];
         uint32_t s0[PF], s1[PF], bk[PF];
         bool want[PF];
-        [[maybe_unused]] uint32_t mh[PF];
+        [[maybe_unused]] float4 mh[PF];
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             cx[k] = x;
@@ -277,7 +287,9 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
             s1[k] = 0u;
             want[k] = base + k < ncell && cell_meets_sphere(x, y, z, sg.cell, w.p, w.c2);
             bk[k] = bucket_of_cell(sg, x, y, z);
-            if constexpr (MG) mh[k] = want[k] && lane == 0u && mw.mr_head ? mw.mr_head[bk[k] & mw.mmask] : MOVER_NIL;
+            if constexpr (MG)
+                mh[k] = want[k] && lane == 0u && mw.mr_head4 ? mw.mr_head4[bk[k] & mw.mmask]
+                                                             : make_float4(0.f, 0.f, 0.f, __uint_as_float(MOVER_NIL));
             if (++x > w.hi[0]) {
                 x = w.lo[0];
                 if (++y > w.hi[1]) {
@@ -327,14 +339,22 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
                 }
             }
             if constexpr (MG) {
-                const float fx = (float)cx[k], fy = (float)cy[k], fz = (float)cz[k];
-                uint32_t t = mh[k];
-                for (uint32_t it = 0; t < n_slots && it < n_slots; ++it, t = mw.mr_next[t]) {
-                    const float4 c = mw.mr_p4[t];  // w: the mover's slot
-                    const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
-                    const float d2 = dot3(d, d);
-                    if ((int)(__float_as_uint(c.w) != self) & (int)in_cell(c, sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2))
-                        visit(d, d2, mw.mr_v4[t]);
+                const uint32_t hw = __float_as_uint(mh[k].w);
+                if (hw != MOVER_NIL) {
+                    const float fx = (float)cx[k], fy = (float)cy[k], fz = (float)cz[k];
+                    uint32_t t = hw & ~MOVER_OVF;
+                    float4 c = mh[k];
+                    uint32_t next = (hw & MOVER_OVF) ? mw.mr_ovf[bk[k] & mw.mmask] : MOVER_NIL;
+                    for (uint32_t it = 0; t < n_slots && it < n_slots; ++it) {
+                        const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
+                        const float d2 = dot3(d, d);
+                        if ((int)(t != self) & (int)in_cell(c, sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2))
+                            visit(d, d2, mw.mr_v4[t]);
+                        t = next;
+                        if (t >= n_slots) break;
+                        c = mw.mr_p4[t];
+                        next = mw.mr_next[t];
+                    }
                 }
             }
         }
@@ -549,9 +569,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
         }
         // mover lists a later step of the period writes start empty
         for (int j = 0; j < 2; ++j)
-            if (w.mc_head[j])
-                for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i <= w.mmask; i += gridDim.x * BLOCK)
-                    w.mc_head[j][i] = MOVER_NIL;
+            if (w.mc_head4[j])
+                for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i <= w.mmask; i += gridDim.x * BLOCK) {
+                    reinterpret_cast<uint32_t *>(&w.mc_head4[j][i])[3] = MOVER_NIL;
+                    w.mc_ovf[j][i] = MOVER_NIL;
+                }
     }
     const uint32_t n_chunks = (g.n_slots + chunk - 1) / chunk;
     for (uint32_t round = 0;; ++round) {  // one chunk per trip; every lane leaves together

@@ -104,12 +104,24 @@ __device__ __forceinline__ bool fly_loaded_lazy(const Seg &sg, const Records &in
 }
 
 // Appends the mover in slot s to the next step's list of its bucket
-// kn & mmask, as entry s (entries are indexed by slot, so no counter is
-// shared): its new position (its slot in w) and velocity.
+// kn & mmask: into the list's head record if the list is empty (claimed by a
+// compare-and-swap of its slot word), else into the overflow chain, as entry
+// s (entries are indexed by slot, so no counter is shared). Its velocity is
+// entry s's either way.
 __device__ __forceinline__ void append_mover(const ResWork &w, uint32_t s, uint32_t kn, float4 p4, float4 v4) {
-    w.mw_p4[s] = make_float4(p4.x, p4.y, p4.z, __uint_as_float(s));
+    const uint32_t m = kn & w.mmask;
+    float *head = reinterpret_cast<float *>(&w.mw_head4[m]);
+    uint32_t *slot_word = reinterpret_cast<uint32_t *>(head + 3);
+    if (atomicCAS(slot_word, MOVER_NIL, s) == MOVER_NIL) {
+        head[0] = p4.x;
+        head[1] = p4.y;
+        head[2] = p4.z;
+    } else {
+        w.mw_p4[s] = make_float4(p4.x, p4.y, p4.z, __uint_as_float(s));
+        w.mw_next[s] = atomicExch(&w.mw_ovf[m], s);
+        atomicOr(slot_word, MOVER_OVF);
+    }
     w.mw_v4[s] = v4;
-    w.mw_next[s] = atomicExch(&w.mw_head[kn & w.mmask], s);
 }
 
 // The mover bits of a wave whose lanes hold 64 consecutive slots from a

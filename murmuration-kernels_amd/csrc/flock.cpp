@@ -60,7 +60,8 @@ struct fh_flock {
     uint32_t fq = 0;     // 1: the current positions, velocities and roll rates are F's
     uint32_t cpar = 0;   // due-counter parity of the next step (the step count's parity)
     uint32_t mheads = 0;                   // list heads per phase (a power of two)
-    uint32_t *mhead = nullptr;             // LAZY_PERIOD arrays of heads; phase 0's stays empty
+    float4 *mhead4 = nullptr;              // LAZY_PERIOD arrays of list heads (phase 0's stay empty),
+    uint32_t *movf = nullptr;              // then LAZY_PERIOD arrays of overflow heads (one region)
     float *mclock = nullptr;               // the clock by step parity (cpar)
     float4 *mp4[2]{}, *mv4[2]{};           // two sets of list entries (by slot): steps alternate
     uint32_t *mnext[2]{};
@@ -214,7 +215,8 @@ static fh_status alloc_resident(fh_flock *h) {
     const bool lazy = (h->prm.flags & FH_FLAG_LAZY) != 0;
     uint32_t mb = 4096;
     while (lazy && mb < nn / 8 && mb < (1u << 20)) mb <<= 1;
-    const size_t o_mhead = lazy ? a.take<uint32_t>((size_t)LAZY_PERIOD * mb) : 0;
+    const size_t o_mhead = lazy ? a.take<float4>((size_t)LAZY_PERIOD * mb) : 0;
+    const size_t o_movf = lazy ? a.take<uint32_t>((size_t)LAZY_PERIOD * mb) : 0;  // right after: one fill clears both
     const size_t o_mclock = lazy ? a.take<float>(2) : 0;
     size_t o_mp4[2]{}, o_mv4[2]{}, o_mnext[2]{};
     for (int k = 0; lazy && k < 2; ++k) {
@@ -254,7 +256,8 @@ static fh_status alloc_resident(fh_flock *h) {
     if (lazy) {
         h->lazy = true;
         h->mheads = mb;
-        h->mhead = reinterpret_cast<uint32_t *>(base + o_mhead);
+        h->mhead4 = reinterpret_cast<float4 *>(base + o_mhead);
+        h->movf = reinterpret_cast<uint32_t *>(base + o_movf);
         h->mclock = reinterpret_cast<float *>(base + o_mclock);
         for (int k = 0; k < 2; ++k) {
             h->mp4[k] = reinterpret_cast<float4 *>(base + o_mp4[k]);
@@ -283,7 +286,7 @@ static fh_status enter_resident(fh_flock *h) {
     // parity 0's counters may hold a step taken before the last leave
     FH_HIP(launch_fill(h->res_due, 4, 0u, h->stream));
     if (h->lazy) {
-        FH_HIP(launch_fill(h->mhead, LAZY_PERIOD * h->mheads, MOVER_NIL, h->stream));
+        FH_HIP(launch_fill(reinterpret_cast<uint32_t *>(h->mhead4), 5u * LAZY_PERIOD * h->mheads, MOVER_NIL, h->stream));
     }
     h->phase = 0;
     h->fq = 0;
@@ -319,7 +322,7 @@ static fh_status settle_lazy(fh_flock *h) {
     FH_HIP(launch_resident_account(h->geo, p, cur, h->W, h->stream));
     FH_HIP(launch_resident_rebuild(h->geo, p, cur, h->R[1u - q], h->W, h->off[q], h->off[1u - q], nullptr, 0u,
                                    h->stream));
-    FH_HIP(launch_fill(h->mhead, LAZY_PERIOD * h->mheads, MOVER_NIL, h->stream));
+    FH_HIP(launch_fill(reinterpret_cast<uint32_t *>(h->mhead4), 5u * LAZY_PERIOD * h->mheads, MOVER_NIL, h->stream));
     h->parity = 1u - q;
     h->fq = 0;
     h->phase = 0;
@@ -381,22 +384,30 @@ static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
 static ResWork lazy_work(const fh_flock *h, uint32_t phase) {
     ResWork w = h->W;
     const uint32_t nh = h->mheads;
-    auto head = [&](uint32_t k) { return h->mhead + (size_t)k * nh; };
+    auto head = [&](uint32_t k) { return h->mhead4 + (size_t)k * nh; };
+    auto ovf = [&](uint32_t k) { return h->movf + (size_t)k * nh; };
     const uint32_t rd = (phase + 1u) % 2u, wr = phase % 2u;
-    w.mr_head = phase ? head(phase) : nullptr;  // the first step after a rebuild has no movers to visit
+    w.mr_head4 = phase ? head(phase) : nullptr;  // the first step after a rebuild has no movers to visit
+    w.mr_ovf = ovf(phase);
     w.mr_next = h->mnext[rd];
     w.mr_p4 = h->mp4[rd];
     w.mr_v4 = h->mv4[rd];
     const bool writes = phase + 1u < LAZY_PERIOD;
-    w.mw_head = writes ? head(phase + 1u) : nullptr;
+    w.mw_head4 = writes ? head(phase + 1u) : nullptr;
+    w.mw_ovf = writes ? ovf(phase + 1u) : nullptr;
     w.mw_next = h->mnext[wr];
     w.mw_p4 = h->mp4[wr];
     w.mw_v4 = h->mv4[wr];
-    w.mc_head[0] = w.mc_head[1] = nullptr;
+    w.mc_head4[0] = w.mc_head4[1] = nullptr;
+    w.mc_ovf[0] = w.mc_ovf[1] = nullptr;
     if (phase == 0) {
-        w.mc_head[0] = head(LAZY_PERIOD - 1u);
+        w.mc_head4[0] = head(LAZY_PERIOD - 1u);
+        w.mc_ovf[0] = ovf(LAZY_PERIOD - 1u);
     } else if (phase + 1u == LAZY_PERIOD) {
-        for (uint32_t k = 1; k + 1u < LAZY_PERIOD; ++k) w.mc_head[k - 1u] = head(k);  // phases 1 and 2
+        for (uint32_t k = 1; k + 1u < LAZY_PERIOD; ++k) {  // phases 1 and 2
+            w.mc_head4[k - 1u] = head(k);
+            w.mc_ovf[k - 1u] = ovf(k);
+        }
     }
     w.mmask = nh - 1u;
     w.mclock_now = h->mclock + h->cpar;

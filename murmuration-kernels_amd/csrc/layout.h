@@ -123,20 +123,24 @@ struct ResWork {
     uint32_t *tsum2;      // per RES_SUPER tiles: their total, summed by the rebuild (large layouts only)
     // Mover lists (FH_FLAG_LAZY): between rebuilds the records stay in the
     // buckets of the last rebuild, flown in place, and every bird whose cell
-    // now hashes to another bucket than its slot's (`key`) is also in a list
-    // of its current bucket & mmask: head[m] is its last mover (MOVER_NIL:
-    // none), next[] links the rest. Entries are indexed by slot; an entry's
-    // p4 holds the position with the slot in w (the bird's identity in a
-    // walk), v4 the velocity. A step reads the lists its previous step wrote
-    // (mr_*; no lists when mr_head is null), writes the next step's (mw_*),
-    // and clears the head arrays that a later step of the period writes
-    // (mc_head).
-    const uint32_t *mr_head, *mr_next;
+    // now hashes to another bucket than its slot's (`key`) is also listed by
+    // its current bucket & mmask. The first bird of a list sits in the list's
+    // head record itself, head4[m] = (position, slot in w), so a walk reads it
+    // with the head; w = MOVER_NIL: empty list; MOVER_OVF set in w: more birds
+    // follow from ovf[m] through next[] (entries indexed by slot: position
+    // with the slot in w, p4; velocity, v4, also for the head's bird). A step
+    // reads the lists its previous step wrote (mr_*; none when mr_head4 is
+    // null), writes the next step's (mw_*), and clears the heads that a later
+    // step of the period writes (mc_*).
+    const float4 *mr_head4;
+    const uint32_t *mr_ovf, *mr_next;
     const float4 *mr_p4, *mr_v4;
-    uint32_t *mw_head, *mw_next;
+    float4 *mw_head4;
+    uint32_t *mw_ovf, *mw_next;
     float4 *mw_p4, *mw_v4;
-    uint32_t *mc_head[2];
-    uint32_t mmask;     // list heads - 1 (a power of two)
+    float4 *mc_head4[2];
+    uint32_t *mc_ovf[2];
+    uint32_t mmask;     // lists - 1 (a power of two)
     // The clock of a lazy flock: the step reads it from mclock_now (its due
     // test) and writes the next step's to mclock_next; a step without a
     // rebuild also commits it to the flock's segment, mtick (a rebuild's
@@ -146,6 +150,7 @@ struct ResWork {
     Seg *mtick;
 };
 constexpr uint32_t MOVER_NIL = 0xffffffffu;
+constexpr uint32_t MOVER_OVF = 0x80000000u;  // in a head's slot word: the list continues at ovf
 
 // Up to RES_DIRECT_TILES tiles, each workgroup of the rebuild scan adds up the
 // totals of the tiles before it itself (at most 8 loads per lane, one launch).
