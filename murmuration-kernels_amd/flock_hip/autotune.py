@@ -26,7 +26,7 @@ import time
 
 import numpy as np
 
-from . import (FLAG_CANONICAL, FLAG_FLIGHT, FLAG_GRAPH, FLAG_LAZY, FLAG_OCC, FLAG_PERSISTENT, FLAG_RESIDENT, FLAG_SPATIAL,
+from . import (FLAG_CANONICAL, FLAG_FLIGHT, FLAG_GRAPH, FLAG_LAZY, FLAG_OCC, FLAG_PERSISTENT, FLAG_RESIDENT, FLAG_SPATIAL, FLAG_UNFUSED,
                FLAG_WINDOW, FLAG_XCD, Batch, Flock, device_info)
 
 BPT = (1, 2, 4, 8, 16)
@@ -121,7 +121,8 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
         f = flags
         for name, bit in SWITCHES.items():
             f |= bit if prm.pop(name) else 0
-        if f & FLAG_LAZY and (kind == "batch" or not (f & FLAG_RESIDENT and f & FLAG_FLIGHT) or f & FLAG_WINDOW):
+        if f & FLAG_LAZY and (kind == "batch" or not (f & FLAG_RESIDENT and f & FLAG_FLIGHT)
+                              or f & (FLAG_WINDOW | FLAG_UNFUSED)):
             return float("inf")  # lazy rebuilds: single flocks, resident layout, fused flight, no window
         if kind == "batch":
             return _time_batch(starts, times, prm, f, warmup, steps)
