@@ -471,7 +471,7 @@ __device__ __forceinline__ void reduce_group(Acc &a) {
 }
 
 // combine (model.rs:95-137, cuda/flock.cu:242-283)
-__device__ __forceinline__ Decision finish(const Acc &a, f3 pos, f3 vel, f3 f_i, float r) {
+__device__ __forceinline__ Decision finish(const Acc &a, f3 pos, float speed, f3 f_i, float r) {
     f3 force = mk3(0.f, 0.f, 0.f);
     if (a.n_all > 0) force = force + a.sep * (W_SEP / (float)a.n_all);
     const float centrality = a.n_g > 0 ? sqrtf(dot3(a.csum, a.csum)) / (float)a.n_g : 0.0f;
@@ -479,7 +479,6 @@ __device__ __forceinline__ Decision finish(const Acc &a, f3 pos, f3 vel, f3 f_i,
     const f3 al = a.n_align > 0 ? a.align / (float)a.n_align - f_i : mk3(0.f, 0.f, 0.f);
     force = force + al * W_ALIGN;
     force = force + mk3(0.f, 0.f, 1.f) * (W_ROOST_V * clampf_((Z0 - pos.z) / H_ROOST_V, -1.0f, 1.0f));
-    const float speed = sqrtf(dot3(vel, vel));
     force = force + f_i * ((MASS / TAU) * (V0 - speed));
     Decision dcs;
     dcs.force = force;
@@ -636,8 +635,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
         const uint32_t k = base + threadIdx.x / G;
         bool active = k < nd;
         uint32_t me = 0, si = 0, slot = 0;
-        f3 pos = mk3(0.f, 0.f, 0.f), vel = pos, f_i = pos;
-        float r = 0.0f;
+        f3 pos = mk3(0.f, 0.f, 0.f), f_i = pos;
+        float r = 0.0f, speed = 0.0f;  // speed: |velocity|
         Acc a;
         acc_zero(a);
         Walk w;
@@ -653,8 +652,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
             } else {
                 si = g.tile_seg ? g.tile_seg[me / TILE] : 0u;
                 pos = mk3(q.x, q.y, q.z);
-                vel = mk3(v.x, v.y, v.z);
-                f_i = forward_of(vel);
+                f_i = forward_and_length(mk3(v.x, v.y, v.z), speed);
                 r = RES ? rec.s4[slot].w : v.w;
                 w = make_walk(pos, r, g.segs[si]);
             }
@@ -738,7 +736,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
         if (G == 1) __syncthreads();  // the union box is reset by the next iteration
         if (G > 1) reduce_group<G>(a);
         if (active && lane_g == 0) {
-            const Decision d = finish(a, pos, vel, f_i, r);
+            const Decision d = finish(a, pos, speed, f_i, r);
             if (RES) {
                 rec.s4[slot] = make_float4(d.force.x, d.force.y, d.force.z, d.radius);
                 rec.p4[slot].w = rec.p4[slot].w + DU;

@@ -14,9 +14,10 @@ __device__ __forceinline__ void integrate_bird(float4 &p4, float4 &v4, const flo
     const float roll = v4.w;
     const f3 steering = mk3(s4.x, s4.y, s4.z);
 
-    const f3 fwd = forward_of(vel);
+    float len;
+    const f3 fwd = forward_and_length(vel, len);
     const f3 side = side_of(fwd);
-    const float speed = fmaxf(sqrtf(dot3(vel, vel)), 1e-6f);
+    const float speed = fmaxf(len, 1e-6f);
 
     const float a_lat = dot3(steering, side) / MASS;
     const float target = clampf_(atanf(a_lat / G), -MAX_BANK, MAX_BANK);

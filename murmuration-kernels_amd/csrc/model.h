@@ -71,6 +71,12 @@ FH_HD f3 forward_of(f3 v) {
     float l2 = dot3(v, v);
     return l2 > 1e-12f ? v * (1.0f / sqrtf(l2)) : mk3(1.f, 0.f, 0.f);
 }
+// forward_of(v), also returning |v| = sqrt(v.v) (the speed), from one square root.
+FH_HD f3 forward_and_length(f3 v, float &len) {
+    const float l2 = dot3(v, v);
+    len = sqrtf(l2);
+    return l2 > 1e-12f ? v * (1.0f / len) : mk3(1.f, 0.f, 0.f);
+}
 FH_HD f3 side_of(f3 f) {
     f3 s = cross3(f, mk3(0.f, 0.f, 1.f));
     float l2 = dot3(s, s);
