@@ -124,14 +124,18 @@ typedef struct fh_params {
  * results equal the unfused resident step's. Ignored without
  * FH_FLAG_RESIDENT. */
 #define FH_FLAG_FLIGHT 2048u
-/* With FH_FLAG_RESIDENT and FH_FLAG_FLIGHT (not FH_FLAG_WINDOW), single
- * flocks only: the layout is rebuilt every 4th step instead of every step.
- * In between, each step flies the records in place, and the birds whose cell
- * now hashes to another bucket than their slot's are also copied into a small
- * second grid that decide's walks visit too, so every bird is found in the
- * bucket of its current cell. Neighbour sets and counts are those of the
- * rebuilt layout; sums are taken in another order (rounding). Any call that
- * reads the state or the grid rebuilds the layout first. */
+/* With FH_FLAG_RESIDENT and FH_FLAG_FLIGHT (not FH_FLAG_WINDOW), for single
+ * flocks and batches: the layout is rebuilt every 4th step instead of every
+ * step. In between, each step flies the records in place, and the birds whose
+ * cell now hashes to another bucket than their slot's are also listed by
+ * their current bucket, in lists that decide's walks visit too, so every bird
+ * is found in the bucket of its current cell (in a batch, only by walks of
+ * its own flock). Neighbour sets and counts are those of the rebuilt layout;
+ * sums are taken in another order (rounding). In a batch, each flock's clock
+ * and step budget between rebuilds follow from its values at the last
+ * rebuild and the steps taken since, and the rebuild commits them. Any call
+ * that reads the state or the grid, or changes a batch's flocks or budgets,
+ * rebuilds the layout first. */
 #define FH_FLAG_LAZY 4096u
 
 /* Per-pass GPU time of fh_step, mean ms per timed step (hipEvents on the

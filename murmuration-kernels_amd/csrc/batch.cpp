@@ -87,11 +87,14 @@ static void refresh_geometry(fh_batch *b) {
     b->graph_valid = false;
 }
 
+static void set_slot_bases(fh_batch *b);
+
 // Uploads the segment table and the tile -> segment entries [t_lo, t_hi).
 static fh_status push_tables(fh_batch *b, uint32_t t_lo, uint32_t t_hi) {
     // The device advances each clock by DT per step; the host mirror in
     // times[] does the same f32 additions, so it is the current value.
     for (size_t f = 0; f < b->segs.size(); ++f) b->segs[f].time = b->times[f];
+    set_slot_bases(b);
     if (!b->segs.empty())
         FH_HIP(hipMemcpyAsync(b->d_segs, b->segs.data(), b->segs.size() * sizeof(Seg), hipMemcpyHostToDevice,
                               b->stream));
@@ -287,6 +290,18 @@ static fh_status batch_alloc_resident(fh_batch *b) {
     const size_t o_fp4 = flight ? a.take<float4>(nn) : 0, o_fv4 = flight ? a.take<float4>(nn) : 0;
     const size_t o_fspin = flight ? a.take<float>(nn) : 0;
     const size_t o_shift = a.take<int32_t>(b->flock_cap ? b->flock_cap : 1);
+    // mover lists (FH_FLAG_LAZY): about one list head per 8 birds, per phase
+    const bool lazy = b->lazy;
+    uint32_t mb = 4096;
+    while (lazy && mb < nn / 8 && mb < (1u << 20)) mb <<= 1;
+    const size_t o_mhead = lazy ? a.take<float4>((size_t)LAZY_PERIOD * mb) : 0;
+    const size_t o_movf = lazy ? a.take<uint32_t>((size_t)LAZY_PERIOD * mb) : 0;  // right after: one fill clears both
+    size_t o_mp4[2]{}, o_mv4[2]{}, o_mnext[2]{};
+    for (int k = 0; lazy && k < 2; ++k) {
+        o_mp4[k] = a.take<float4>(nn);
+        o_mv4[k] = a.take<float4>(nn);
+        o_mnext[k] = a.take<uint32_t>(nn);
+    }
     hipError_t e = hipMalloc(&b->res_arena, a.used);
     if (e != hipSuccess) {
         b->res_arena = nullptr;
@@ -317,9 +332,28 @@ static fh_status batch_alloc_resident(fh_batch *b) {
     b->off[1] = reinterpret_cast<uint32_t *>(base + o_off1);
     b->res_due = reinterpret_cast<uint32_t *>(base + o_due);
     b->d_shift = reinterpret_cast<int32_t *>(base + o_shift);
+    if (lazy) {
+        b->ml.heads = mb;
+        b->ml.head4 = reinterpret_cast<float4 *>(base + o_mhead);
+        b->ml.ovf = reinterpret_cast<uint32_t *>(base + o_movf);
+        for (int k = 0; k < 2; ++k) {
+            b->ml.p4[k] = reinterpret_cast<float4 *>(base + o_mp4[k]);
+            b->ml.v4[k] = reinterpret_cast<float4 *>(base + o_mv4[k]);
+            b->ml.next[k] = reinterpret_cast<uint32_t *>(base + o_mnext[k]);
+        }
+    }
     // arrivals start at zero and every rebuild zeroes them again
     FH_HIP(hipMemsetAsync(b->res_arena, 0, a.used, b->stream));
     FH_HIP(hipStreamSynchronize(b->stream));
+    return FH_OK;
+}
+
+// Every list of a lazy batch empty (at the start of each period after a
+// settle or an entry; the steps of a period clear the lists they reuse).
+static fh_status clear_mover_lists(fh_batch *b) {
+    if (b->lazy && b->ml.head4)
+        FH_HIP(launch_fill(reinterpret_cast<uint32_t *>(b->ml.head4), 5u * LAZY_PERIOD * b->ml.heads, MOVER_NIL,
+                           b->stream));
     return FH_OK;
 }
 
@@ -339,6 +373,11 @@ static fh_status batch_enter_resident(fh_batch *b) {
     b->resident = true;
     b->hashed = false;  // the counts now follow the layout; the ranks are stale
     b->counts_dirty = false;
+    b->phase = b->fq = b->cpar = 0;
+    if (b->lazy) {
+        if ((st = clear_mover_lists(b))) return st;
+        if ((st = push_tables(b, 0, 0))) return st;  // the flocks' slot ranges (the lists need them)
+    }
     return FH_OK;
 }
 
@@ -363,6 +402,13 @@ static std::vector<uint32_t> resident_slot_starts(const fh_batch *b) {
         at += b->segs[i].n;
     }
     return start;
+}
+
+// Each segment's first slot in the resident layout (slot_base; pushed with
+// the segment table).
+static void set_slot_bases(fh_batch *b) {
+    const std::vector<uint32_t> start = resident_slot_starts(b);
+    for (size_t f = 0; f < b->segs.size(); ++f) b->segs[f].slot_base = start[f];
 }
 
 // Splice shifts: the live flocks whose buckets lie after flock f's move by
@@ -461,15 +507,26 @@ static fh_status batch_resident_launches(fh_batch *b, hipEvent_t *ev) {
     return FH_OK;
 }
 
-static fh_status batch_capture_resident(fh_batch *b) {
-    if (!b->rgraph_valid) {
-        for (auto &gx : b->rgraph)
+// Drops the resident step graphs when the geometry changed since they were
+// captured.
+static void drop_stale_resident_graphs(fh_batch *b) {
+    if (b->rgraph_valid) return;
+    for (auto &gx : b->rgraph)
+        if (gx) {
+            (void)hipGraphExecDestroy(gx);
+            gx = nullptr;
+        }
+    for (auto &row : b->lgraph)
+        for (auto &gx : row)
             if (gx) {
                 (void)hipGraphExecDestroy(gx);
                 gx = nullptr;
             }
-        b->rgraph_valid = true;
-    }
+    b->rgraph_valid = true;
+}
+
+static fh_status batch_capture_resident(fh_batch *b) {
+    drop_stale_resident_graphs(b);
     const uint32_t q = b->parity;
     if (b->rgraph[q]) return FH_OK;
     hipGraph_t g = nullptr;
@@ -490,6 +547,116 @@ static fh_status batch_capture_resident(fh_batch *b) {
     return FH_OK;
 }
 
+
+// ---- lazy rebuilds (FH_FLAG_LAZY) ------------------------------------------
+
+// The records of the current lazy step: R[parity]'s, with the positions,
+// velocities and roll rates F's when `f`.
+static Records batch_lazy_view(const fh_batch *b, uint32_t f) {
+    Records r = b->R[b->parity];
+    if (f) {
+        r.p4 = b->F.p4;
+        r.v4 = b->F.v4;
+        r.spin = b->F.spin;
+    }
+    return r;
+}
+
+// One lazy step of every live flock: decide (its walks visit the mover lists
+// too) with the fused flight from the current records into the other set.
+// Each flock's clock and budget follow from the segment table and the phase.
+// Between rebuilds the flight lists its movers for the next step; every
+// LAZY_PERIOD-th step it counts them against the layout instead, and the
+// rebuild follows, committing the period's steps. Events as
+// batch_resident_launches.
+static fh_status batch_lazy_launches(fh_batch *b, hipEvent_t *ev) {
+    const uint32_t q = b->parity, c = b->cpar;
+    const bool rebuild = b->phase + 1u == LAZY_PERIOD;
+    Pools p = b->P;
+    p.offsets = b->off[q];
+    p.due_count = b->res_due + c;
+    p.work = b->res_due + 2 + c;
+    const Records cur = batch_lazy_view(b, b->fq), flown = batch_lazy_view(b, b->fq ^ 1u);
+    const int wf = walk_flags(b->prm.flags) | WALK_MOVERS | (rebuild ? 0 : WALK_LAZY);
+    ResWork w = with_mover_lists(b->W, b->ml, b->phase);
+    w.mclock_now = nullptr;  // clocks from the segment table and the phase
+    w.mclock_next = nullptr;
+    w.mtick = nullptr;
+    w.mphase = b->phase;
+    if (ev) FH_HIP(hipEventRecord(ev[0], b->stream));
+    FH_HIP(launch_decide_resident(b->geo, p, cur, b->prm.decide_block, b->prm.birds_per_thread, b->prm.decide_group,
+                                  wf, b->res_due + (1u - c), b->res_due + 2 + (1u - c), w, flown, b->stream,
+                                  ev ? DecideTiming{ev[6], ev[7]} : DecideTiming{}));
+    if (ev) {
+        FH_HIP(hipEventRecord(ev[1], b->stream));
+        FH_HIP(hipEventRecord(ev[2], b->stream));
+    }
+    if (rebuild) {
+        FH_HIP(launch_resident_rebuild(b->geo, p, flown, b->R[1u - q], b->W, b->off[q], b->off[1u - q], b->d_segs,
+                                       (uint32_t)b->segs.size(), b->stream, LAZY_PERIOD));
+        b->parity = 1u - q;
+        b->fq = 0;
+        b->phase = 0;
+    } else {
+        b->fq ^= 1u;
+        b->phase += 1u;
+    }
+    b->cpar = 1u - c;
+    if (ev)
+        for (int k = 3; k < 6; ++k) FH_HIP(hipEventRecord(ev[k], b->stream));
+    return FH_OK;
+}
+
+// A lazy batch between rebuilds -> the layout of the current positions: the
+// movers counted against their slots' buckets without a flight, then the
+// rebuild, which commits the steps of the period so far; every list empty.
+// Nothing to do at a rebuild boundary.
+static fh_status batch_settle(fh_batch *b) {
+    if (!b->lazy || !b->resident || b->phase == 0) return FH_OK;
+    const uint32_t q = b->parity;
+    Pools p = b->P;
+    p.offsets = b->off[q];
+    const Records cur = batch_lazy_view(b, b->fq);
+    FH_HIP(launch_resident_account(b->geo, p, cur, b->W, b->stream));
+    FH_HIP(launch_resident_rebuild(b->geo, p, cur, b->R[1u - q], b->W, b->off[q], b->off[1u - q], b->d_segs,
+                                   (uint32_t)b->segs.size(), b->stream, b->phase));
+    b->parity = 1u - q;
+    b->fq = 0;
+    b->phase = 0;
+    return clear_mover_lists(b);
+}
+
+// LAZY_PERIOD lazy steps from phase 0 as one graph, captured once per layout
+// parity and counter parity (the host state is restored after the capture;
+// the caller replays it and applies its effect).
+static fh_status batch_capture_lazy(fh_batch *b) {
+    drop_stale_resident_graphs(b);
+    const uint32_t q = b->parity, c = b->cpar;
+    hipGraphExec_t *out = &b->lgraph[q][c];
+    if (*out) return FH_OK;
+    hipGraph_t g = nullptr;
+    FH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    fh_status st = FH_OK;
+    for (uint32_t k = 0; k < LAZY_PERIOD && !st; ++k) st = batch_lazy_launches(b, nullptr);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    b->parity = q;
+    b->cpar = c;
+    b->phase = 0;
+    b->fq = 0;
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    return FH_OK;
+}
+
 namespace fh {
 
 size_t resident_bytes_per_bird(bool flight) {
@@ -500,14 +667,22 @@ size_t resident_bytes_per_bucket() { return 4 /*arr*/ + 4 /*abase*/ + 4 /*offset
 
 // One step of every live flock (launched, not waited for), with per-pass
 // events when `ev` is given.
-static fh_status one_step(fh_batch *b, bool graph, hipEvent_t *ev) {
-    for (size_t f = 0; f < b->segs.size(); ++f)  // mirrors the device clock add (and budget)
+// The host mirror of one step: each stepping flock's clock advances by DT
+// (one f32 addition, as on the device) and its budget by one step.
+static void mirror_step(fh_batch *b) {
+    for (size_t f = 0; f < b->segs.size(); ++f)
         if (in_layout(b, (uint32_t)f) && (!b->resident || b->segs[f].left)) {
             b->times[f] += DT;
             if (b->resident && b->segs[f].left != SEG_UNLIMITED) b->segs[f].left -= 1u;
         }
+}
+
+static fh_status one_step(fh_batch *b, bool graph, hipEvent_t *ev) {
+    mirror_step(b);
     fh_status st = FH_OK;
-    if (b->resident) {
+    if (b->resident && b->lazy) {
+        st = batch_lazy_launches(b, ev);  // whole periods replay as graphs (lazy_period)
+    } else if (b->resident) {
         if (graph) {
             if ((st = batch_capture_resident(b))) return st;
             hipError_t e = hipGraphLaunch(b->rgraph[b->parity], b->stream);
@@ -527,6 +702,25 @@ static fh_status one_step(fh_batch *b, bool graph, hipEvent_t *ev) {
         st = batch_step_launches(b, ev);
     }
     return st;
+}
+
+// LAZY_PERIOD steps of a lazy batch at phase 0 as one graph replay.
+static fh_status lazy_period(fh_batch *b) {
+    fh_status st = batch_capture_lazy(b);
+    if (st) return st;
+    hipError_t e = hipGraphLaunch(b->lgraph[b->parity][b->cpar], b->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    for (uint32_t k = 0; k < LAZY_PERIOD; ++k) mirror_step(b);
+    b->parity ^= 1u;  // one rebuild per period; the counter parity comes back
+    return FH_OK;
+}
+
+// Whether the next LAZY_PERIOD of `steps` steps (from step s, the first
+// `warmup` untimed) replay as one graph: a lazy batch at phase 0, graphs on,
+// no per-pass timing, and the period on one side of the warm-up boundary.
+static bool period_fits(const fh_batch *b, bool graph, bool per_pass, uint32_t s, uint32_t steps, uint32_t warmup) {
+    return graph && !per_pass && b->resident && b->lazy && b->phase == 0 && s + LAZY_PERIOD <= steps &&
+           (s >= warmup || s + LAZY_PERIOD <= warmup);
 }
 
 fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *out) {
@@ -549,13 +743,19 @@ fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *o
     };
     fh_status st = FH_OK;
     std::chrono::steady_clock::time_point t0{}, t1{};
-    for (uint32_t s = 0; s < steps && st == FH_OK; ++s) {
+    for (uint32_t s = 0; s < steps && st == FH_OK;) {
         if (s == warmup) {
             (void)hipStreamSynchronize(b->stream);
             t0 = std::chrono::steady_clock::now();
             if (out && graph) (void)hipEventRecord(ev[0], b->stream);
         }
-        st = one_step(b, graph, (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 8] : nullptr);
+        if (period_fits(b, graph, per_pass, s, steps, warmup)) {
+            st = lazy_period(b);
+            s += LAZY_PERIOD;
+        } else {
+            st = one_step(b, graph, (per_pass && s >= warmup) ? &ev[(size_t)(s - warmup) * 8] : nullptr);
+            s += 1;
+        }
     }
     if (st) {
         (void)hipStreamSynchronize(b->stream);
@@ -603,8 +803,10 @@ fh_status batch_step(fh_batch *b, uint32_t steps, uint32_t warmup, fh_pass_ms *o
             }
         }
         uint32_t due = 0;
-        FH_HIP(hipMemcpy(&due, b->resident ? b->res_due + (b->parity ^ 1u) : b->P.due_count, 4,
-                         hipMemcpyDeviceToHost));
+        const uint32_t *last = !b->resident ? b->P.due_count
+                               : b->lazy    ? b->res_due + (b->cpar ^ 1u)
+                                            : b->res_due + (b->parity ^ 1u);
+        FH_HIP(hipMemcpy(&due, last, 4, hipMemcpyDeviceToHost));
         out->due_last = due;
     }
     destroy();
@@ -724,6 +926,8 @@ fh_status batch_leave_async(fh_batch *b, uint32_t fid, void *dst, float *time, h
         return set_error(FH_ERR_INVALID, "batch_leave_async: no flock %u in the layout", fid);
     if (!b->resident) return set_error(FH_ERR_STATE, "batch_leave_async: the batch has not stepped");
     if (!dst) return set_error(FH_ERR_INVALID, "batch_leave_async: NULL destination");
+    fh_status st = batch_settle(b);
+    if (st) return st;
     const Seg sg = b->segs[fid];
     FH_HIP(launch_resident_flush_range(b->geo, b->P, b->R[b->parity], resident_slot_starts(b)[fid], sg.n,
                                        b->stream));
@@ -764,6 +968,7 @@ static fh_status table_staging(fh_batch *b, Seg **sa, Seg **sb, uint32_t **ta, u
 // Uploads the segment table and the tile -> segment entries of the given
 // segments' ranges from pinned staging `seg_st` / `tile_st` (no host wait).
 static fh_status push_tables_async(fh_batch *b, Seg *seg_st, uint32_t *tile_st, const std::vector<uint32_t> &fs) {
+    set_slot_bases(b);
     for (size_t f = 0; f < b->segs.size(); ++f) {
         seg_st[f] = b->segs[f];
         seg_st[f].time = b->times[f];
@@ -780,6 +985,8 @@ static fh_status push_tables_async(fh_batch *b, Seg *seg_st, uint32_t *tile_st, 
 
 fh_status batch_apply(fh_batch *b) {
     if (b->joins.empty() && b->leaves.empty()) return FH_OK;
+    fh_status st0 = batch_settle(b);
+    if (st0) return st0;
     if (b->uploads) {
         FH_HIP(hipStreamWaitEvent(b->stream, b->up_done, 0));
         b->uploads = false;
@@ -877,8 +1084,15 @@ fh_status batch_step_async(fh_batch *b, uint32_t steps) {
         fh_status st = batch_enter_resident(b);
         if (st) return st;
     }
-    for (uint32_t k = 0; k < steps; ++k) {
-        fh_status st = one_step(b, graph, nullptr);
+    for (uint32_t k = 0; k < steps;) {
+        fh_status st;
+        if (period_fits(b, graph, false, k, steps, 0)) {
+            st = lazy_period(b);
+            k += LAZY_PERIOD;
+        } else {
+            st = one_step(b, graph, nullptr);
+            k += 1;
+        }
         if (st) return st;
     }
     return FH_OK;
@@ -915,12 +1129,12 @@ fh_status fh_batch_create(uint32_t bird_capacity, uint32_t bucket_capacity, uint
     fh_params prm;
     fh_status st = resolve_params(params, &prm, "fh_batch_create");
     if (st) return st;
-    if (prm.flags & FH_FLAG_LAZY) return set_error(FH_ERR_INVALID, "fh_batch_create: FH_FLAG_LAZY is for single flocks");
     st = require_device();
     if (st) return st;
 
     fh_batch *b = new fh_batch();
     b->prm = prm;
+    b->lazy = (prm.flags & FH_FLAG_LAZY) != 0;  // with FH_FLAG_RESIDENT and FH_FLAG_FLIGHT (resolve_params)
     const uint32_t tile_cap = (bird_capacity + TILE - 1) / TILE;
     b->bird_cap = tile_cap * TILE;
     b->bucket_cap_units = (bucket_capacity + SCAN_TILE - 1) / SCAN_TILE;
@@ -1009,6 +1223,9 @@ fh_status fh_batch_destroy(fh_batch *b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (auto &gx : b->rgraph)
         if (gx) (void)hipGraphExecDestroy(gx);
+    for (auto &row : b->lgraph)
+        for (auto &gx : row)
+            if (gx) (void)hipGraphExecDestroy(gx);
     if (b->copy) (void)hipStreamSynchronize(b->copy);
     if (b->res_arena) (void)hipFree(b->res_arena);
     if (b->pin) (void)hipHostFree(b->pin);
@@ -1028,6 +1245,7 @@ fh_status fh_batch_add(fh_batch *b, const float *words, uint32_t n, float time, 
     if (st) return st;
     if (!flock_id || (n && !words) || n == 0)
         return set_error(FH_ERR_INVALID, "fh_batch_add: need n > 0, words and flock_id");
+    if ((st = batch_settle(b))) return st;  // a lazy batch: the layout and clocks of the current step
     const uint32_t buckets = flock_buckets(n, b->prm);
     const uint32_t ntile = (n + TILE - 1) / TILE;
     const uint32_t nunit = (buckets + SCAN_TILE - 1) / SCAN_TILE;
@@ -1125,6 +1343,7 @@ fh_status fh_batch_remove(fh_batch *b, uint32_t flock_id) {
     if (st) return st;
     if (flock_id >= b->segs.size() || !b->segs[flock_id].live)
         return set_error(FH_ERR_INVALID, "fh_batch_remove: no live flock %u", flock_id);
+    if ((st = batch_settle(b))) return st;
     const bool splice = b->resident;  // leaves the resident layout in place
     if (splice && (st = resident_leave_one(b, flock_id))) return st;
     Seg &s = b->segs[flock_id];
@@ -1150,6 +1369,7 @@ fh_status fh_batch_download(fh_batch *b, uint32_t flock_id, float *words, float 
     const Seg &s = b->segs[flock_id];
     const uint32_t n = s.n;
     if (!words) return set_error(FH_ERR_INVALID, "fh_batch_download: NULL words");
+    if ((st = batch_settle(b))) return st;
     if (b->resident)  // this flock's slots back to bird order; the layout stays
         FH_HIP(launch_resident_flush_range(b->geo, b->P, b->R[b->parity], resident_slot_starts(b)[flock_id], n,
                                            b->stream));
@@ -1189,6 +1409,7 @@ fh_status fh_batch_limit(fh_batch *b, uint32_t flock_id, uint32_t steps) {
         return set_error(FH_ERR_INVALID, "fh_batch_limit: step budgets need FH_FLAG_RESIDENT");
     if (flock_id >= b->segs.size() || !b->segs[flock_id].live)
         return set_error(FH_ERR_INVALID, "fh_batch_limit: no live flock %u", flock_id);
+    if ((st = batch_settle(b))) return st;  // budgets change only at a rebuild boundary
     b->segs[flock_id].left = steps;
     return push_tables(b, 0, 0);
 }

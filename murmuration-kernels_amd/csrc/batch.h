@@ -93,6 +93,20 @@ struct fh_batch {
     uint32_t parity = 0;
     hipGraphExec_t rgraph[2]{};
     bool rgraph_valid = false;
+    // Lazy rebuilds (FH_FLAG_LAZY), as for one flock: the layout is rebuilt
+    // every LAZY_PERIOD steps; in between each step flies the records in
+    // place (alternating between R[parity]'s positions, velocities and roll
+    // rates and F's) and lists the birds out of their slot's bucket. The
+    // segment table keeps the clocks and budgets of the last rebuild; a lazy
+    // step derives each flock's from them and its phase, and the rebuild
+    // commits the period's steps. Membership, budget and download calls
+    // first rebuild the layout of the current positions (settle).
+    bool lazy = false;
+    uint32_t phase = 0;  // steps since the last rebuild
+    uint32_t fq = 0;     // 1: the current positions, velocities and roll rates are F's
+    uint32_t cpar = 0;   // due-counter parity of the next lazy step
+    fh::MoverLists ml;
+    hipGraphExec_t lgraph[2][2]{};  // one period of lazy steps from phase 0: layout parity, counter parity
     // Pinned host staging for fh_batch_add / fh_batch_download (grown on
     // demand, reused): copies from pageable memory are staged by the runtime
     // and run at a fraction of the link's speed.

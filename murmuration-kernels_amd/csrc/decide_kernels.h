@@ -239,11 +239,14 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
                     uint32_t t = hw & ~MOVER_OVF;  // the head's bird, then the overflow chain
                     float4 c = mh[k];
                     uint32_t next = (hw & MOVER_OVF) ? mw.mr_ovf[bk[k] & mw.mmask] : MOVER_NIL;
-                    // a list holds each slot once at most: n_slots entries bound any walk of it
+                    // a list holds each slot once at most: n_slots entries bound any walk of it.
+                    // Lists are shared by the flocks of a batch: a listed bird counts only for
+                    // walks of its own flock (its slot in the flock's slot range).
                     for (uint32_t it = 0; t < n_slots && it < n_slots; ++it) {
                         const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
                         const float d2 = dot3(d, d);
-                        if ((int)(t != self) & (int)in_cell(c, sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2))
+                        if ((int)(t != self) & (int)in_cell(c, sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2) &
+                            (int)(t - sg.slot_base < sg.n))
                             visit_m(t, d, d2);
                         t = next;
                         if (t >= n_slots) break;
@@ -348,7 +351,8 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
                     for (uint32_t it = 0; t < n_slots && it < n_slots; ++it) {
                         const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
                         const float d2 = dot3(d, d);
-                        if ((int)(t != self) & (int)in_cell(c, sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2))
+                        if ((int)(t != self) & (int)in_cell(c, sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2) &
+                            (int)(t - sg.slot_base < sg.n))
                             visit(d, d2, mw.mr_v4[t]);
                         t = next;
                         if (t >= n_slots) break;
@@ -557,10 +561,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
         if (due_clear) *due_clear = 0;
         if (work_clear) *work_clear = 0;
     }
-    [[maybe_unused]] float now = 0.0f;  // the step's clock (MG: from the lazy clock)
+    [[maybe_unused]] float now = 0.0f;  // the step's clock (one lazy flock: from the lazy clock)
     if constexpr (MG) {
-        now = *w.mclock_now;
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // One lazy flock reads its clock from mclock_now and commits the
+        // next; a batch's flocks take theirs from the segment table (seg_step).
+        if (w.mclock_now) now = *w.mclock_now;
+        if (w.mclock_now && blockIdx.x == 0 && threadIdx.x == 0) {
             // The next step's clock; between rebuilds also the segment's (its
             // clock field is read by no launch of this step).
             *w.mclock_next = now + DT;
@@ -620,7 +626,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
         bool due;
         if (RES) {
             const Seg &sg = g.segs[slot_segment(g, rec, s)];
-            due = sg.left && (MG ? now : sg.time) + DT >= cand_pos<STRIDE, WIN>(cand, s).w;  // frozen flocks never decide
+            if (MG && !w.mclock_now) {  // a batch's lazy step: each flock's clock mphase steps on
+                float t;
+                const bool live = seg_step(sg, w.mphase, t);
+                due = live && t + DT >= cand_pos<STRIDE, WIN>(cand, s).w;
+            } else {
+                due = sg.left && (MG ? now : sg.time) + DT >= cand_pos<STRIDE, WIN>(cand, s).w;  // frozen flocks never decide
+            }
         } else {
             due = time >= p.snext[s];
         }

@@ -63,15 +63,17 @@ __device__ __forceinline__ void count_mover(const ResWork &w, uint32_t s, uint32
 
 // Resident flight of slot s from its loaded records (position and decision
 // time p4, velocity and roll v4, steering and radius s4, old bucket ko):
-// banked flight into out[s] (out may be in), its new bucket by its flock's
-// hash, and the mover accounting. A bird whose new cell falls in another
+// banked flight into out[s] (out may be in) unless its flock does not step
+// (`live` false: frozen), its new bucket by its flock's hash, and the mover
+// accounting. A bird whose new cell falls in another
 // bucket is a mover: it moves its count from its old bucket to its new one,
 // takes an arrival rank there, and moves its count between the scan tiles'
 // totals when the tile changes. Returns whether the bird is a mover.
-__device__ __forceinline__ bool fly_loaded(const Seg &sg, const Records &in, const Records &out, const ResWork &w,
-                                           uint32_t s, float4 p4, float4 v4, const float4 s4, uint32_t ko) {
+__device__ __forceinline__ bool fly_loaded(const Seg &sg, bool live, const Records &in, const Records &out,
+                                           const ResWork &w, uint32_t s, float4 p4, float4 v4, const float4 s4,
+                                           uint32_t ko) {
     float rate;
-    if (sg.left)
+    if (live)
         integrate_bird(p4, v4, s4, rate);
     else
         rate = in.spin[s];  // a frozen flock keeps its state
@@ -89,10 +91,11 @@ __device__ __forceinline__ bool fly_loaded(const Seg &sg, const Records &in, con
 // banked flight into out[s]; returns the bird's new bucket through kn and
 // whether it is a mover (its bucket now differs from its slot's, ko: the
 // grid keeps the buckets of the last rebuild).
-__device__ __forceinline__ bool fly_loaded_lazy(const Seg &sg, const Records &in, const Records &out, uint32_t s,
-                                                float4 &p4, float4 &v4, const float4 s4, uint32_t ko, uint32_t &kn) {
+__device__ __forceinline__ bool fly_loaded_lazy(const Seg &sg, bool live, const Records &in, const Records &out,
+                                                uint32_t s, float4 &p4, float4 &v4, const float4 s4, uint32_t ko,
+                                                uint32_t &kn) {
     float rate;
-    if (sg.left)
+    if (live)
         integrate_bird(p4, v4, s4, rate);
     else
         rate = in.spin[s];
@@ -142,7 +145,7 @@ __device__ __forceinline__ void resident_flight(const Geometry &g, const Records
     bool mover = false;
     if (active) {
         const Seg &sg = g.segs[slot_segment(g, in, s)];
-        mover = fly_loaded(sg, in, out, w, s, in.p4[s], in.v4[s], in.s4[s], in.key[s]);
+        mover = fly_loaded(sg, sg.left != 0u, in, out, w, s, in.p4[s], in.v4[s], in.s4[s], in.key[s]);
     }
     write_mover_bits(g, w, s, mover);
 }
@@ -163,16 +166,19 @@ __device__ __forceinline__ void resident_flight2(const Geometry &g, const Record
     const float4 p0 = in.p4[l0], v0 = in.v4[l0], q0 = in.s4[l0];
     const float4 p1 = in.p4[l1], v1 = in.v4[l1], q1 = in.s4[l1];
     const uint32_t k0 = in.key[l0], k1 = in.key[l1];
+    // a batch's lazy steps: budgets as of mphase steps after the last commit
+    float t_unused;
+    const bool live0 = seg_step(sg0, MG ? w.mphase : 0u, t_unused), live1 = seg_step(sg1, MG ? w.mphase : 0u, t_unused);
     if (MG && lazy) {
         float4 f0 = p0, g0 = v0, f1 = p1, g1 = v1;
         uint32_t n0 = 0, n1 = 0;
-        if (a0 && fly_loaded_lazy(sg0, in, out, s0, f0, g0, q0, k0, n0)) append_mover(w, s0, n0, f0, g0);
-        if (a1 && fly_loaded_lazy(sg1, in, out, s1, f1, g1, q1, k1, n1)) append_mover(w, s1, n1, f1, g1);
+        if (a0 && fly_loaded_lazy(sg0, live0, in, out, s0, f0, g0, q0, k0, n0)) append_mover(w, s0, n0, f0, g0);
+        if (a1 && fly_loaded_lazy(sg1, live1, in, out, s1, f1, g1, q1, k1, n1)) append_mover(w, s1, n1, f1, g1);
         return;
     }
-    const bool m0 = a0 && fly_loaded(sg0, in, out, w, s0, p0, v0, q0, k0);
+    const bool m0 = a0 && fly_loaded(sg0, live0, in, out, w, s0, p0, v0, q0, k0);
     if ((s0 & ~63u) < end) write_mover_bits(g, w, s0, m0);  // uniform over the wave
-    const bool m1 = a1 && fly_loaded(sg1, in, out, w, s1, p1, v1, q1, k1);
+    const bool m1 = a1 && fly_loaded(sg1, live1, in, out, w, s1, p1, v1, q1, k1);
     if ((s1 & ~63u) < end) write_mover_bits(g, w, s1, m1);
 }
 

@@ -68,9 +68,8 @@ struct fh_flock {
     hipGraphExec_t lgraph_run[2][2][2]{};  // GRAPH_RUNS[i] lazy steps from phase 0, layout parity, counter parity
 };
 
-// Steps per rebuild with FH_FLAG_LAZY. A divisor of every GRAPH_RUNS entry,
-// so that a graph run starts and ends on a rebuild.
-constexpr uint32_t LAZY_PERIOD = 4;
+// LAZY_PERIOD (steps per rebuild with FH_FLAG_LAZY) divides every GRAPH_RUNS
+// entry, so that a graph run starts and ends on a rebuild.
 
 // Events per timed step with per-pass timing: six on the stream between the
 // passes, two recorded by the decide launch at its kernel's start and end.
@@ -375,47 +374,24 @@ static fh_status resident_step_launches(fh_flock *h, hipEvent_t *ev) {
     return FH_OK;
 }
 
-// The mover lists of a step at `phase` of the period: it reads the lists the
-// step before wrote (heads of its phase; phase 0's stay empty), writes the
-// next phase's (not at the rebuild, the period's last step), and clears the
-// heads a later phase writes: the first step clears the last writer's, the
-// rebuild the others. List entries alternate between two sets (written at
-// phases 0 and 2, and 1).
+// The work arrays of a lazy step at `phase` of the period: the mover lists
+// (with_mover_lists) and the flock's double-buffered clock.
 static ResWork lazy_work(const fh_flock *h, uint32_t phase) {
-    ResWork w = h->W;
-    const uint32_t nh = h->mheads;
-    auto head = [&](uint32_t k) { return h->mhead4 + (size_t)k * nh; };
-    auto ovf = [&](uint32_t k) { return h->movf + (size_t)k * nh; };
-    const uint32_t rd = (phase + 1u) % 2u, wr = phase % 2u;
-    w.mr_head4 = phase ? head(phase) : nullptr;  // the first step after a rebuild has no movers to visit
-    w.mr_ovf = ovf(phase);
-    w.mr_next = h->mnext[rd];
-    w.mr_p4 = h->mp4[rd];
-    w.mr_v4 = h->mv4[rd];
-    const bool writes = phase + 1u < LAZY_PERIOD;
-    w.mw_head4 = writes ? head(phase + 1u) : nullptr;
-    w.mw_ovf = writes ? ovf(phase + 1u) : nullptr;
-    w.mw_next = h->mnext[wr];
-    w.mw_p4 = h->mp4[wr];
-    w.mw_v4 = h->mv4[wr];
-    w.mc_head4[0] = w.mc_head4[1] = nullptr;
-    w.mc_ovf[0] = w.mc_ovf[1] = nullptr;
-    if (phase == 0) {
-        w.mc_head4[0] = head(LAZY_PERIOD - 1u);
-        w.mc_ovf[0] = ovf(LAZY_PERIOD - 1u);
-    } else if (phase + 1u == LAZY_PERIOD) {
-        for (uint32_t k = 1; k + 1u < LAZY_PERIOD; ++k) {  // phases 1 and 2
-            w.mc_head4[k - 1u] = head(k);
-            w.mc_ovf[k - 1u] = ovf(k);
-        }
+    MoverLists m;
+    m.heads = h->mheads;
+    m.head4 = h->mhead4;
+    m.ovf = h->movf;
+    for (int k = 0; k < 2; ++k) {
+        m.p4[k] = h->mp4[k];
+        m.v4[k] = h->mv4[k];
+        m.next[k] = h->mnext[k];
     }
-    w.mmask = nh - 1u;
+    ResWork w = with_mover_lists(h->W, m, phase);
     w.mclock_now = h->mclock + h->cpar;
     w.mclock_next = h->mclock + (1u - h->cpar);
     w.mtick = h->d_seg;
     return w;
 }
-static_assert(LAZY_PERIOD == 4, "lazy_work clears two head arrays at the rebuild: phases 1 and 2");
 
 // One step with FH_FLAG_LAZY: decide (the walks visit the mover lists too)
 // with the fused flight from the current records into the other set. Between
@@ -461,6 +437,7 @@ static fh_status lazy_step_launches(fh_flock *h, hipEvent_t *ev) {
 // GRAPH_RUNS[i] steps as one graph where it can, longest first. Even, so that
 // a resident run ends at the buffer parity it started from.
 constexpr uint32_t GRAPH_RUNS[2] = {16, 4};
+static_assert(GRAPH_RUNS[0] % LAZY_PERIOD == 0 && GRAPH_RUNS[1] % LAZY_PERIOD == 0, "a lazy run holds whole periods");
 
 // Captures `body` (launches on h->stream) into *out, once.
 template <class Body>

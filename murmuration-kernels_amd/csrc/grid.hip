@@ -521,8 +521,9 @@ __global__ __launch_bounds__(256) void k_r_scan_apply(const uint32_t *__restrict
                                                       const uint32_t *__restrict__ tsum,
                                                       const uint32_t *__restrict__ tsum2, uint32_t ntiles,
                                                       uint32_t *__restrict__ noff, uint32_t *__restrict__ abase,
-                                                      uint32_t *__restrict__ occ, Seg *tick, uint32_t ntick) {
-    commit_clocks(tick, ntick, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
+                                                      uint32_t *__restrict__ occ, Seg *tick, uint32_t ntick,
+                                                      uint32_t tick_steps) {
+    commit_clocks(tick, ntick, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u, tick_steps);
     uint32_t part = 0;
     if (tsum2) {
         const uint32_t sup = blockIdx.x / RES_SUPER;
@@ -684,16 +685,16 @@ hipError_t launch_resident_flush_range(const Geometry &g, const Pools &p, const 
 }
 
 // Commits the clocks of a layout with no buckets (every flock left a batch).
-__global__ void k_commit(Seg *tick, uint32_t ntick) {
-    commit_clocks(tick, ntick, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+__global__ void k_commit(Seg *tick, uint32_t ntick, uint32_t tick_steps) {
+    commit_clocks(tick, ntick, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, tick_steps);
 }
 
 hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
                                    const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, Seg *tick,
-                                   uint32_t ntick, hipStream_t s) {
+                                   uint32_t ntick, hipStream_t s, uint32_t tick_steps) {
     const uint32_t ntiles = (g.n_buckets + RES_TILE - 1) / RES_TILE;
     if (ntiles == 0) {
-        if (tick && ntick) hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, s, tick, ntick);
+        if (tick && ntick) hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, s, tick, ntick, tick_steps);
         return hipGetLastError();
     }
     const uint32_t *tsum2 = nullptr;
@@ -703,7 +704,7 @@ hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Reco
         tsum2 = w.tsum2;
     }
     hipLaunchKernelGGL(k_r_scan_apply, dim3(ntiles), dim3(256), 0, s, w.counts, w.arr, w.tsum, tsum2, ntiles,
-                       off_nxt, w.abase, p.occ, tick, tick ? ntick : 0u);
+                       off_nxt, w.abase, p.occ, tick, tick ? ntick : 0u, tick_steps);
     if (g.n_slots)
         hipLaunchKernelGGL(k_r_move, dim3((g.n_slots + TILE - 1) / TILE), dim3(TILE), 0, s, g, p, cur, nxt, w,
                            off_cur, off_nxt);

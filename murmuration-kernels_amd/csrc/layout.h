@@ -60,6 +60,7 @@ struct Seg {
     uint32_t live;         // 0 for a freed segment
     uint32_t mode;         // SEG_* bits
     uint32_t left = 0xffffffffu;  // resident steps left (SEG_UNLIMITED: no budget); 0: frozen
+    uint32_t slot_base = 0;       // first slot of the flock's records in a resident layout
 };
 
 constexpr uint32_t SEG_SPATIAL = 1u;  // buckets by spatial_hash instead of the reference hash
@@ -148,9 +149,67 @@ struct ResWork {
     const float *mclock_now;
     float *mclock_next;
     Seg *mtick;
+    // A batch's lazy steps (mclock_now null): the steps taken since the
+    // segment table was last committed (the rebuild commits them all), from
+    // which each flock's clock and budget follow (seg_step).
+    uint32_t mphase;
 };
 constexpr uint32_t MOVER_NIL = 0xffffffffu;
 constexpr uint32_t MOVER_OVF = 0x80000000u;  // in a head's slot word: the list continues at ovf
+
+// Steps per rebuild with FH_FLAG_LAZY.
+constexpr uint32_t LAZY_PERIOD = 4;
+
+// The mover lists of one flock or one batch (FH_FLAG_LAZY): `heads` list
+// heads per phase (a power of two), LAZY_PERIOD arrays of them (phase 0's stay
+// empty), then LAZY_PERIOD arrays of overflow heads in the same region (one
+// fill of 5 * LAZY_PERIOD * heads words clears both), and two sets of list
+// entries by slot (position, velocity, next), steps alternating between them.
+struct MoverLists {
+    uint32_t heads = 0;
+    float4 *head4 = nullptr;
+    uint32_t *ovf = nullptr;
+    float4 *p4[2]{}, *v4[2]{};
+    uint32_t *next[2]{};
+};
+
+// The list fields of `w` for a step at `phase` of the period: it reads the
+// lists the step before wrote (heads of its phase; none at phase 0), writes
+// the next phase's (not at the rebuild, the period's last step), and clears
+// the heads a later phase writes: the first step clears the last writer's,
+// the rebuild the others. Entries alternate between the two sets (written at
+// phases 0 and 2, and 1).
+inline ResWork with_mover_lists(ResWork w, const MoverLists &m, uint32_t phase) {
+    static_assert(LAZY_PERIOD == 4, "the rebuild clears two head arrays: phases 1 and 2");
+    const uint32_t nh = m.heads;
+    auto head = [&](uint32_t k) { return m.head4 + (size_t)k * nh; };
+    auto ovf = [&](uint32_t k) { return m.ovf + (size_t)k * nh; };
+    const uint32_t rd = (phase + 1u) % 2u, wr = phase % 2u;
+    w.mr_head4 = phase ? head(phase) : nullptr;
+    w.mr_ovf = ovf(phase);
+    w.mr_next = m.next[rd];
+    w.mr_p4 = m.p4[rd];
+    w.mr_v4 = m.v4[rd];
+    const bool writes = phase + 1u < LAZY_PERIOD;
+    w.mw_head4 = writes ? head(phase + 1u) : nullptr;
+    w.mw_ovf = writes ? ovf(phase + 1u) : nullptr;
+    w.mw_next = m.next[wr];
+    w.mw_p4 = m.p4[wr];
+    w.mw_v4 = m.v4[wr];
+    w.mc_head4[0] = w.mc_head4[1] = nullptr;
+    w.mc_ovf[0] = w.mc_ovf[1] = nullptr;
+    if (phase == 0) {
+        w.mc_head4[0] = head(LAZY_PERIOD - 1u);
+        w.mc_ovf[0] = ovf(LAZY_PERIOD - 1u);
+    } else if (phase + 1u == LAZY_PERIOD) {
+        for (uint32_t k = 1; k + 1u < LAZY_PERIOD; ++k) {  // phases 1 and 2
+            w.mc_head4[k - 1u] = head(k);
+            w.mc_ovf[k - 1u] = ovf(k);
+        }
+    }
+    w.mmask = nh - 1u;
+    return w;
+}
 
 // Up to RES_DIRECT_TILES tiles, each workgroup of the rebuild scan adds up the
 // totals of the tiles before it itself (at most 8 loads per lane, one launch).
@@ -184,15 +243,35 @@ __device__ __forceinline__ uint32_t slot_segment(const Geometry &g, const Record
 // thread `lane` of `lanes` (a grid-stride split over the launch) commits
 // segments lane, lane + lanes, ..., so a batch's clocks take one round of
 // loads instead of one dependent round per flock.
-__device__ __forceinline__ void commit_clocks(Seg *segs, uint32_t nseg, uint32_t lane, uint32_t lanes) {
+__device__ __forceinline__ void commit_clocks(Seg *segs, uint32_t nseg, uint32_t lane, uint32_t lanes,
+                                              uint32_t steps = 1u) {
     for (uint32_t i = lane; i < nseg; i += lanes) {
         Seg &sg = segs[i];
-        const uint32_t left = sg.left;
-        if (sg.live && left) {
-            sg.time = sg.time + DT;
-            if (left != SEG_UNLIMITED) sg.left = left - 1u;
+        if (!sg.live) continue;
+        float t = sg.time;
+        uint32_t left = sg.left;
+        for (uint32_t k = 0; k < steps && left; ++k) {  // one f32 addition per step, as the host mirror
+            t = t + DT;
+            if (left != SEG_UNLIMITED) --left;
         }
+        sg.time = t;
+        sg.left = left;
     }
+}
+
+// A flock's clock and whether it steps, `k` steps after its segment entry was
+// last committed: the clock of k committed steps (one DT addition each, while
+// its budget lasts) and whether steps are left after them. With k = 0: the
+// entry's own clock, and whether the flock is not frozen.
+__device__ __forceinline__ bool seg_step(const Seg &sg, uint32_t k, float &time) {
+    float t = sg.time;
+    uint32_t left = sg.left;
+    for (uint32_t i = 0; i < k && left; ++i) {
+        t = t + DT;
+        if (left != SEG_UNLIMITED) --left;
+    }
+    time = t;
+    return left != 0u;
 }
 
 // Launchers (stream-ordered, no host synchronisation). Each returns the
@@ -281,9 +360,11 @@ hipError_t launch_resident_splice(const Geometry &g, const Pools &p, const Recor
 hipError_t launch_resident_tile_sums(const Geometry &g, const Pools &p, const ResWork &w, hipStream_t s);
 hipError_t launch_resident_integrate(const Geometry &g, const Pools &p, const Records &rec, const ResWork &w,
                                      hipStream_t s);
+// tick_steps: the steps each ticked clock and budget commits (the steps of a
+// lazy batch's period, or of its part before a settle; 1 otherwise).
 hipError_t launch_resident_rebuild(const Geometry &g, const Pools &p, const Records &cur, const Records &nxt,
                                    const ResWork &w, const uint32_t *off_cur, uint32_t *off_nxt, Seg *tick,
-                                   uint32_t ntick, hipStream_t s);
+                                   uint32_t ntick, hipStream_t s, uint32_t tick_steps = 1u);
 hipError_t launch_resident_flush(const Geometry &g, const Pools &p, const Records &rec, hipStream_t s);
 // The movers of the current records `cur` against the buckets of their slots,
 // counted as a fused flight counts them at a rebuild (arrival ranks, counts,

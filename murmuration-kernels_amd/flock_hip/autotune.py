@@ -5,8 +5,9 @@ workgroup size, the birds each decide workgroup scans per lane
 (birds_per_thread), the lanes that share one bird's walk (decide_group) and
 whether decide tests the bucket occupancy bitmap, runs as a persistent grid
 or maps chunks XCD by XCD, whether it stages each chunk's positions in LDS,
-whether cells are bucketed by the spatial hash and
-whether the state stays in bucket order between steps and flies each chunk inside decide, by timing
+whether cells are bucketed by the spatial hash,
+whether the state stays in bucket order between steps and flies each chunk inside decide, and whether
+that layout is rebuilt only every fourth step (lazy rebuilds), by timing
 short runs of the real workload. Each candidate is timed over `steps` steps
 after `warmup` untimed ones; the fastest few are timed again and the lowest
 median kept. Results can be cached in a JSON file keyed by
@@ -121,9 +122,8 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
         f = flags
         for name, bit in SWITCHES.items():
             f |= bit if prm.pop(name) else 0
-        if f & FLAG_LAZY and (kind == "batch" or not (f & FLAG_RESIDENT and f & FLAG_FLIGHT)
-                              or f & (FLAG_WINDOW | FLAG_UNFUSED)):
-            return float("inf")  # lazy rebuilds: single flocks, resident layout, fused flight, no window
+        if f & FLAG_LAZY and (not (f & FLAG_RESIDENT and f & FLAG_FLIGHT) or f & (FLAG_WINDOW | FLAG_UNFUSED)):
+            return float("inf")  # lazy rebuilds: resident layout, fused flight, no window
         if kind == "batch":
             return _time_batch(starts, times, prm, f, warmup, steps)
         return _time_plain(starts[0], times[0], prm, f, warmup, steps)
@@ -166,13 +166,15 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
             sweep([nv])
     # the resident layout keeps no canonical order
     resident = (0, 1) if not (flags & FLAG_CANONICAL) else (0,)
-    # the fused flight exists only in the resident layout, lazy rebuilds only
-    # with the fused flight in a single flock
-    lazy = resident if kind == "plain" else (0,)
-    for nv in (("resident", resident), ("spatial", spatial), ("cell", cells), ("bucket_shift", shifts),
-               ("occ", (0, 1)), ("persist", (0, 1)), ("xcd", (0, 1)), ("window", (0, 1)), ("flight", resident),
-               ("lazy", lazy), ("cell", cells)):
-        sweep([nv])
+    # The fused flight exists only in the resident layout, lazy rebuilds only
+    # with the fused flight and without the window: the three are swept
+    # together (a batch's fused flight alone is slower than the separate pass,
+    # so a flight sweep on its own would never reach lazy rebuilds).
+    for nvs in ((("resident", resident),), (("spatial", spatial),), (("cell", cells),), (("bucket_shift", shifts),),
+                (("occ", (0, 1)),), (("persist", (0, 1)),), (("xcd", (0, 1)),),
+                (("window", (0, 1)), ("flight", resident), ("lazy", resident)), (("cell", cells),),
+                (("bucket_shift", shifts),)):
+        sweep(list(nvs))
     if thorough:
         sweep(shape[:2])
     else:

@@ -3,13 +3,21 @@ every flock's state stays in bucket order between steps, as for one flock.
 Arrivals take their slots in atomic order, so results are compared with
 stepping alone by the decision clocks (exactly) and positions, velocities
 and neighbour counts (to rounding of reordered sums); the oracle agrees
-with a batched flock like with a flock stepped alone."""
+with a batched flock like with a flock stepped alone.
+
+With FH_FLAG_LAZY (and the fused flight) the layout is rebuilt every fourth
+step; in between the flocks' clocks and budgets follow from the segment
+table and the step's phase, and every membership, budget or download call
+first rebuilds the layout of the current positions. The same comparisons
+hold at every phase: the lazy flag sets below run through each test."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 RES, GRAPH, CANON = 64, 4, 1
+FLIGHT, LAZY = 2048, 4096
+LZ = RES | FLIGHT | LAZY
 
 
 def cube(fh, n, seed):
@@ -35,7 +43,7 @@ def close(got, want, tol=1e-4):
 
 
 @pytest.mark.parametrize("flags", [RES, RES | GRAPH, RES | GRAPH | 2048, RES | 128 | 512 | GRAPH,
-                                   RES | 256 | 1024, RES | 2048 | 128])
+                                   RES | 256 | 1024, RES | 2048 | 128, LZ, LZ | GRAPH | 128, LZ | 256 | 512])
 def test_resident_batch_equals_alone(fh, flags):
     sizes = [1000, 3000, 257, 20_000, 1, 64_000]
     starts = [cube(fh, n, 30 + i) for i, n in enumerate(sizes)]
@@ -49,10 +57,11 @@ def test_resident_batch_equals_alone(fh, flags):
             close(got, want)
 
 
-def test_resident_batch_matches_the_oracle(fh, fr):
+@pytest.mark.parametrize("flags", [RES | GRAPH | 2048, LZ | GRAPH | 128])
+def test_resident_batch_matches_the_oracle(fh, fr, flags):
     w0 = fh.gen_reference(2000, 1)
     other = cube(fh, 30_000, 2)
-    with fh.Batch(100_000, 1 << 21, 4, flags=RES | GRAPH | 2048) as b:
+    with fh.Batch(100_000, 1 << 21, 4, flags=flags) as b:
         b.add(other)
         fid = b.add(w0)
         b.step(10)
@@ -62,12 +71,13 @@ def test_resident_batch_matches_the_oracle(fh, fr):
     assert t == tc and d["ok"], d
 
 
-def test_resident_batch_join_and_leave(fh):
+@pytest.mark.parametrize("res_flags", [RES | GRAPH, LZ | GRAPH])
+def test_resident_batch_join_and_leave(fh, res_flags):
     # Adding and removing splice the flock into or out of the resident layout
     # in place; a download flushes that flock's slots only.
     a, c, d = cube(fh, 5000, 1), cube(fh, 9000, 2), cube(fh, 700, 3)
     out = {}
-    for name, flags in (("resident", RES | GRAPH), ("canonical", CANON)):
+    for name, flags in (("resident", res_flags), ("canonical", CANON)):
         with fh.Batch(50_000, 1 << 21, 8, flags=flags) as b:
             ia = b.add(a)
             b.step(5)
@@ -84,7 +94,7 @@ def test_resident_batch_join_and_leave(fh):
         close(g, w)
 
 
-@pytest.mark.parametrize("flags", [RES | GRAPH, RES | GRAPH | 2048 | 128, RES | 256 | 512])
+@pytest.mark.parametrize("flags", [RES | GRAPH, RES | GRAPH | 2048 | 128, RES | 256 | 512, LZ | GRAPH | 128])
 def test_resident_batch_splices_follow_the_canonical_batch(fh, flags):
     # Flocks join and leave a resident batch in place: at the end of the
     # bucket space, in the ranges of flocks that left, the first and a middle
@@ -114,7 +124,8 @@ def test_resident_batch_splices_follow_the_canonical_batch(fh, flags):
         close(g, w)
 
 
-def test_resident_batch_random_joins_and_leaves(fh):
+@pytest.mark.parametrize("res_flags", [RES | 128 | 2048, LZ | 128, LZ | GRAPH])
+def test_resident_batch_random_joins_and_leaves(fh, res_flags):
     # A seeded sequence of adds, removes, downloads and steps over up to 8
     # flocks of mixed sizes; the resident batch (splicing flocks in and out)
     # must keep every live flock equal to the canonical batch.
@@ -138,7 +149,7 @@ def test_resident_batch_random_joins_and_leaves(fh):
             ops.append(("step", int(rng.integers(1, 4))))
     ops.append(("step", 3))
     out = {}
-    for name, fl in (("resident", RES | 128 | 2048), ("canonical", CANON)):
+    for name, fl in (("resident", res_flags), ("canonical", CANON)):
         with fh.Batch(300_000, 1 << 22, 8, flags=fl) as b:
             ids, mids = {}, []
             for op, arg in ops:
@@ -157,9 +168,10 @@ def test_resident_batch_random_joins_and_leaves(fh):
         close(g, w)
 
 
-def test_resident_batch_clock_per_flock(fh):
+@pytest.mark.parametrize("flags", [RES, LZ])
+def test_resident_batch_clock_per_flock(fh, flags):
     w = cube(fh, 2000, 4)
-    with fh.Batch(10_000, 1 << 18, 4, flags=RES) as b:
+    with fh.Batch(10_000, 1 << 18, 4, flags=flags) as b:
         i0 = b.add(w, 0.0)
         i1 = b.add(w, 1.25)
         b.step(3)
@@ -171,16 +183,19 @@ def test_resident_batch_clock_per_flock(fh):
     close(got1, want1)
 
 
-def test_resident_batch_timing_and_bytes(fh):
+@pytest.mark.parametrize("flags", [RES, LZ])
+def test_resident_batch_timing_and_bytes(fh, flags):
     starts = [cube(fh, n, 40 + n) for n in (30_000, 50_000)]
-    with fh.Batch(100_000, 1 << 21, 4, flags=RES) as b:
+    with fh.Batch(100_000, 1 << 21, 4, flags=flags) as b:
         for w in starts:
             b.add(w)
         ms = b.step(4, 1, timing=True)
     assert ms["timed_steps"] == 3
-    assert ms["decide"] > 0 and ms["integrate"] > 0 and ms["grid_scatter"] > 0
+    assert ms["decide"] > 0 and ms["grid_scatter"] > 0
+    assert ms["integrate"] > 0 or flags & FLIGHT  # the fused flight runs inside decide
     assert 4_000 < ms["due_last"] < 12_000  # about a tenth of the 80,000 birds decide each step
     assert fh.flock_bytes(1000, flags=RES) > fh.flock_bytes(1000)
+    assert fh.flock_bytes(1000, flags=LZ) > fh.flock_bytes(1000, flags=RES | FLIGHT)
 
 
 def test_resident_batch_rejects_canonical(fh):
@@ -189,7 +204,7 @@ def test_resident_batch_rejects_canonical(fh):
     assert e.value.status == fh.ERR_INVALID
 
 
-@pytest.mark.parametrize("flags", [RES, RES | GRAPH, RES | GRAPH | 2048])
+@pytest.mark.parametrize("flags", [RES, RES | GRAPH, RES | GRAPH | 2048, LZ, LZ | GRAPH])
 def test_step_budgets_freeze_each_flock_on_its_last_step(fh, flags):
     # Budgets of 3, 7 and unlimited steps, then 10 steps of the batch: each
     # flock matches that many steps alone, and a frozen flock continues once
@@ -224,7 +239,7 @@ def test_step_budgets_need_the_resident_layout(fh):
         assert e.value.status == fh.ERR_INVALID
 
 
-@pytest.mark.parametrize("flags", [RES, RES | GRAPH | 2048])
+@pytest.mark.parametrize("flags", [RES, RES | GRAPH | 2048, LZ | GRAPH])
 def test_empty_resident_batch_steps_and_takes_new_flocks(fh, flags):
     # Every flock leaves a resident batch, the empty batch steps, and a new
     # flock joins it: that flock must follow stepping alone.
@@ -243,7 +258,7 @@ def test_empty_resident_batch_steps_and_takes_new_flocks(fh, flags):
     close(got, want)
 
 
-@pytest.mark.parametrize("flags", [RES | GRAPH, RES | GRAPH | 2048 | 128])
+@pytest.mark.parametrize("flags", [RES | GRAPH, RES | GRAPH | 2048 | 128, LZ | GRAPH | 128])
 def test_resident_scan_beyond_the_direct_tile_count(fh, fr, flags):
     # A resident layout of more than 2048 scan tiles (over 2M buckets) sums
     # the tile totals through super-tile totals; its flocks must follow the
@@ -263,3 +278,41 @@ def test_resident_scan_beyond_the_direct_tile_count(fh, fr, flags):
     cpu, tc = fr.step(small, 0.0, 6)
     d = fr.diff(cpu, out["resident"][2][0])
     assert out["resident"][2][1] == tc and d["ok"], d
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3, 4, 5, 7, 8, 9, 13])
+def test_lazy_batch_at_every_phase(fh, steps):
+    # Steps ending at every phase of the period (so downloads settle from
+    # each), in graph periods and single steps; flocks of the batch overlap in
+    # space, so a mover listed by another flock would show in the neighbour
+    # counts. Budgets end inside periods.
+    sizes = [40_000, 3000, 1, 257, 20_000, 90_000]
+    starts = [cube(fh, n, 80 + i) for i, n in enumerate(sizes)]
+    budgets = [None, 2, None, 5, 3, None]
+    with fh.Batch(200_000, 1 << 22, 8, flags=LZ | GRAPH | 128, decide_group=4) as b:
+        ids = [b.add(w) for w in starts]
+        for i, k in zip(ids, budgets):
+            if k is not None:
+                b.limit(i, k)
+        b.step(steps)
+        got = [b.download(i) for i in ids]
+    for (g, t), w, k in zip(got, starts, budgets):
+        want, tw = alone(fh, w, steps if k is None else min(k, steps), CANON)
+        assert t == tw
+        close(g, want)
+
+
+def test_lazy_batch_graph_periods_match_single_steps(fh):
+    # The same lazy batch stepped as graph periods and as single steps,
+    # split in several ways: equal clocks and decision times.
+    starts = [cube(fh, n, 90 + n) for n in (60_000, 7000, 25_000)]
+    out = {}
+    for name, flags, splits in (("graph", LZ | GRAPH, (8, 3, 5, 4)), ("eager", LZ, (1, 6, 2, 11))):
+        with fh.Batch(200_000, 1 << 22, 4, flags=flags) as b:
+            ids = [b.add(w) for w in starts]
+            for k in splits:
+                b.step(k)
+            out[name] = [b.download(i) for i in ids]
+    for (g, tg), (w, tw) in zip(out["graph"], out["eager"]):
+        assert tg == tw
+        close(g, w)

@@ -45,19 +45,22 @@ def test_each_result_equals_the_flock_alone(fh):
                 assert r["checksum"] == fh.state_checksum(want), rid
 
 
-@pytest.mark.parametrize("flags", [64 | 4 | 2048, 64, 64 | 4])
-def test_resident_served_results_match_alone(fh, fr, flags):
+@pytest.mark.parametrize("flags,spi", [(64 | 4 | 2048, 4), (64, 4), (64 | 4, 4), (64 | 4 | 2048 | 4096, 4),
+                                       (64 | 2048 | 4096, 6)])
+def test_resident_served_results_match_alone(fh, fr, flags, spi):
     # The served loop over a resident batch (arrivals take their slots in
     # atomic order; every flock has a step budget): each result has the clock
     # and decision times of the flock stepped alone, positions and velocities
     # within the reference's 1e-3 bar and 1e-4 relative, and the checksum of
     # the state the server returns. Two requests of 100k+ birds give the
-    # flight hundreds of workgroups, with or without the fused flight.
+    # flight hundreds of workgroups, with or without the fused flight. With
+    # lazy rebuilds (4096) an iteration of 4 steps is one period; of 6 steps,
+    # the flocks that leave mid-period are settled first.
     rng = np.random.default_rng(6)
     reqs = [(int(b), int(s), 200 + i) for i, (b, s) in
             enumerate(zip(rng.integers(2000, 30_000, 10), rng.integers(10, 50, 10)))]
     reqs += [(110_000, 13, 300), (140_000, 21, 301)]
-    with fh.Server(max_active=4, steps_per_iter=4, keep_states=1, bird_capacity=1 << 21,
+    with fh.Server(max_active=4, steps_per_iter=spi, keep_states=1, bird_capacity=1 << 21,
                    bucket_capacity=1 << 23, flags=flags) as srv:
         ids = [srv.submit(b, s, seed) for b, s, seed in reqs]
         srv.drain(300)
