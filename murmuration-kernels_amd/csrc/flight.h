@@ -25,9 +25,12 @@ __device__ __forceinline__ void integrate_bird(float4 &p4, float4 &v4, const flo
     const float beta = clampf_(roll + rate * DT, -MAX_BANK, MAX_BANK);
 
     // body_up (model.rs:34-38): (side x f) cos b + side sin b
+    // One range reduction for both (the device library's sincosf returns the
+    // bits of its sinf and cosf).
     const f3 up0 = cross3(side, fwd);
-    const float cb = cosf(beta);
-    const f3 up = up0 * cb + side * sinf(beta);
+    float sb, cb;
+    sincosf(beta, &sb, &cb);
+    const f3 up = up0 * cb + side * sb;
     const float cos_beta = fmaxf(cb, 0.2f);
     const float sr = speed / V0;
     const float lift = sr * sr * L0 / cos_beta;
