@@ -128,16 +128,16 @@ __device__ __forceinline__ bool bucket_occupied(const Pools &p, uint32_t b) {
 // Cells of the box are visited in z-y-x order (x fastest), as the nested
 // loops of cuda/flock.cu:209-211. A lane visits every G-th cell starting at
 // `first` (G = 1: every cell). Its cells are taken PF at a time: the bucket
-// ranges of all PF cells are loaded together, then the first candidate of
-// each non-empty bucket, so a lane has up to 2 * PF independent loads in
-// flight instead of a chain of dependent ones; within a bucket, candidates
-// are loaded AHEAD at a time. With `use_occ` the occupancy bits of the PF
-// buckets are read first (a small, cache-resident table) and empty buckets
-// never touch the offsets table. Within a lane the visit order is the walk
-// order.
+// ranges of all PF cells are loaded together, and their candidates then form
+// one run, loaded AHEAD at a time across bucket ends, so a wave waits one
+// memory latency per AHEAD candidates of its fullest lane's run instead of
+// once more for every bucket of every lane. With `use_occ` the occupancy bits
+// of the PF buckets are read first (a small, cache-resident table) and empty
+// buckets never touch the offsets table. Within a lane the visit order is the
+// walk order.
 //
-// With MG (FH_FLAG_LAZY) each cell's mover list is visited too, after the
-// cell's own bucket: `visit_m(t, d, d2)` sees list entry t. A mover's record
+// With MG (FH_FLAG_LAZY) the PF cells' mover lists are visited too, after
+// their buckets: `visit_m(t, d, d2)` sees list entry t. A mover's record
 // in its old bucket is never taken (its position lies in another cell), so
 // each bird is seen once.
 template <int G, int PF, int STRIDE, bool WIN, bool MG, class Visit, class VisitM>
@@ -145,6 +145,7 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
                                            uint32_t self, int32_t first, bool use_occ, Visit &&visit,
                                            const ResWork &mw, VisitM &&visit_m) {
     if (!w.ok) return;
+    constexpr int AHEAD = 4;  // 8 spills at the register budgets of G 2-8
     const int32_t nx = w.hi[0] - w.lo[0] + 1, ny = w.hi[1] - w.lo[1] + 1, nz = w.hi[2] - w.lo[2] + 1;
     const int32_t ncell = nx * ny * nz;
     int32_t x = w.lo[0], y = w.lo[1], z = w.lo[2];
@@ -200,42 +201,60 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
                 s0[k] = r.x;
                 s1[k] = r.y;
             }
-        float4 head[PF];
+        // The candidates of the PF buckets are taken as one run, in walk order,
+        // AHEAD loads at a time across bucket ends: a wave's rounds follow its
+        // lanes' total counts over the PF buckets (one memory latency each),
+        // not the fullest lane of every bucket in turn.
+        uint32_t fend[PF], foff[PF];
+        float fcx[PF], fcy[PF], fcz[PF];
+        uint32_t total = 0u;
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             s1[k] = min(s1[k], n_slots);
-            head[k] = s0[k] < s1[k] ? cand_pos<STRIDE, WIN>(cand, s0[k]) : make_float4(0.f, 0.f, 0.f, 0.f);
+            foff[k] = s0[k] - total;  // slot = run index + foff[k] within bucket k (wrapping)
+            total += s0[k] < s1[k] ? s1[k] - s0[k] : 0u;
+            fend[k] = total;
+            fcx[k] = (float)cx[k];
+            fcy[k] = (float)cy[k];
+            fcz[k] = (float)cz[k];
         }
-        // AHEAD candidates of a bucket are loaded before the first is tested, so
-        // one memory latency covers them (a walk is a chain of dependent
-        // loads; this is what bounds decide). Same candidates, same order.
-        constexpr int AHEAD = 4;
+        for (uint32_t j0 = 0; j0 < total; j0 += AHEAD) {
+            float4 c[AHEAD];
+            uint32_t tt[AHEAD];
 #pragma unroll
-        for (int k = 0; k < PF; ++k) {
-            for (uint32_t t0 = s0[k]; t0 < s1[k]; t0 += AHEAD) {
-                float4 c[AHEAD];
+            for (int u = 0; u < AHEAD; ++u) {
+                const uint32_t j = j0 + (uint32_t)u;
+                uint32_t o = foff[PF - 1];
 #pragma unroll
-                for (int u = 0; u < AHEAD; ++u) {
-                    const uint32_t t = t0 + (uint32_t)u;
-                    c[u] = u == 0 && t == s0[k] ? head[k]
-                           : (t < s1[k] ? cand_pos<STRIDE, WIN>(cand, t) : make_float4(0.f, 0.f, 0.f, 0.f));
-                }
-                // one predicate, no branch per test: inside the bucket, not
-                // the bird itself, in this cell, within 2r
-                const float fx = (float)cx[k], fy = (float)cy[k], fz = (float)cz[k];
-#pragma unroll
-                for (int u = 0; u < AHEAD; ++u) {
-                    const uint32_t t = t0 + (uint32_t)u;
-                    const f3 d = mk3(c[u].x - w.p.x, c[u].y - w.p.y, c[u].z - w.p.z);
-                    const float d2 = dot3(d, d);
-                    if ((int)(t < s1[k]) & (int)(t != self) & (int)in_cell(c[u], sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2))
-                        visit(t, d, d2);
-                }
+                for (int k = PF - 2; k >= 0; --k) o = j < fend[k] ? foff[k] : o;
+                tt[u] = j + o;
+                c[u] = j < total ? cand_pos<STRIDE, WIN>(cand, tt[u]) : make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            if constexpr (MG) {
+#pragma unroll
+            for (int u = 0; u < AHEAD; ++u) {
+                const uint32_t j = j0 + (uint32_t)u;
+                float fx = fcx[PF - 1], fy = fcy[PF - 1], fz = fcz[PF - 1];
+#pragma unroll
+                for (int k = PF - 2; k >= 0; --k)
+                    if (j < fend[k]) {
+                        fx = fcx[k];
+                        fy = fcy[k];
+                        fz = fcz[k];
+                    }
+                const f3 d = mk3(c[u].x - w.p.x, c[u].y - w.p.y, c[u].z - w.p.z);
+                const float d2 = dot3(d, d);
+                // one predicate, no branch per test: inside the run, not the
+                // bird itself, in this cell, within 2r
+                if ((int)(j < total) & (int)(tt[u] != self) & (int)in_cell(c[u], sg.inv_cell, fx, fy, fz) &
+                    (int)(d2 <= w.c2))
+                    visit(tt[u], d, d2);
+            }
+        }
+        if constexpr (MG) {
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
                 const uint32_t hw = __float_as_uint(mh[k].w);
                 if (hw != MOVER_NIL) {
-                    const float fx = (float)cx[k], fy = (float)cy[k], fz = (float)cz[k];
                     uint32_t t = hw & ~MOVER_OVF;  // the head's bird, then the overflow chain
                     float4 c = mh[k];
                     uint32_t next = (hw & MOVER_OVF) ? mw.mr_ovf[bk[k] & mw.mmask] : MOVER_NIL;
@@ -245,8 +264,8 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
                     for (uint32_t it = 0; t < n_slots && it < n_slots; ++it) {
                         const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
                         const float d2 = dot3(d, d);
-                        if ((int)(t != self) & (int)in_cell(c, sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2) &
-                            (int)(t - sg.slot_base < sg.n))
+                        if ((int)(t != self) & (int)in_cell(c, sg.inv_cell, fcx[k], fcy[k], fcz[k]) &
+                            (int)(d2 <= w.c2) & (int)(t - sg.slot_base < sg.n))
                             visit_m(t, d, d2);
                         t = next;
                         if (t >= n_slots) break;
