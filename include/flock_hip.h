@@ -208,8 +208,8 @@ fh_status fh_download_words(fh_flock *h, float *words, float *time);
 fh_status fh_hash_count(fh_flock *h);
 /* grid_scan_blocks/totals/add (cuda/flock.cu:144-172): offsets[B+1]. */
 fh_status fh_scan(fh_flock *h);
-/* grid_scatter (cuda/flock.cu:175-179) into the sorted position/velocity
- * buffers that decide walks. */
+/* grid_scatter (cuda/flock.cu:175-179) into the sorted position/heading
+ * records that decide walks. */
 fh_status fh_scatter(fh_flock *h);
 /* decide (cuda/flock.cu:190-290) for birds with time >= next_decision. */
 fh_status fh_decide(fh_flock *h, float time);

@@ -402,8 +402,10 @@ __device__ __forceinline__ void acc_zero(Acc &a) {
 // square root, reciprocal, exponential and reciprocal square root (1 ulp
 // each) instead of a correctly rounded square root and divisions: the unit
 // offset is d times the reciprocal of the distance, the separation weight
-// the hardware exponential, a neighbour's heading its velocity times the
-// reciprocal square root of its length. Neighbour sets and counts are
+// the hardware exponential, and (resident records, which hold velocities) a
+// neighbour's heading its velocity times the reciprocal square root of its
+// length; the sorted copies hold each bird's heading, computed exactly once
+// per step by the scatter. Neighbour sets and counts are
 // unchanged (they follow d^2 alone, computed as the oracle does); forces move
 // by a few roundings, far inside the reference's bars. Measured on MI355X
 // against correctly rounded arithmetic: the 1M-bird step 6.8% faster, the
@@ -423,8 +425,9 @@ __device__ __forceinline__ void accumulate_2r(Acc &a, f3 d, float d2, f3 &dhat, 
 }
 
 // walk 1 (model.rs:67-93) for a candidate within r; `velocity()` gives its
-// velocity (read only for an aligned neighbour).
-template <class Velocity>
+// velocity (read only for an aligned neighbour), or with HEAD its heading
+// forward_of(velocity) as the sorted copies hold it (exact, no per-hit root).
+template <bool HEAD, class Velocity>
 __device__ __forceinline__ void accumulate_r(Acc &a, f3 f_i, f3 dhat, float dist, Velocity &&velocity) {
     ++a.n_all;
     const float g = dist <= R_H ? 1.0f : __expf(-((dist - R_H) * (dist - R_H)) * (1.0f / SEP_DENOM));
@@ -436,37 +439,43 @@ __device__ __forceinline__ void accumulate_r(Acc &a, f3 f_i, f3 dhat, float dist
     if (dot3(f_i, dhat) >= BLIND_COS) {
         const float4 vj = velocity();
         const f3 v = mk3(vj.x, vj.y, vj.z);
-        const float l2 = dot3(v, v);
-        a.align = a.align + (l2 > 1e-12f ? v * __builtin_amdgcn_rsqf(l2) : mk3(1.f, 0.f, 0.f));  // forward_of
+        if constexpr (HEAD) {
+            a.align = a.align + v;
+        } else {
+            const float l2 = dot3(v, v);
+            a.align = a.align + (l2 > 1e-12f ? v * __builtin_amdgcn_rsqf(l2) : mk3(1.f, 0.f, 0.f));  // forward_of
+        }
         ++a.n_align;
     }
 }
 
 // One candidate with offset d and d^2 <= 4r^2.
-template <class Velocity>
+template <bool HEAD, class Velocity>
 __device__ __forceinline__ void accumulate(Acc &a, f3 f_i, float r2, f3 d, float d2, Velocity &&velocity) {
     f3 dhat;
     float dist;
     accumulate_2r(a, d, d2, dhat, dist);
-    if (d2 <= r2) accumulate_r(a, f_i, dhat, dist, velocity);
+    if (d2 <= r2) accumulate_r<HEAD>(a, f_i, dhat, dist, velocity);
 }
 
 template <int G, int PF, int STRIDE, bool WIN, bool MG>
 __device__ __forceinline__ Acc walk_sums(const Pools &p, Cand cand, const Seg &sg, uint32_t n_slots, uint32_t self,
                                          const Walk &w, f3 f_i, int32_t first, bool use_occ, const ResWork &mw) {
+    constexpr bool HEAD = STRIDE == 2;  // sorted copies hold headings, resident records velocities
     Acc a;
     acc_zero(a);
     if constexpr (G <= 8)
         walk_cells<G, PF, STRIDE, WIN, MG>(
             w, sg, p, cand, n_slots, self, first, use_occ,
             [&](uint32_t t, f3 d, float d2) {
-                accumulate(a, f_i, w.r2, d, d2, [&] { return cand_vel<STRIDE, WIN>(cand, t); });
+                accumulate<HEAD>(a, f_i, w.r2, d, d2, [&] { return cand_vel<STRIDE, WIN>(cand, t); });
             },
-            mw, [&](uint32_t t, f3 d, float d2) { accumulate(a, f_i, w.r2, d, d2, [&] { return mw.mr_v4[t]; }); });
+            mw,
+            [&](uint32_t t, f3 d, float d2) { accumulate<false>(a, f_i, w.r2, d, d2, [&] { return mw.mr_v4[t]; }); });
     else
         walk_cells_split<G, PF, STRIDE, WIN, MG>(w, sg, p, cand, n_slots, self, (uint32_t)first, use_occ,
                                                  [&](f3 d, float d2, float4 vj) {
-                                                     accumulate(a, f_i, w.r2, d, d2, [&] { return vj; });
+                                                     accumulate<HEAD>(a, f_i, w.r2, d, d2, [&] { return vj; });
                                                  },
                                                  mw);
     return a;
@@ -675,7 +684,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
         if (active) {
             slot = due_slots[k];
             const float4 q = cand_pos<STRIDE, WIN>(cand, slot);
-            const float4 v = cand_vel<STRIDE, WIN>(cand, slot);  // sorted copies: w is the bird's radius
+            // sorted copies: the heading and the radius; resident records: the velocity
+            const float4 v = cand_vel<STRIDE, WIN>(cand, slot);
             me = RES ? rec.id[slot] : __float_as_uint(q.w);
             if (me >= g.n_birds) {
                 if (lane_g == 0) atomicOr(p.err, ERR_SLOT);
@@ -683,7 +693,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
             } else {
                 si = g.tile_seg ? g.tile_seg[me / TILE] : 0u;
                 pos = mk3(q.x, q.y, q.z);
-                f_i = forward_and_length(mk3(v.x, v.y, v.z), speed);
+                if constexpr (RES) {
+                    f_i = forward_and_length(mk3(v.x, v.y, v.z), speed);
+                } else {
+                    const float4 vb = p.vel4[me];
+                    f_i = mk3(v.x, v.y, v.z);  // forward_of(velocity), written by the scatter
+                    speed = sqrtf(dot3(mk3(vb.x, vb.y, vb.z), mk3(vb.x, vb.y, vb.z)));
+                }
                 r = RES ? rec.s4[slot].w : v.w;
                 w = make_walk(pos, r, g.segs[si]);
             }
@@ -754,7 +770,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
                                     const f3 d = mk3(c.x - w.p.x, c.y - w.p.y, c.z - w.p.z);
                                     const float d2 = dot3(d, d);
                                     if (d2 > w.c2) continue;
-                                    accumulate(a, f_i, w.r2, d, d2, [&] { return cand_vel<STRIDE, WIN>(cand, ts); });
+                                    accumulate<STRIDE == 2>(a, f_i, w.r2, d, d2,
+                                                            [&] { return cand_vel<STRIDE, WIN>(cand, ts); });
                                 }
                             }
                             __syncthreads();

@@ -14,9 +14,11 @@
 //   offsets[bucket]  exclusive scan, buckets + 1 entries
 //   occ[bucket / 32] bit (bucket % 32) set when the bucket holds a bird (scan)
 // and the scatter writes the sorted copies decide walks, in slot order, as one
-// 32-byte record per slot (a candidate's velocity shares its position's line):
+// 32-byte record per slot (a candidate's heading shares its position's line):
 //   sbird[2 slot]     = (px, py, pz, bird index bits)
-//   sbird[2 slot + 1] = (vx, vy, vz, radius)
+//   sbird[2 slot + 1] = (hx, hy, hz, radius), h = forward_of(velocity), the
+//                       heading an aligned neighbour adds (model.rs:88-90),
+//                       computed once per bird instead of once per hit
 //   snext[slot] = next_decision (or next_decision - time, see Seg::time).
 //
 // Several flocks can share one set of pools ("segments"): flock f owns birds
