@@ -223,12 +223,14 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
             uint32_t tt[AHEAD];
 #pragma unroll
             for (int u = 0; u < AHEAD; ++u) {
-                const uint32_t j = j0 + (uint32_t)u;
+                // past the run's end a lane re-reads its last candidate (every
+                // load in bounds, no branch); the predicate drops it
+                const uint32_t j = min(j0 + (uint32_t)u, total - 1u);
                 uint32_t o = foff[PF - 1];
 #pragma unroll
                 for (int k = PF - 2; k >= 0; --k) o = j < fend[k] ? foff[k] : o;
                 tt[u] = j + o;
-                c[u] = j < total ? cand_pos<STRIDE, WIN>(cand, tt[u]) : make_float4(0.f, 0.f, 0.f, 0.f);
+                c[u] = cand_pos<STRIDE, WIN>(cand, tt[u]);
             }
 #pragma unroll
             for (int u = 0; u < AHEAD; ++u) {
@@ -341,13 +343,11 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
                 float4 c[UNROLL], v[UNROLL];
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) {
-                    const uint32_t t = t0 + (uint32_t)u * G;
-                    c[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    v[u] = c[u];
-                    if (t < s1[k]) {
-                        c[u] = cand_pos<STRIDE, WIN>(cand, t);
-                        v[u] = cand_vel<STRIDE, WIN>(cand, t);
-                    }
+                    // past the bucket's end a lane re-reads its last candidate
+                    // (every load in bounds, no branch); the predicate drops it
+                    const uint32_t t = min(t0 + (uint32_t)u * G, s1[k] - 1u);
+                    c[u] = cand_pos<STRIDE, WIN>(cand, t);
+                    v[u] = cand_vel<STRIDE, WIN>(cand, t);
                 }
                 const float fx = (float)cx[k], fy = (float)cy[k], fz = (float)cz[k];
 #pragma unroll

@@ -136,6 +136,31 @@ def test_dense_cell_and_hash_collisions(fh, fr):
                 assert np.array_equal(got, want), (i, walk)
 
 
+@pytest.mark.parametrize("group", [16, 64])
+def test_split_walks_with_hash_collisions_match_brute_force(fh, fr, group):
+    # The split walks (lanes share a bucket's candidates) where most buckets
+    # hold birds of several cells: 64 buckets for a dense cell and a spread
+    # over hundreds of cells.
+    rng = np.random.default_rng(4)
+    w = fh.gen_reference(6000, 5)
+    w[:2500, 0:3] = (rng.random((2500, 3)) * 3.9 + np.array([0.05, 0.05, 100.05])).astype(np.float32)
+    w[:, 8] = np.float32(1.5)
+    cells = np.unique(np.floor(w[:, 0:3].astype(np.float64) / 4.0), axis=0)
+    assert len(cells) > 3 * 64  # three cells per bucket on average
+    t = np.float32(0.0) + DT
+    with fh.Flock.from_words(w, buckets=64, decide_group=group, decide_block=256) as f:
+        build_grid(f)
+        f.decide(t)
+        got, _ = f.download()
+    due = np.nonzero(t >= w[:, 12])[0]
+    assert len(due) > 500
+    for i in due:
+        d = fr.decide(w, int(i))
+        assert got[i, 13].view(np.uint32) == d["neighbours"], i
+        assert rel_err(got[i, 9:12], d["force"]) <= 1e-4, i
+        assert abs(got[i, 14] - d["centrality"]) <= 1e-5, i
+
+
 def ties_ok(w, i, walk, got, want):
     """Set difference allowed only for birds within 1e-5 of the radius."""
     r = float(w[i, 8]) * walk
