@@ -402,10 +402,10 @@ __device__ __forceinline__ void acc_zero(Acc &a) {
 // square root, reciprocal, exponential and reciprocal square root (1 ulp
 // each) instead of a correctly rounded square root and divisions: the unit
 // offset is d times the reciprocal of the distance, the separation weight
-// the hardware exponential, and (resident records, which hold velocities) a
-// neighbour's heading its velocity times the reciprocal square root of its
-// length; the sorted copies hold each bird's heading, computed exactly once
-// per step by the scatter. Neighbour sets and counts are
+// the hardware exponential, a neighbour's heading its velocity times the
+// reciprocal square root of its length (hit_heading: per aligned hit on
+// resident records, once per bird and step by the scatter for the sorted
+// copies, the same bits). Neighbour sets and counts are
 // unchanged (they follow d^2 alone, computed as the oracle does); forces move
 // by a few roundings, far inside the reference's bars. Measured on MI355X
 // against correctly rounded arithmetic: the 1M-bird step 6.8% faster, the
@@ -426,7 +426,7 @@ __device__ __forceinline__ void accumulate_2r(Acc &a, f3 d, float d2, f3 &dhat, 
 
 // walk 1 (model.rs:67-93) for a candidate within r; `velocity()` gives its
 // velocity (read only for an aligned neighbour), or with HEAD its heading
-// forward_of(velocity) as the sorted copies hold it (exact, no per-hit root).
+// hit_heading(velocity) as the sorted copies hold it (the same bits).
 template <bool HEAD, class Velocity>
 __device__ __forceinline__ void accumulate_r(Acc &a, f3 f_i, f3 dhat, float dist, Velocity &&velocity) {
     ++a.n_all;
@@ -439,12 +439,10 @@ __device__ __forceinline__ void accumulate_r(Acc &a, f3 f_i, f3 dhat, float dist
     if (dot3(f_i, dhat) >= BLIND_COS) {
         const float4 vj = velocity();
         const f3 v = mk3(vj.x, vj.y, vj.z);
-        if constexpr (HEAD) {
+        if constexpr (HEAD)
             a.align = a.align + v;
-        } else {
-            const float l2 = dot3(v, v);
-            a.align = a.align + (l2 > 1e-12f ? v * __builtin_amdgcn_rsqf(l2) : mk3(1.f, 0.f, 0.f));  // forward_of
-        }
+        else
+            a.align = a.align + hit_heading(v);
         ++a.n_align;
     }
 }
@@ -693,13 +691,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(decide_mi
             } else {
                 si = g.tile_seg ? g.tile_seg[me / TILE] : 0u;
                 pos = mk3(q.x, q.y, q.z);
-                if constexpr (RES) {
-                    f_i = forward_and_length(mk3(v.x, v.y, v.z), speed);
-                } else {
-                    const float4 vb = p.vel4[me];
-                    f_i = mk3(v.x, v.y, v.z);  // forward_of(velocity), written by the scatter
-                    speed = sqrtf(dot3(mk3(vb.x, vb.y, vb.z), mk3(vb.x, vb.y, vb.z)));
-                }
+                // the bird's own heading is exact (forward_of); sorted copies
+                // hold the hit heading, so the velocity comes from vel4
+                const float4 vb = RES ? v : p.vel4[me];
+                f_i = forward_and_length(mk3(vb.x, vb.y, vb.z), speed);
                 r = RES ? rec.s4[slot].w : v.w;
                 w = make_walk(pos, r, g.segs[si]);
             }

@@ -236,7 +236,7 @@ __global__ __launch_bounds__(TILE) void k_scatter(Geometry g, Pools p, int canon
     }
     const float4 q = p.pos4[bird];
     const float4 v = p.vel4[bird];
-    const f3 h = forward_of(mk3(v.x, v.y, v.z));
+    const f3 h = hit_heading(mk3(v.x, v.y, v.z));
     p.sbird[2u * s] = make_float4(q.x, q.y, q.z, __uint_as_float(bird));
     p.sbird[2u * s + 1u] = make_float4(h.x, h.y, h.z, p.steer4[bird].w);
     p.snext[s] = relative ? q.w - sg.time : q.w;
@@ -249,7 +249,7 @@ __device__ __forceinline__ void write_sorted(const Geometry &g, const Pools &p, 
     const float4 v = p.vel4[bird];
     float t = 0.0f;
     if (relative) t = g.segs[g.tile_seg ? g.tile_seg[bird / TILE] : 0u].time;
-    const f3 h = forward_of(mk3(v.x, v.y, v.z));
+    const f3 h = hit_heading(mk3(v.x, v.y, v.z));
     p.sbird[2u * c] = make_float4(q.x, q.y, q.z, __uint_as_float(bird));
     p.sbird[2u * c + 1u] = make_float4(h.x, h.y, h.z, p.steer4[bird].w);
     p.snext[c] = relative ? q.w - t : q.w;

@@ -16,7 +16,7 @@
 // and the scatter writes the sorted copies decide walks, in slot order, as one
 // 32-byte record per slot (a candidate's heading shares its position's line):
 //   sbird[2 slot]     = (px, py, pz, bird index bits)
-//   sbird[2 slot + 1] = (hx, hy, hz, radius), h = forward_of(velocity), the
+//   sbird[2 slot + 1] = (hx, hy, hz, radius), h = hit_heading(velocity), the
 //                       heading an aligned neighbour adds (model.rs:88-90),
 //                       computed once per bird instead of once per hit
 //   snext[slot] = next_decision (or next_decision - time, see Seg::time).
@@ -91,6 +91,16 @@ struct Pools {
     uint32_t *big_count;   // entries of `big` this step (zeroed by the scan)
     uint32_t big_cap;      // capacity of `big`: big_list_cap(bird capacity)
 };
+
+// forward_of(v) (model.rs:15-20) as a walk's hit sums it: v times the
+// hardware reciprocal square root of v.v (1 ulp) instead of a correctly
+// rounded root and division. The resident walks compute it per aligned hit
+// from the neighbour's velocity; the scatter stores it in the sorted copies,
+// so both paths add the same bits.
+__device__ __forceinline__ f3 hit_heading(f3 v) {
+    const float l2 = dot3(v, v);
+    return l2 > 1e-12f ? v * __builtin_amdgcn_rsqf(l2) : mk3(1.f, 0.f, 0.f);
+}
 
 // [offsets[b], offsets[b + 1]) with one 8-byte load: the two words share a
 // cache line (except when b is the last word of one), so one request fetches
