@@ -9,6 +9,7 @@ namespace fh {
 
 // Banked flight of one bird (model.rs:149-176): new (pos, next), (vel, roll)
 // and the roll rate, from its (pos, next), (vel, roll) and (steering, radius).
+// Divisions by the model constants take div_model_const (the same bits).
 __device__ __forceinline__ void integrate_bird(float4 &p4, float4 &v4, const float4 s4, float &rate_out) {
     f3 vel = mk3(v4.x, v4.y, v4.z);
     const float roll = v4.w;
@@ -19,9 +20,9 @@ __device__ __forceinline__ void integrate_bird(float4 &p4, float4 &v4, const flo
     const f3 side = side_of(fwd);
     const float speed = fmaxf(len, 1e-6f);
 
-    const float a_lat = dot3(steering, side) / MASS;
-    const float target = clampf_(atanf(a_lat / G), -MAX_BANK, MAX_BANK);
-    const float rate = clampf_((target - roll) / TAU_ROLL, -MAX_ROLL_RATE, MAX_ROLL_RATE);
+    const float a_lat = div_model_const(dot3(steering, side), MASS, 1.0f / MASS);
+    const float target = clampf_(atanf(div_model_const(a_lat, G, 1.0f / G)), -MAX_BANK, MAX_BANK);
+    const float rate = clampf_(div_model_const(target - roll, TAU_ROLL, 1.0f / TAU_ROLL), -MAX_ROLL_RATE, MAX_ROLL_RATE);
     const float beta = clampf_(roll + rate * DT, -MAX_BANK, MAX_BANK);
 
     // body_up (model.rs:34-38): (side x f) cos b + side sin b
@@ -32,13 +33,13 @@ __device__ __forceinline__ void integrate_bird(float4 &p4, float4 &v4, const flo
     sincosf(beta, &sb, &cb);
     const f3 up = up0 * cb + side * sb;
     const float cos_beta = fmaxf(cb, 0.2f);
-    const float sr = speed / V0;
+    const float sr = div_model_const(speed, V0, 1.0f / V0);
     const float lift = sr * sr * L0 / cos_beta;
     const float drag = sr * sr * D0;
     const f3 flight = ((up * lift - fwd * drag) + fwd * D0) - (mk3(0.f, 0.f, 1.f) * MASS) * G;
 
     const f3 control = steering - side * dot3(steering, side);
-    vel = vel + ((control + flight) / MASS) * DT;
+    vel = vel + div_model_const(control + flight, MASS, 1.0f / MASS) * DT;
     const float sp = sqrtf(dot3(vel, vel));
     if (sp > V_MAX)
         vel = vel * (V_MAX / sp);

@@ -66,6 +66,27 @@ FH_HD f3 cross3(f3 a, f3 b) {
 }
 FH_HD float clampf_(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
 
+// x / c, correctly rounded, for c one of the model constants the flight
+// divides by (MASS, G, TAU_ROLL, V0), with rc = 1 / c rounded: the product
+// q = x rc, then one correction q + (x - q c) rc, both steps fused
+// multiply-adds (the remainder x - q c is exact). Compared with the division
+// for every float x with 2^-104 <= |x| <= 2^100 and each of the four
+// constants, not one bit differs (tests/div_check.cpp, exhaustive mode).
+// Other x divide: zero (the correction would lose the sign of -0), subnormal
+// and tiny values, huge values, infinities and NaN.
+FH_HD float div_model_const(float x, float c, float rc) {
+    float q = x * rc;
+    const float ax = fabsf(x);
+    if (ax >= 0x1p-104f && ax <= 0x1p100f)
+        q = fmaf(fmaf(-q, c, x), rc, q);
+    else
+        q = x / c;
+    return q;
+}
+FH_HD f3 div_model_const(f3 v, float c, float rc) {
+    return mk3(div_model_const(v.x, c, rc), div_model_const(v.y, c, rc), div_model_const(v.z, c, rc));
+}
+
 // glam normalize: v * (1 / |v|); forward/side_of fallbacks from model.rs:15-31.
 FH_HD f3 forward_of(f3 v) {
     float l2 = dot3(v, v);
