@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+BATCH_TUNE_STEPS = 100  # timed steps of a batched-leg tuning trial (the leg's default length)
 _T0 = time.perf_counter()
 
 
@@ -730,7 +731,10 @@ def batched_leg(args, plain_pt) -> dict:
     starts = workloads.mix(args.flocks, seed=args.seed)
     sizes = [int(w.shape[0]) for w in starts]
     total = sum(sizes)
-    tuned = autotune.tune(starts, "batch", flags=fh.FLAG_GRAPH, warmup=2, steps=5,
+    # trials time the leg's own window (3 warm-up + 100 steps, whatever
+    # --batch-steps says): the radii grow over the steps, and a short window
+    # picks cells that lose over the leg (cell 6 against 10 m: 0.78 against 0.68 ms)
+    tuned = autotune.tune(starts, "batch", flags=fh.FLAG_GRAPH, warmup=3, steps=BATCH_TUNE_STEPS,
                           progress=log, thorough=False, **tune_cache(args, f"mix-{args.flocks}-seed{args.seed}"))
     prm, flags = tuned["params"], tuned["flags"]
     buckets = sum(autotune._buckets(s) for s in sizes)

@@ -175,10 +175,13 @@ def tune(starts, kind: str = "batch", flags: int = FLAG_GRAPH, warmup: int = 3, 
                 (("window", (0, 1)), ("flight", resident), ("lazy", resident)), (("cell", cells),),
                 (("bucket_shift", shifts),)):
         sweep(list(nvs))
+    # the first shape sweep ran on the default layout: the shape is searched
+    # again on the chosen one (resident lazy batches prefer other blocks)
     if thorough:
         sweep(shape[:2])
     else:
-        sweep([shape[0]])
+        for nv in shape:
+            sweep([nv])
     # Short trials are noisy, so a search can settle a few percent away from
     # the best point: the fastest FINALISTS distinct points are timed
     # CONFIRM times more on fresh flocks and the lowest median wins.
