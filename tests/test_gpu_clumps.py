@@ -21,7 +21,8 @@ def rel_err(a, b):
 
 
 @pytest.mark.parametrize("n,cell,flags,group", [(1_000_000, 4.0, 0, 4), (1_000_000, 2.0, 8, 4),
-                                               (1_000_000, 1.0, 32, 16), (1_000_000, 2.0, 0, 64)])
+                                               (1_000_000, 1.0, 32, 16), (1_000_000, 2.0, 0, 64),
+                                               (1_000_000, 2.0, 0, 2)])
 def test_clumped_decide_matches_the_oracle(fh, fr, n, cell, flags, group):
     w = fh.gen_clumps(n, 8, 5.0, 1000.0, 3)
     t = np.float32(0.0) + DT
