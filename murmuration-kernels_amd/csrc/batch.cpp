@@ -292,8 +292,7 @@ static fh_status batch_alloc_resident(fh_batch *b) {
     const size_t o_shift = a.take<int32_t>(b->flock_cap ? b->flock_cap : 1);
     // mover lists (FH_FLAG_LAZY): about one list head per 8 birds, per phase
     const bool lazy = b->lazy;
-    uint32_t mb = 4096;
-    while (lazy && mb < nn / 8 && mb < (1u << 20)) mb <<= 1;
+    const uint32_t mb = lazy ? mover_heads(nn) : 4096u;
     const size_t o_mhead = lazy ? a.take<float4>((size_t)LAZY_PERIOD * mb) : 0;
     const size_t o_movf = lazy ? a.take<uint32_t>((size_t)LAZY_PERIOD * mb) : 0;  // right after: one fill clears both
     size_t o_mp4[2]{}, o_mv4[2]{}, o_mnext[2]{};
@@ -1444,8 +1443,7 @@ size_t fh_flock_bytes(uint32_t n, const fh_params *params) {
     if (prm.flags & FH_FLAG_RESIDENT)
         bytes += birds * resident_bytes_per_bird((prm.flags & FH_FLAG_FLIGHT) != 0) + bk * resident_bytes_per_bucket();
     if (prm.flags & FH_FLAG_LAZY) {  // two sets of mover list entries, and 4 arrays of list heads
-        size_t mb = 4096;
-        while (mb < birds / 8 && mb < (1u << 20)) mb <<= 1;
+        const size_t mb = mover_heads(birds);
         bytes += birds * 2 * 36 + mb * 4 * 20 + 64;
     }
     return bytes;

@@ -212,8 +212,7 @@ static fh_status alloc_resident(fh_flock *h) {
     const size_t o_tsum2 = a.take<uint32_t>(nb / RES_TILE / RES_SUPER + 1);
     // mover lists: about one list head per 8 birds, per phase
     const bool lazy = (h->prm.flags & FH_FLAG_LAZY) != 0;
-    uint32_t mb = 4096;
-    while (lazy && mb < nn / 8 && mb < (1u << 20)) mb <<= 1;
+    const uint32_t mb = lazy ? mover_heads(nn) : 4096u;
     const size_t o_mhead = lazy ? a.take<float4>((size_t)LAZY_PERIOD * mb) : 0;
     const size_t o_movf = lazy ? a.take<uint32_t>((size_t)LAZY_PERIOD * mb) : 0;  // right after: one fill clears both
     const size_t o_mclock = lazy ? a.take<float>(2) : 0;
