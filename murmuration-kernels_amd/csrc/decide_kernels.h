@@ -95,6 +95,29 @@ __device__ __forceinline__ bool in_cell(float4 c, float inv, float fx, float fy,
     return (int)(floorf(c.x * inv) == fx) & (int)(floorf(c.y * inv) == fy) & (int)(floorf(c.z * inv) == fz);
 }
 
+// Whether every cell of the walk's box widened by one cell per side falls in
+// a bucket of its own, so that the cell test can be left out: a candidate in
+// the bucket of a visited cell is then either in that cell or in a cell
+// outside the widened box, more than 2r away on some axis (the margin covers
+// the rounding of the cell coordinates, under 1/16 cell while they stay
+// below 2^20). It holds under the spatial hash when the widened box lies in
+// one block of 1024 cells per axis (the hash adds one constant for the block
+// to the Morton code of the low 10 bits) and spans at most 2^(b/3) cells per
+// axis, b the bits of the bucket mask: the masked Morton code keeps the low
+// b/3 bits of every axis, which differ for two cells closer than 2^(b/3).
+constexpr int32_t DISTINCT_CELL_MAX = 1 << 20;
+__device__ __forceinline__ bool box_buckets_distinct(const Walk &w, const Seg &sg) {
+    if (!(sg.mode & SEG_SPATIAL) || (sg.mask & (sg.mask + 1u))) return false;
+    const int32_t span = 1 << (__popc(sg.mask) / 3);
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int32_t lo = w.lo[a] - 1, hi = w.hi[a] + 1;  // |w.lo|, |w.hi| < 2^24 when w.ok
+        ok = ok && hi - lo < span && (lo >> 10) == (hi >> 10) && lo > -DISTINCT_CELL_MAX && hi < DISTINCT_CELL_MAX;
+    }
+    return ok;
+}
+
 __device__ __forceinline__ Walk make_walk(f3 p, float r, const Seg &sg) {
     Walk w;
     w.p = p;
@@ -140,7 +163,7 @@ __device__ __forceinline__ bool bucket_occupied(const Pools &p, uint32_t b) {
 // their buckets: `visit_m(t, d, d2)` sees list entry t. A mover's record
 // in its old bucket is never taken (its position lies in another cell), so
 // each bird is seen once.
-template <int G, int PF, int STRIDE, bool WIN, bool MG, class Visit, class VisitM>
+template <int G, int PF, int STRIDE, bool WIN, bool MG, bool CT = true, class Visit, class VisitM>
 __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const Pools &p, Cand cand, uint32_t n_slots,
                                            uint32_t self, int32_t first, bool use_occ, Visit &&visit,
                                            const ResWork &mw, VisitM &&visit_m) {
@@ -247,7 +270,7 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
                 const float d2 = dot3(d, d);
                 // one predicate, no branch per test: inside the run, not the
                 // bird itself, in this cell, within 2r
-                if ((int)(j < total) & (int)(tt[u] != self) & (int)in_cell(c[u], sg.inv_cell, fx, fy, fz) &
+                if ((int)(j < total) & (int)(tt[u] != self) & (int)(!CT || in_cell(c[u], sg.inv_cell, fx, fy, fz)) &
                     (int)(d2 <= w.c2))
                     visit(tt[u], d, d2);
             }
@@ -289,7 +312,7 @@ __device__ __forceinline__ void walk_cells(const Walk &w, const Seg &sg, const P
 // memory latency covers UNROLL candidates. `visit(d, d2, velocity)`.
 // With MG, each cell's mover list follows its own bucket, taken by the
 // group's first lane (lists hold a few birds at most).
-template <int G, int PF, int STRIDE, bool WIN, bool MG, class Visit>
+template <int G, int PF, int STRIDE, bool WIN, bool MG, bool CT, class Visit>
 __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, const Pools &p, Cand cand,
                                                  uint32_t n_slots, uint32_t self, uint32_t lane, bool use_occ,
                                                  Visit &&visit, const ResWork &mw) {
@@ -356,7 +379,8 @@ __device__ __forceinline__ void walk_cells_split(const Walk &w, const Seg &sg, c
                     const f3 d = mk3(c[u].x - w.p.x, c[u].y - w.p.y, c[u].z - w.p.z);
                     const float d2 = dot3(d, d);
                     // inside the bucket, not the bird itself, in this cell, within 2r
-                    if ((int)(t < s1[k]) & (int)(t != self) & (int)in_cell(c[u], sg.inv_cell, fx, fy, fz) & (int)(d2 <= w.c2))
+                    if ((int)(t < s1[k]) & (int)(t != self) & (int)(!CT || in_cell(c[u], sg.inv_cell, fx, fy, fz)) &
+                        (int)(d2 <= w.c2))
                         visit(d, d2, v[u]);
                 }
             }
@@ -462,20 +486,30 @@ __device__ __forceinline__ Acc walk_sums(const Pools &p, Cand cand, const Seg &s
     constexpr bool HEAD = STRIDE == 2;  // sorted copies hold headings, resident records velocities
     Acc a;
     acc_zero(a);
-    if constexpr (G <= 8)
-        walk_cells<G, PF, STRIDE, WIN, MG>(
-            w, sg, p, cand, n_slots, self, first, use_occ,
-            [&](uint32_t t, f3 d, float d2) {
-                accumulate<HEAD>(a, f_i, w.r2, d, d2, [&] { return cand_vel<STRIDE, WIN>(cand, t); });
-            },
-            mw,
-            [&](uint32_t t, f3 d, float d2) { accumulate<false>(a, f_i, w.r2, d, d2, [&] { return mw.mr_v4[t]; }); });
-    else
-        walk_cells_split<G, PF, STRIDE, WIN, MG>(w, sg, p, cand, n_slots, self, (uint32_t)first, use_occ,
-                                                 [&](f3 d, float d2, float4 vj) {
-                                                     accumulate<HEAD>(a, f_i, w.r2, d, d2, [&] { return vj; });
-                                                 },
-                                                 mw);
+    // Between lazy rebuilds a bucket's slots also hold birds that have left
+    // its cell (listed again where they are now): those walks keep the cell
+    // test. Others leave it out where the box's buckets are all distinct.
+    auto walk = [&](auto ct) {
+        constexpr bool CT = decltype(ct)::value;
+        if constexpr (G <= 8)
+            walk_cells<G, PF, STRIDE, WIN, MG, CT>(
+                w, sg, p, cand, n_slots, self, first, use_occ,
+                [&](uint32_t t, f3 d, float d2) {
+                    accumulate<HEAD>(a, f_i, w.r2, d, d2, [&] { return cand_vel<STRIDE, WIN>(cand, t); });
+                },
+                mw,
+                [&](uint32_t t, f3 d, float d2) { accumulate<false>(a, f_i, w.r2, d, d2, [&] { return mw.mr_v4[t]; }); });
+        else
+            walk_cells_split<G, PF, STRIDE, WIN, MG, CT>(w, sg, p, cand, n_slots, self, (uint32_t)first, use_occ,
+                                                         [&](f3 d, float d2, float4 vj) {
+                                                             accumulate<HEAD>(a, f_i, w.r2, d, d2, [&] { return vj; });
+                                                         },
+                                                         mw);
+    };
+    if (MG || !box_buckets_distinct(w, sg))
+        walk(std::true_type{});
+    else if constexpr (!MG)
+        walk(std::false_type{});
     return a;
 }
 

@@ -161,6 +161,44 @@ def test_split_walks_with_hash_collisions_match_brute_force(fh, fr, group):
         assert abs(got[i, 14] - d["centrality"]) <= 1e-5, i
 
 
+@pytest.mark.parametrize("group,buckets", [(2, 512), (4, 4096), (16, 512), (64, 4096)])
+def test_spatial_walks_without_cell_test_match_brute_force(fh, fr, group, buckets):
+    # Under the spatial hash a walk leaves out the cell test where the buckets
+    # of its box, widened by one cell per side, are all distinct: at most
+    # 2^(bits/3) cells per axis (8 with 512 buckets, 16 with 4096) inside one
+    # block of 1024 cells. Every bucket here holds birds of many cells, so a
+    # skip where the buckets are not distinct would change the sums. The flock
+    # sits away from the origin (cells -1 and 0 lie in different blocks).
+    rng = np.random.default_rng(5)
+    w = fh.gen_reference(20_000, 7)
+    w[:2500, 0:3] = (rng.random((2500, 3)) * 3.9 + np.array([200.05, 200.05, 100.05])).astype(np.float32)
+    w[2500:, 0:3] = (rng.random((17_500, 3)) * 120.0 + np.array([150.0, 150.0, 40.0])).astype(np.float32)
+    w[:, 8] = np.float32(1.5)
+    cell, flags = np.float32(4.0), 128  # FH_FLAG_SPATIAL
+    # the walks that skip the test, by the rule above (float32 cell coordinates)
+    inv = np.float32(1.0) / cell
+    rr = np.float32(2.0) * w[:, 8]
+    lo = np.floor((w[:, 0:3] - rr[:, None]) * inv).astype(np.int64) - 1
+    hi = np.floor((w[:, 0:3] + rr[:, None]) * inv).astype(np.int64) + 1
+    span = 1 << (int(np.log2(buckets)) // 3)
+    skip = np.all((hi - lo < span) & (lo >> 10 == hi >> 10), axis=1)
+    cells = np.unique(np.floor(w[:, 0:3].astype(np.float64) / 4.0), axis=0)
+    assert len(cells) > 3 * buckets  # three cells per bucket on average
+    t = np.float32(0.0) + DT
+    with fh.Flock.from_words(w, cell=float(cell), buckets=buckets, decide_group=group, decide_block=256,
+                             flags=flags) as f:
+        build_grid(f)
+        f.decide(t)
+        got, _ = f.download()
+    due = np.nonzero(t >= w[:, 12])[0]
+    assert len(due) > 1000 and skip[due].mean() > 0.5
+    for i in due:
+        d = fr.decide(w, int(i))
+        assert got[i, 13].view(np.uint32) == d["neighbours"], i
+        assert rel_err(got[i, 9:12], d["force"]) <= 1e-4, i
+        assert abs(got[i, 14] - d["centrality"]) <= 1e-5, i
+
+
 def ties_ok(w, i, walk, got, want):
     """Set difference allowed only for birds within 1e-5 of the radius."""
     r = float(w[i, 8]) * walk
